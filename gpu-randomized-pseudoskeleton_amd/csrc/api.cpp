@@ -1,0 +1,1525 @@
+// Host side of the C ABI: contexts, geometry upload, the per-block adaptive
+// loop (the reference's cuda_cur_compress3 driving cuda_complete_compression),
+// the batched driver and the drop-in entry point.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "farcur.h"
+#include "element.hpp"
+#include "geometry.hpp"
+#include "kernels.hpp"
+#include "philox.hpp"
+
+using namespace farcur;
+
+struct farcur_mesh {
+    Mesh m;
+    bool rwg = false;
+};
+struct farcur_partition {
+    BlockPartition p;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPTRY(expr)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return fail(FARCUR_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define TRY(expr)                   \
+    do {                            \
+        int rc_ = (expr);           \
+        if (rc_ != FARCUR_OK) return rc_; \
+    } while (0)
+
+// Grow-only device buffer.
+struct DBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap && p) return FARCUR_OK;
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(bytes, 256);
+        if (hipMalloc(&p, want) != hipSuccess) {
+            p = nullptr;
+            return fail(FARCUR_ENOMEM, "hipMalloc of %zu bytes failed", want);
+        }
+        cap = want;
+        return FARCUR_OK;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T>
+    T* as() const {
+        return (T*)p;
+    }
+};
+
+// Bump allocator over device chunks; reset() keeps the chunks.
+struct Arena {
+    std::vector<std::pair<char*, size_t>> chunks;
+    size_t cur = 0, off = 0;
+    size_t chunk_bytes = (size_t)1 << 30;
+    void reset() {
+        cur = 0;
+        off = 0;
+    }
+    int alloc(size_t bytes, void** out) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        while (true) {
+            if (cur < chunks.size() && off + bytes <= chunks[cur].second) {
+                *out = chunks[cur].first + off;
+                off += bytes;
+                return FARCUR_OK;
+            }
+            if (cur + 1 < chunks.size()) {
+                ++cur;
+                off = 0;
+                continue;
+            }
+            const size_t sz = std::max(chunk_bytes, bytes);
+            char* p = nullptr;
+            if (hipMalloc((void**)&p, sz) != hipSuccess) return fail(FARCUR_ENOMEM, "arena hipMalloc %zu failed", sz);
+            chunks.emplace_back(p, sz);
+            cur = chunks.size() - 1;
+            off = 0;
+        }
+    }
+    void release() {
+        for (auto& c : chunks) hipFree(c.first);
+        chunks.clear();
+        reset();
+    }
+};
+
+__global__ void k_set_ones(c128* y, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] = c128{1.0, 0.0};
+}
+
+void launch_ones(c128* y, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_set_ones, dim3(grid), dim3(256), 0, s, y, n);
+}
+
+inline int r_start_rule(int D) { return D < 10 ? 1 : D / 10; }
+
+// Per-kernel-class device timing by event pairs.
+struct KTimer {
+    static constexpr int kClasses = 4;  // sample, fill, pinv, residual
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+    size_t used = 0;
+    double ms[kClasses] = {0, 0, 0, 0};
+    int launches[kClasses] = {0, 0, 0, 0};
+    bool on = false;
+    hipEvent_t get() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            hipEventCreate(&e);
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void begin(int cls, hipStream_t s, hipEvent_t* a) {
+        if (!on) return;
+        *a = get();
+        hipEventRecord(*a, s);
+        (void)cls;
+    }
+    void end(int cls, hipStream_t s, hipEvent_t a) {
+        if (!on) return;
+        hipEvent_t b = get();
+        hipEventRecord(b, s);
+        marks.push_back({cls, {a, b}});
+    }
+    // After a stream sync: accumulate and recycle.
+    void collect() {
+        for (auto& mk : marks) {
+            float t = 0;
+            hipEventElapsedTime(&t, mk.second.first, mk.second.second);
+            ms[mk.first] += t;
+            launches[mk.first] += 1;
+        }
+        marks.clear();
+        used = 0;
+    }
+    void clear() {
+        for (int i = 0; i < kClasses; ++i) {
+            ms[i] = 0;
+            launches[i] = 0;
+        }
+    }
+    ~KTimer() {
+        for (auto e : pool) hipEventDestroy(e);
+    }
+};
+
+}  // namespace
+
+struct farcur_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double k = 2.0 * M_PI;
+    double eta = 120.0 * M_PI;
+    int ne = 0, nt = 0;
+    std::vector<int32_t> tri_of_rwg;  // 2 * ne: T+, T- (host, for accounting)
+    DBuf rec64, rec32;
+    uint64_t geom_hash = 0;
+    // single-block scratch
+    DBuf d_rows, d_cols, d_cs, d_rs, d_C, d_R, d_W, d_V, d_U, d_sigma, d_v, d_y0, d_y1, d_plan, d_out;
+    int last_m = 0, last_n = 0, last_r = 0, last_f32 = 0, last_y = 0;
+    bool has_result = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    int select() {
+        HIPTRY(hipSetDevice(device));
+        return FARCUR_OK;
+    }
+    FillParams fill_params(int f32) const {
+        FillParams fp;
+        fp.rec = f32 ? rec32.p : rec64.p;
+        fp.ne = ne;
+        fp.f32 = f32;
+        fp.k = k;
+        fp.eta = eta;
+        return fp;
+    }
+};
+
+namespace {
+
+// RWG records (kRecFields x ne SoA) from the reference tables, 0-based ids.
+void build_records(const std::vector<double>& vertex, const std::vector<int32_t>& topol,
+                   const std::vector<int32_t>& edges, const std::vector<double>& ln, std::vector<double>& rec) {
+    const int ne = (int)ln.size();
+    rec.assign((size_t)kRecFields * ne, 0.0);
+    const double b[3][3] = {{2.0 / 3, 1.0 / 6, 1.0 / 6}, {1.0 / 6, 2.0 / 3, 1.0 / 6}, {1.0 / 6, 1.0 / 6, 2.0 / 3}};
+    for (int e = 0; e < ne; ++e) {
+        for (int h = 0; h < 2; ++h) {
+            const int t = edges[4 * e + h];
+            const int vf = edges[4 * e + 2 + h];
+            const double s = h == 0 ? 1.0 : -1.0;
+            const double* V0 = &vertex[3 * topol[3 * t]];
+            const double* V1 = &vertex[3 * topol[3 * t + 1]];
+            const double* V2 = &vertex[3 * topol[3 * t + 2]];
+            for (int i = 0; i < 3; ++i) {
+                for (int c = 0; c < 3; ++c) {
+                    const double q = b[i][0] * V0[c] + b[i][1] * V1[c] + b[i][2] * V2[c];
+                    rec[(size_t)(h * 9 + i * 3 + c) * ne + e] = q;
+                    rec[(size_t)(18 + h * 9 + i * 3 + c) * ne + e] = s * (q - vertex[3 * vf + c]);
+                }
+            }
+        }
+        rec[(size_t)36 * ne + e] = ln[e];
+    }
+}
+
+uint64_t fnv1a(uint64_t h, const void* data, size_t n) {
+    const unsigned char* p = (const unsigned char*)data;
+    // 8 bytes at a time for speed; still position-sensitive.
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, p + i, 8);
+        h ^= w;
+        h *= 0x100000001B3ull;
+        h ^= h >> 29;
+    }
+    for (; i < n; ++i) {
+        h ^= p[i];
+        h *= 0x100000001B3ull;
+    }
+    return h;
+}
+
+int set_geometry_impl(farcur_ctx* ctx, const std::vector<double>& vertex, const std::vector<int32_t>& topol,
+                      const std::vector<int32_t>& edges, const std::vector<double>& ln) {
+    const int nv = (int)(vertex.size() / 3), nt = (int)(topol.size() / 3), ne = (int)ln.size();
+    if (nv <= 0 || nt <= 0 || ne <= 0) return fail(FARCUR_EINVAL, "empty geometry (nv=%d nt=%d ne=%d)", nv, nt, ne);
+    for (size_t i = 0; i < topol.size(); ++i)
+        if (topol[i] < 0 || topol[i] >= nv) return fail(FARCUR_EINVAL, "topol entry %zu out of range", i);
+    for (int e = 0; e < ne; ++e) {
+        for (int h = 0; h < 2; ++h) {
+            if (edges[4 * e + h] < 0 || edges[4 * e + h] >= nt)
+                return fail(FARCUR_EINVAL, "edges(%d,%d) triangle out of range", h + 1, e + 1);
+            if (edges[4 * e + 2 + h] < 0 || edges[4 * e + 2 + h] >= nv)
+                return fail(FARCUR_EINVAL, "edges(%d,%d) vertex out of range", h + 3, e + 1);
+        }
+        if (!(ln[e] > 0) || !std::isfinite(ln[e])) return fail(FARCUR_EINVAL, "ln(%d) not positive finite", e + 1);
+    }
+    for (double x : vertex)
+        if (!std::isfinite(x)) return fail(FARCUR_EINVAL, "non-finite vertex coordinate");
+    std::vector<double> rec;
+    build_records(vertex, topol, edges, ln, rec);
+    std::vector<float> rec32(rec.begin(), rec.end());
+    TRY(ctx->select());
+    TRY(ctx->rec64.ensure(rec.size() * sizeof(double)));
+    TRY(ctx->rec32.ensure(rec32.size() * sizeof(float)));
+    HIPTRY(hipMemcpyAsync(ctx->rec64.p, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPTRY(hipMemcpyAsync(ctx->rec32.p, rec32.data(), rec32.size() * sizeof(float), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    ctx->ne = ne;
+    ctx->nt = nt;
+    ctx->tri_of_rwg.resize(2 * (size_t)ne);
+    for (int e = 0; e < ne; ++e) {
+        ctx->tri_of_rwg[2 * e] = edges[4 * e];
+        ctx->tri_of_rwg[2 * e + 1] = edges[4 * e + 1];
+    }
+    ctx->has_result = false;
+    return FARCUR_OK;
+}
+
+int check_ids(const int* ids, int n, int ne, const char* what) {
+    for (int i = 0; i < n; ++i)
+        if (ids[i] < 0 || ids[i] >= ne) return fail(FARCUR_EINVAL, "%s[%d] = %d out of range [0, %d)", what, i, ids[i], ne);
+    return FARCUR_OK;
+}
+
+// Panel element size.
+inline size_t psize(int f32) { return f32 ? sizeof(c64) : sizeof(c128); }
+
+// Runs the kernels of one round for the uploaded job list.
+struct RoundPlan {
+    int njobs = 0;
+    int f32 = 0;
+    bool pivot = false;
+    int fill_grid = 0;
+    int pinv_threads_global = 256;
+    int resid_threads = 256;
+    double atol = 1e-10, rtol = 0;
+    uint64_t seed = 0;
+    // device pointers
+    const BlockJob* jobs = nullptr;
+    const int32_t* prefix_all = nullptr;  // fill prefix: all panels (or C only when pivoting)
+    const int32_t* prefix_wr = nullptr;   // fill prefix: W + R (pivot second pass)
+    int total_all = 0, total_wr = 0;
+    const int32_t* list_lds = nullptr;
+    int nlds = 0;
+    const int32_t* list_glb = nullptr;
+    int nglb = 0;
+    JobOut* out = nullptr;
+};
+
+void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt) {
+    hipStream_t s = ctx->stream;
+    const FillParams fp = ctx->fill_params(P.f32);
+    hipEvent_t a;
+    kt.begin(0, s, &a);
+    launch_sample(P.jobs, P.njobs, P.seed, s);
+    kt.end(0, s, a);
+    if (P.pivot) {
+        kt.begin(1, s, &a);
+        launch_fill(P.jobs, P.njobs, P.prefix_all, P.total_all, kPanelC, fp, P.fill_grid, s);
+        kt.end(1, s, a);
+        kt.begin(0, s, &a);
+        launch_pivot_rows(P.jobs, P.njobs, P.f32, s);
+        kt.end(0, s, a);
+        kt.begin(1, s, &a);
+        launch_fill(P.jobs, P.njobs, P.prefix_wr, P.total_wr, kPanelW | kPanelR, fp, P.fill_grid, s);
+        kt.end(1, s, a);
+    } else {
+        kt.begin(1, s, &a);
+        launch_fill(P.jobs, P.njobs, P.prefix_all, P.total_all, kPanelC | kPanelW | kPanelR, fp, P.fill_grid, s);
+        kt.end(1, s, a);
+    }
+    kt.begin(2, s, &a);
+    launch_pinv(P.jobs, P.list_lds, P.nlds, P.atol, P.rtol, 1, 256, P.out, s);
+    launch_pinv(P.jobs, P.list_glb, P.nglb, P.atol, P.rtol, 0, P.pinv_threads_global, P.out, s);
+    kt.end(2, s, a);
+    kt.begin(3, s, &a);
+    launch_residual(P.jobs, P.njobs, P.f32, P.resid_threads, P.out, s);
+    kt.end(3, s, a);
+}
+
+// Host-side packing of a round: jobs, fill prefixes, pinv lists, into one
+// device upload.
+struct PackedRound {
+    std::vector<char> bytes;
+    size_t off_jobs = 0, off_pa = 0, off_pwr = 0, off_ll = 0, off_lg = 0, off_out = 0;
+    int total_all = 0, total_wr = 0, nlds = 0, nglb = 0;
+};
+
+void pack_round(const std::vector<BlockJob>& jobs, bool pivot, PackedRound& pk) {
+    const int nj = (int)jobs.size();
+    std::vector<int32_t> pa(nj + 1), pwr(nj + 1), ll, lg;
+    pa[0] = 0;
+    pwr[0] = 0;
+    for (int j = 0; j < nj; ++j) {
+        const BlockJob& J = jobs[j];
+        const int all = pivot ? fill_tiles(J.m, J.n, J.r, kPanelC) : fill_tiles(J.m, J.n, J.r, kPanelC | kPanelW | kPanelR);
+        pa[j + 1] = pa[j] + all;
+        pwr[j + 1] = pwr[j] + (pivot ? fill_tiles(J.m, J.n, J.r, kPanelW | kPanelR) : 0);
+        (J.r <= kPinvLdsMaxR ? ll : lg).push_back(j);
+    }
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t o = 0;
+    pk.off_jobs = o;
+    o = align(o + nj * sizeof(BlockJob));
+    pk.off_pa = o;
+    o = align(o + pa.size() * 4);
+    pk.off_pwr = o;
+    o = align(o + pwr.size() * 4);
+    pk.off_ll = o;
+    o = align(o + std::max<size_t>(ll.size(), 1) * 4);
+    pk.off_lg = o;
+    o = align(o + std::max<size_t>(lg.size(), 1) * 4);
+    pk.off_out = o;
+    o = align(o + nj * sizeof(JobOut));
+    pk.bytes.assign(o, 0);
+    std::memcpy(pk.bytes.data() + pk.off_jobs, jobs.data(), nj * sizeof(BlockJob));
+    std::memcpy(pk.bytes.data() + pk.off_pa, pa.data(), pa.size() * 4);
+    std::memcpy(pk.bytes.data() + pk.off_pwr, pwr.data(), pwr.size() * 4);
+    if (!ll.empty()) std::memcpy(pk.bytes.data() + pk.off_ll, ll.data(), ll.size() * 4);
+    if (!lg.empty()) std::memcpy(pk.bytes.data() + pk.off_lg, lg.data(), lg.size() * 4);
+    pk.total_all = pa[nj];
+    pk.total_wr = pwr[nj];
+    pk.nlds = (int)ll.size();
+    pk.nglb = (int)lg.size();
+}
+
+RoundPlan plan_from(const PackedRound& pk, void* dev, int njobs) {
+    char* d = (char*)dev;
+    RoundPlan P;
+    P.njobs = njobs;
+    P.jobs = (const BlockJob*)(d + pk.off_jobs);
+    P.prefix_all = (const int32_t*)(d + pk.off_pa);
+    P.prefix_wr = (const int32_t*)(d + pk.off_pwr);
+    P.total_all = pk.total_all;
+    P.total_wr = pk.total_wr;
+    P.list_lds = (const int32_t*)(d + pk.off_ll);
+    P.nlds = pk.nlds;
+    P.list_glb = (const int32_t*)(d + pk.off_lg);
+    P.nglb = pk.nglb;
+    P.out = (JobOut*)(d + pk.off_out);
+    return P;
+}
+
+int validate_opts(const farcur_opts* o) {
+    if (!o) return fail(FARCUR_EINVAL, "opts is NULL");
+    if (!(o->tol >= 0)) return fail(FARCUR_EINVAL, "tol must be >= 0");
+    if (!(o->pinv_tol >= 0) || !(o->pinv_rtol >= 0)) return fail(FARCUR_EINVAL, "pinv thresholds must be >= 0");
+    if (o->precision != FARCUR_F64 && o->precision != FARCUR_F32) return fail(FARCUR_EINVAL, "bad precision");
+    if (o->row_mode != FARCUR_ROWS_UNIFORM && o->row_mode != FARCUR_ROWS_PIVOT) return fail(FARCUR_EINVAL, "bad row_mode");
+    if (o->r_start < 0 || o->max_rank < 0) return fail(FARCUR_EINVAL, "r_start and max_rank must be >= 0");
+    return FARCUR_OK;
+}
+
+// Next sample count after a round, following cuda_cur_compress3
+// (compres_single_block_spheres_for_sweep2.jl:910-914 after the first call,
+// :955-967 inside the loop). Returns false when the loop stops at the cap.
+bool next_r(int r, int cap, int round, int* rn) {
+    if (round == 0) {
+        *rn = (2 * r < cap) ? 2 * r : cap;
+        return true;
+    }
+    if (r >= cap) return false;
+    *rn = (2 * r <= cap) ? 2 * r : cap;
+    return true;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char* farcur_last_error(void) { return g_err.c_str(); }
+const char* farcur_version(void) { return "farcur 0.1 (gfx950)"; }
+
+void farcur_opts_default(farcur_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->tol = 1e-4;
+    o->pinv_tol = 1e-10;
+    o->pinv_rtol = 0.0;
+    o->seed = 12345;
+    o->precision = FARCUR_F64;
+    o->row_mode = FARCUR_ROWS_UNIFORM;
+}
+
+// ------------------------------------------------------------------ meshes
+int farcur_mesh_plate(double lx, double ly, int nx, int ny, double x0, double y0, double z0, farcur_mesh** out) {
+    if (!out || nx <= 0 || ny <= 0 || !(lx > 0) || !(ly > 0)) return fail(FARCUR_EINVAL, "bad plate arguments");
+    *out = new farcur_mesh{mesh_plate(lx, ly, nx, ny, x0, y0, z0)};
+    return FARCUR_OK;
+}
+
+int farcur_mesh_octasphere(double radius, int n_levels, farcur_mesh** out) {
+    if (!out || n_levels < 0 || n_levels > 8 || !(radius > 0)) return fail(FARCUR_EINVAL, "bad octasphere arguments");
+    *out = new farcur_mesh{mesh_octasphere(radius, n_levels)};
+    return FARCUR_OK;
+}
+
+int farcur_mesh_icosphere(double ax, double ay, double az, int level, double jitter, uint64_t seed,
+                          farcur_mesh** out) {
+    if (!out || level < 0 || level > 8 || !(ax > 0) || !(ay > 0) || !(az > 0) || !(jitter >= 0) || jitter >= 0.5)
+        return fail(FARCUR_EINVAL, "bad icosphere arguments");
+    *out = new farcur_mesh{mesh_icosphere(ax, ay, az, level, jitter, seed)};
+    return FARCUR_OK;
+}
+
+int farcur_mesh_append(farcur_mesh* dst, const farcur_mesh* src, double dx, double dy, double dz) {
+    if (!dst || !src) return fail(FARCUR_EINVAL, "NULL mesh");
+    mesh_append(dst->m, src->m, dx, dy, dz);
+    dst->rwg = false;
+    return FARCUR_OK;
+}
+
+int farcur_mesh_build_rwg(farcur_mesh* mesh) {
+    if (!mesh) return fail(FARCUR_EINVAL, "NULL mesh");
+    std::string err;
+    if (!build_rwg(mesh->m, err)) return fail(FARCUR_EINVAL, "%s", err.c_str());
+    mesh->rwg = true;
+    return FARCUR_OK;
+}
+
+int farcur_mesh_sizes(const farcur_mesh* mesh, int* nv, int* nt, int* ne) {
+    if (!mesh) return fail(FARCUR_EINVAL, "NULL mesh");
+    if (nv) *nv = mesh->m.nv();
+    if (nt) *nt = mesh->m.nt();
+    if (ne) *ne = mesh->rwg ? mesh->m.ne() : 0;
+    return FARCUR_OK;
+}
+
+int farcur_mesh_tables(const farcur_mesh* mesh, double* vertex, double* topol, double* trian, double* edges,
+                       double* un, double* ds, double* ln, double* cent) {
+    if (!mesh) return fail(FARCUR_EINVAL, "NULL mesh");
+    const Mesh& m = mesh->m;
+    if (vertex) std::copy(m.vertex.begin(), m.vertex.end(), vertex);
+    if (topol)
+        for (size_t i = 0; i < m.topol.size(); ++i) topol[i] = m.topol[i] + 1;
+    if ((trian || edges || un || ds || ln || cent) && !mesh->rwg)
+        return fail(FARCUR_ESTATE, "RWG tables not built (call farcur_mesh_build_rwg)");
+    if (trian)
+        for (size_t i = 0; i < m.trian.size(); ++i) trian[i] = m.trian[i];
+    if (edges)
+        for (size_t i = 0; i < m.edges.size(); ++i) edges[i] = m.edges[i] + 1;
+    if (un) std::copy(m.un.begin(), m.un.end(), un);
+    if (ds) std::copy(m.ds.begin(), m.ds.end(), ds);
+    if (ln) std::copy(m.ln.begin(), m.ln.end(), ln);
+    if (cent) std::copy(m.cent.begin(), m.cent.end(), cent);
+    return FARCUR_OK;
+}
+
+void farcur_mesh_destroy(farcur_mesh* mesh) { delete mesh; }
+
+int farcur_partition_build(const farcur_mesh* mesh, double leaf_size, double eta_adm, farcur_partition** out) {
+    if (!mesh || !out) return fail(FARCUR_EINVAL, "NULL argument");
+    if (!mesh->rwg) return fail(FARCUR_ESTATE, "RWG tables not built");
+    if (!(leaf_size > 0) || !(eta_adm > 0)) return fail(FARCUR_EINVAL, "leaf_size and eta_adm must be > 0");
+    *out = new farcur_partition{build_partition(mesh->m, leaf_size, eta_adm)};
+    return FARCUR_OK;
+}
+
+int farcur_partition_sizes(const farcur_partition* p, int* ngroups, int* nrwg, int* nfar, int* nnear) {
+    if (!p) return fail(FARCUR_EINVAL, "NULL partition");
+    if (ngroups) *ngroups = (int)p->p.group_diam.size();
+    if (nrwg) *nrwg = (int)p->p.group_rwg.size();
+    if (nfar) *nfar = (int)(p->p.far_pairs.size() / 2);
+    if (nnear) *nnear = (int)(p->p.near_pairs.size() / 2);
+    return FARCUR_OK;
+}
+
+int farcur_partition_tables(const farcur_partition* p, int* group_offsets, int* group_rwg, double* centre,
+                            double* diam, int* far_pairs, int* near_pairs) {
+    if (!p) return fail(FARCUR_EINVAL, "NULL partition");
+    const BlockPartition& P = p->p;
+    if (group_offsets) std::copy(P.group_offsets.begin(), P.group_offsets.end(), group_offsets);
+    if (group_rwg) std::copy(P.group_rwg.begin(), P.group_rwg.end(), group_rwg);
+    if (centre) std::copy(P.group_centre.begin(), P.group_centre.end(), centre);
+    if (diam) std::copy(P.group_diam.begin(), P.group_diam.end(), diam);
+    if (far_pairs) std::copy(P.far_pairs.begin(), P.far_pairs.end(), far_pairs);
+    if (near_pairs) std::copy(P.near_pairs.begin(), P.near_pairs.end(), near_pairs);
+    return FARCUR_OK;
+}
+
+void farcur_partition_destroy(farcur_partition* p) { delete p; }
+
+// ------------------------------------------------------------------ context
+int farcur_create(int device, farcur_ctx** out) {
+    if (!out) return fail(FARCUR_EINVAL, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(FARCUR_EHIP, "no HIP device available");
+    if (device < 0 || device >= n) return fail(FARCUR_EINVAL, "device %d out of range (have %d)", device, n);
+    auto ctx = std::make_unique<farcur_ctx>();
+    ctx->device = device;
+    HIPTRY(hipSetDevice(device));
+    HIPTRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIPTRY(hipEventCreate(&ctx->ev0));
+    HIPTRY(hipEventCreate(&ctx->ev1));
+    *out = ctx.release();
+    return FARCUR_OK;
+}
+
+void farcur_destroy(farcur_ctx* ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    DBuf* bufs[] = {&ctx->rec64, &ctx->rec32, &ctx->d_rows, &ctx->d_cols, &ctx->d_cs,    &ctx->d_rs,
+                    &ctx->d_C,   &ctx->d_R,   &ctx->d_W,    &ctx->d_V,    &ctx->d_U,     &ctx->d_sigma,
+                    &ctx->d_v,   &ctx->d_y0,  &ctx->d_y1,   &ctx->d_plan, &ctx->d_out};
+    for (DBuf* b : bufs) b->release();
+    if (ctx->ev0) hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int farcur_set_em(farcur_ctx* ctx, double k, double eta) {
+    if (!ctx) return fail(FARCUR_EINVAL, "NULL context");
+    if (!(k > 0) || !(eta > 0) || !std::isfinite(k) || !std::isfinite(eta))
+        return fail(FARCUR_EINVAL, "k and eta must be positive finite");
+    ctx->k = k;
+    ctx->eta = eta;
+    return FARCUR_OK;
+}
+
+int farcur_set_geometry(farcur_ctx* ctx, const double* vertex, int nv, const double* topol, int nt,
+                        const double* edges, int ne, const double* ln) {
+    if (!ctx || !vertex || !topol || !edges || !ln) return fail(FARCUR_EINVAL, "NULL argument");
+    if (nv <= 0 || nt <= 0 || ne <= 0) return fail(FARCUR_EINVAL, "empty geometry");
+    std::vector<double> V(vertex, vertex + 3 * (size_t)nv), L(ln, ln + ne);
+    std::vector<int32_t> T(3 * (size_t)nt), E(4 * (size_t)ne);
+    for (size_t i = 0; i < T.size(); ++i) T[i] = (int32_t)std::lround(topol[i]) - 1;
+    for (size_t i = 0; i < E.size(); ++i) E[i] = (int32_t)std::lround(edges[i]) - 1;
+    return set_geometry_impl(ctx, V, T, E, L);
+}
+
+int farcur_set_mesh(farcur_ctx* ctx, const farcur_mesh* mesh) {
+    if (!ctx || !mesh) return fail(FARCUR_EINVAL, "NULL argument");
+    if (!mesh->rwg) return fail(FARCUR_ESTATE, "RWG tables not built");
+    return set_geometry_impl(ctx, mesh->m.vertex, mesh->m.topol, mesh->m.edges, mesh->m.ln);
+}
+
+int farcur_fill(farcur_ctx* ctx, const int* rows, int nr, const int* cols, int nc, int precision, farcur_c128* Z) {
+    if (!ctx || !rows || !cols || !Z) return fail(FARCUR_EINVAL, "NULL argument");
+    if (ctx->ne == 0) return fail(FARCUR_ESTATE, "no geometry");
+    if (nr <= 0 || nc <= 0) return fail(FARCUR_EINVAL, "empty panel");
+    if (precision != FARCUR_F64 && precision != FARCUR_F32) return fail(FARCUR_EINVAL, "bad precision");
+    TRY(check_ids(rows, nr, ctx->ne, "rows"));
+    TRY(check_ids(cols, nc, ctx->ne, "cols"));
+    TRY(ctx->select());
+    const int f32 = precision == FARCUR_F32;
+    TRY(ctx->d_rows.ensure(nr * 4));
+    TRY(ctx->d_cols.ensure(nc * 4));
+    TRY(ctx->d_C.ensure((size_t)nr * nc * psize(f32)));
+    HIPTRY(hipMemcpyAsync(ctx->d_rows.p, rows, nr * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPTRY(hipMemcpyAsync(ctx->d_cols.p, cols, nc * 4, hipMemcpyHostToDevice, ctx->stream));
+    launch_fill_panel(ctx->d_C.p, nr, ctx->d_rows.as<int32_t>(), nr, ctx->d_cols.as<int32_t>(), nc,
+                      ctx->fill_params(f32), ctx->stream);
+    HIPTRY(hipGetLastError());
+    if (f32) {
+        std::vector<c64> tmp((size_t)nr * nc);
+        HIPTRY(hipMemcpyAsync(tmp.data(), ctx->d_C.p, tmp.size() * sizeof(c64), hipMemcpyDeviceToHost, ctx->stream));
+        HIPTRY(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < tmp.size(); ++i) Z[i] = farcur_c128{tmp[i].re, tmp[i].im};
+    } else {
+        HIPTRY(hipMemcpyAsync(Z, ctx->d_C.p, (size_t)nr * nc * sizeof(c128), hipMemcpyDeviceToHost, ctx->stream));
+        HIPTRY(hipStreamSynchronize(ctx->stream));
+    }
+    return FARCUR_OK;
+}
+
+int farcur_pinv(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, double rtol, farcur_c128* U,
+                double* sigma, int* kept) {
+    if (!ctx || !W || !U) return fail(FARCUR_EINVAL, "NULL argument");
+    if (r <= 0 || r > 1024) return fail(FARCUR_EINVAL, "r = %d outside [1, 1024]", r);
+    if (!(atol >= 0) || !(rtol >= 0)) return fail(FARCUR_EINVAL, "thresholds must be >= 0");
+    TRY(ctx->select());
+    const size_t rr = (size_t)r * r * sizeof(c128);
+    TRY(ctx->d_W.ensure(rr));
+    TRY(ctx->d_V.ensure(rr));
+    TRY(ctx->d_U.ensure(rr));
+    TRY(ctx->d_sigma.ensure(r * sizeof(double)));
+    HIPTRY(hipMemcpyAsync(ctx->d_W.p, W, rr, hipMemcpyHostToDevice, ctx->stream));
+    BlockJob J{};
+    J.r = r;
+    J.W = ctx->d_W.as<c128>();
+    J.V = ctx->d_V.as<c128>();
+    J.U = ctx->d_U.as<c128>();
+    J.sigma = ctx->d_sigma.as<double>();
+    struct Plan {
+        BlockJob job;
+        int32_t list[4];
+        JobOut out;
+    } plan{};
+    plan.job = J;
+    TRY(ctx->d_plan.ensure(sizeof(Plan)));
+    HIPTRY(hipMemcpyAsync(ctx->d_plan.p, &plan, sizeof(Plan), hipMemcpyHostToDevice, ctx->stream));
+    Plan* dp = ctx->d_plan.as<Plan>();
+    const int lds = r <= kPinvLdsMaxR;
+    launch_pinv(&dp->job, dp->list, 1, atol, rtol, lds, lds ? 256 : (r > 128 ? 1024 : 256), &dp->out, ctx->stream);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(U, ctx->d_U.p, rr, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> s(r);
+    HIPTRY(hipMemcpyAsync(s.data(), ctx->d_sigma.p, r * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(hipMemcpyAsync(&plan, ctx->d_plan.p, sizeof(Plan), hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    std::sort(s.begin(), s.end(), [](double a, double b) { return a > b; });
+    if (sigma) std::copy(s.begin(), s.end(), sigma);
+    if (kept) *kept = plan.out.kept;
+    return FARCUR_OK;
+}
+
+int farcur_cur_residual(farcur_ctx* ctx, const farcur_c128* C, int m, int r, const farcur_c128* U,
+                        const farcur_c128* R, int n, const farcur_c128* v, const farcur_c128* y_old,
+                        farcur_c128* y_new, double* err) {
+    if (!ctx || !C || !U || !R || !v || !y_old || !y_new || !err) return fail(FARCUR_EINVAL, "NULL argument");
+    if (m <= 0 || n <= 0 || r <= 0 || r > 1024) return fail(FARCUR_EINVAL, "bad sizes m=%d n=%d r=%d", m, n, r);
+    TRY(ctx->select());
+    TRY(ctx->d_C.ensure((size_t)m * r * 16));
+    TRY(ctx->d_U.ensure((size_t)r * r * 16));
+    TRY(ctx->d_R.ensure((size_t)r * n * 16));
+    TRY(ctx->d_v.ensure((size_t)n * 16));
+    TRY(ctx->d_y0.ensure((size_t)m * 16));
+    TRY(ctx->d_y1.ensure((size_t)m * 16));
+    hipStream_t s = ctx->stream;
+    HIPTRY(hipMemcpyAsync(ctx->d_C.p, C, (size_t)m * r * 16, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(ctx->d_U.p, U, (size_t)r * r * 16, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(ctx->d_R.p, R, (size_t)r * n * 16, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(ctx->d_v.p, v, (size_t)n * 16, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(ctx->d_y0.p, y_old, (size_t)m * 16, hipMemcpyHostToDevice, s));
+    struct Plan {
+        BlockJob job;
+        JobOut out;
+    } plan{};
+    plan.job.m = m;
+    plan.job.n = n;
+    plan.job.r = r;
+    plan.job.C = ctx->d_C.p;
+    plan.job.R = ctx->d_R.p;
+    plan.job.U = ctx->d_U.as<c128>();
+    plan.job.v = ctx->d_v.as<c128>();
+    plan.job.y_old = ctx->d_y0.as<c128>();
+    plan.job.y_new = ctx->d_y1.as<c128>();
+    TRY(ctx->d_plan.ensure(sizeof(Plan)));
+    HIPTRY(hipMemcpyAsync(ctx->d_plan.p, &plan, sizeof(Plan), hipMemcpyHostToDevice, s));
+    Plan* dp = ctx->d_plan.as<Plan>();
+    launch_residual(&dp->job, 1, 0, 1024, &dp->out, s);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(y_new, ctx->d_y1.p, (size_t)m * 16, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(&plan, ctx->d_plan.p, sizeof(Plan), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    *err = plan.out.err;
+    return FARCUR_OK;
+}
+
+int farcur_sample(farcur_ctx* ctx, uint64_t seed, int block, int round, int purpose, int N, int r, int* out) {
+    if (!ctx || !out) return fail(FARCUR_EINVAL, "NULL argument");
+    if (N <= 0 || r <= 0 || r > N) return fail(FARCUR_EINVAL, "need 0 < r <= N (r=%d N=%d)", r, N);
+    if (purpose != kStreamCols && purpose != kStreamRows) return fail(FARCUR_EINVAL, "purpose must be 1 or 2");
+    if (block < 0 || round < 0 || round > 0xffffff) return fail(FARCUR_EINVAL, "bad block/round");
+    TRY(ctx->select());
+    std::vector<int32_t> ids(N);
+    for (int i = 0; i < N; ++i) ids[i] = i;
+    TRY(ctx->d_rows.ensure(N * 4));
+    TRY(ctx->d_cs.ensure(r * 4));
+    HIPTRY(hipMemcpyAsync(ctx->d_rows.p, ids.data(), N * 4, hipMemcpyHostToDevice, ctx->stream));
+    BlockJob J{};
+    J.b = block;
+    J.round = round;
+    J.r = r;
+    J.sample_mode = 0;
+    if (purpose == kStreamCols) {
+        J.n = N;
+        J.cols = ctx->d_rows.as<int32_t>();
+        J.cs = ctx->d_cs.as<int32_t>();
+        J.m = 0;
+        J.rs = nullptr;
+    } else {
+        J.m = N;
+        J.rows = ctx->d_rows.as<int32_t>();
+        J.rs = ctx->d_cs.as<int32_t>();
+        J.n = 0;
+        J.cs = nullptr;
+    }
+    TRY(ctx->d_plan.ensure(sizeof(BlockJob)));
+    HIPTRY(hipMemcpyAsync(ctx->d_plan.p, &J, sizeof J, hipMemcpyHostToDevice, ctx->stream));
+    // k_sample runs both kinds for the job; the kind without candidates draws nothing.
+    launch_sample(ctx->d_plan.as<BlockJob>(), 1, seed, ctx->stream);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(out, ctx->d_cs.p, r * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    return FARCUR_OK;
+}
+
+int farcur_vtest(uint64_t seed, int block, int n, farcur_c128* v) {
+    if (!v || n < 0 || block < 0) return fail(FARCUR_EINVAL, "bad arguments");
+    for (int j = 0; j < n; ++j) vtest_entry(seed, (uint32_t)block, (uint32_t)j, &v[j].re, &v[j].im);
+    return FARCUR_OK;
+}
+
+int farcur_pivot_rows(farcur_ctx* ctx, const farcur_c128* C, int m, int r, int* rows) {
+    if (!ctx || !C || !rows) return fail(FARCUR_EINVAL, "NULL argument");
+    if (m <= 0 || r <= 0) return fail(FARCUR_EINVAL, "bad sizes");
+    TRY(ctx->select());
+    std::vector<int32_t> ids(m);
+    for (int i = 0; i < m; ++i) ids[i] = i;
+    TRY(ctx->d_rows.ensure(m * 4));
+    TRY(ctx->d_rs.ensure(r * 4));
+    TRY(ctx->d_C.ensure((size_t)m * r * 16));
+    HIPTRY(hipMemcpyAsync(ctx->d_rows.p, ids.data(), m * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPTRY(hipMemcpyAsync(ctx->d_C.p, C, (size_t)m * r * 16, hipMemcpyHostToDevice, ctx->stream));
+    BlockJob J{};
+    J.m = m;
+    J.r = r;
+    J.sample_mode = 2;
+    J.rows = ctx->d_rows.as<int32_t>();
+    J.rs = ctx->d_rs.as<int32_t>();
+    J.C = ctx->d_C.p;
+    TRY(ctx->d_plan.ensure(sizeof(BlockJob)));
+    HIPTRY(hipMemcpyAsync(ctx->d_plan.p, &J, sizeof J, hipMemcpyHostToDevice, ctx->stream));
+    launch_pivot_rows(ctx->d_plan.as<BlockJob>(), 1, 0, ctx->stream);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(rows, ctx->d_rs.p, r * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    return FARCUR_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ per-block loop
+namespace {
+
+// Device buffers of one block's round.
+struct RoundBufs {
+    int32_t *cs, *rs;
+    void *C, *R;
+    c128 *W, *V, *U;
+    double* sigma;
+};
+
+struct BlockLoopIO {
+    const int32_t* d_rows;
+    const int32_t* d_cols;
+    const c128* d_v;
+    c128* d_y[2];
+    int m, n, b;
+};
+
+// Runs the cuda_cur_compress3 loop (SW2:878-973) for one block with a sync per
+// round. `alloc` returns device buffers for a round of sample count r.
+template <typename Alloc>
+int block_loop(farcur_ctx* ctx, const BlockLoopIO& io, const farcur_opts& o, Alloc alloc, DBuf& plan_buf,
+               KTimer& kt, farcur_block_info* info, RoundBufs* final_bufs, int* final_y) {
+    const int D = std::min(io.m, io.n);
+    const int cap = o.max_rank > 0 ? std::min(D, o.max_rank) : D;
+    if (cap > 1024) return fail(FARCUR_EINVAL, "sample count cap %d exceeds 1024 (set max_rank)", cap);
+    int r = o.r_start > 0 ? std::min(o.r_start, cap) : std::min(r_start_rule(D), cap);
+    const int f32 = o.precision == FARCUR_F32;
+    int round = 0, ycur = 0;
+    double err = INFINITY;
+    JobOut res{};
+    RoundBufs rb{};
+    double dev_ms = 0;
+    int final_r = r;
+    bool converged = false;
+    while (true) {
+        TRY(alloc(r, rb));
+        BlockJob J{};
+        J.b = io.b;
+        J.m = io.m;
+        J.n = io.n;
+        J.r = r;
+        J.round = round;
+        J.sample_mode = o.row_mode == FARCUR_ROWS_PIVOT ? 2 : 0;
+        J.rows = io.d_rows;
+        J.cols = io.d_cols;
+        J.cs = rb.cs;
+        J.rs = rb.rs;
+        J.C = rb.C;
+        J.R = rb.R;
+        J.W = rb.W;
+        J.V = rb.V;
+        J.U = rb.U;
+        J.sigma = rb.sigma;
+        J.v = io.d_v;
+        J.y_old = io.d_y[ycur];
+        J.y_new = io.d_y[ycur ^ 1];
+        std::vector<BlockJob> jobs{J};
+        PackedRound pk;
+        pack_round(jobs, o.row_mode == FARCUR_ROWS_PIVOT, pk);
+        TRY(plan_buf.ensure(pk.bytes.size()));
+        HIPTRY(hipMemcpyAsync(plan_buf.p, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
+        RoundPlan P = plan_from(pk, plan_buf.p, 1);
+        P.f32 = f32;
+        P.pivot = o.row_mode == FARCUR_ROWS_PIVOT;
+        P.atol = o.pinv_tol;
+        P.rtol = o.pinv_rtol;
+        P.seed = o.seed;
+        const bool big = (size_t)io.m * r > (size_t)256 * 1024 || r > 128;
+        P.pinv_threads_global = r > 128 ? 1024 : 256;
+        P.resid_threads = big ? 1024 : 256;
+        HIPTRY(hipEventRecord(ctx->ev0, ctx->stream));
+        run_round(ctx, P, kt);
+        HIPTRY(hipEventRecord(ctx->ev1, ctx->stream));
+        HIPTRY(hipGetLastError());
+        HIPTRY(hipMemcpyAsync(&res, P.out, sizeof res, hipMemcpyDeviceToHost, ctx->stream));
+        HIPTRY(hipStreamSynchronize(ctx->stream));
+        float ms = 0;
+        hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        dev_ms += ms;
+        kt.collect();
+        ycur ^= 1;
+        final_r = r;
+        *final_bufs = rb;
+        if (round == 0) {
+            // The first round's error compares against ones(M): discarded (SW2:902-906).
+            int rn;
+            next_r(r, cap, 0, &rn);
+            r = rn;
+            round = 1;
+            continue;
+        }
+        err = res.err;
+        ++round;
+        if (!(err > o.tol)) {
+            converged = true;
+            break;
+        }
+        int rn;
+        if (!next_r(r, cap, round, &rn)) break;
+        r = rn;
+    }
+    info->r = final_r;
+    info->rounds = round;
+    info->kept = res.kept;
+    info->converged = converged ? 1 : 0;
+    info->err = err;
+    info->heldout_err = -1;
+    info->sigma_max = res.sigma_max;
+    info->time_ms = dev_ms;
+    *final_y = ycur;
+    return FARCUR_OK;
+}
+
+}  // namespace
+
+extern "C" int farcur_compress_block(farcur_ctx* ctx, const int* rows, int m, const int* cols, int n,
+                                     const farcur_opts* opts, farcur_block_info* info) {
+    if (!ctx || !rows || !cols || !info) return fail(FARCUR_EINVAL, "NULL argument");
+    TRY(validate_opts(opts));
+    if (ctx->ne == 0) return fail(FARCUR_ESTATE, "no geometry");
+    if (m <= 0 || n <= 0) return fail(FARCUR_EINVAL, "empty block");
+    TRY(check_ids(rows, m, ctx->ne, "rows"));
+    TRY(check_ids(cols, n, ctx->ne, "cols"));
+    TRY(ctx->select());
+    ctx->has_result = false;
+    const int f32 = opts->precision == FARCUR_F32;
+    TRY(ctx->d_rows.ensure(m * 4));
+    TRY(ctx->d_cols.ensure(n * 4));
+    TRY(ctx->d_v.ensure((size_t)n * 16));
+    TRY(ctx->d_y0.ensure((size_t)m * 16));
+    TRY(ctx->d_y1.ensure((size_t)m * 16));
+    HIPTRY(hipMemcpyAsync(ctx->d_rows.p, rows, m * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPTRY(hipMemcpyAsync(ctx->d_cols.p, cols, n * 4, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<c128> v(n);
+    for (int j = 0; j < n; ++j) vtest_entry(opts->seed, (uint32_t)opts->block_id, (uint32_t)j, &v[j].re, &v[j].im);
+    HIPTRY(hipMemcpyAsync(ctx->d_v.p, v.data(), (size_t)n * 16, hipMemcpyHostToDevice, ctx->stream));
+    launch_ones(ctx->d_y0.as<c128>(), m, ctx->stream);
+    BlockLoopIO io{ctx->d_rows.as<int32_t>(), ctx->d_cols.as<int32_t>(), ctx->d_v.as<c128>(),
+                   {ctx->d_y0.as<c128>(), ctx->d_y1.as<c128>()}, m, n, opts->block_id};
+    auto alloc = [&](int r, RoundBufs& rb) -> int {
+        TRY(ctx->d_cs.ensure(r * 4));
+        TRY(ctx->d_rs.ensure(r * 4));
+        TRY(ctx->d_C.ensure((size_t)m * r * psize(f32)));
+        TRY(ctx->d_R.ensure((size_t)r * n * psize(f32)));
+        TRY(ctx->d_W.ensure((size_t)r * r * 16));
+        TRY(ctx->d_V.ensure((size_t)r * r * 16));
+        TRY(ctx->d_U.ensure((size_t)r * r * 16));
+        TRY(ctx->d_sigma.ensure(r * 8));
+        rb = RoundBufs{ctx->d_cs.as<int32_t>(), ctx->d_rs.as<int32_t>(), ctx->d_C.p,         ctx->d_R.p,
+                       ctx->d_W.as<c128>(),    ctx->d_V.as<c128>(),    ctx->d_U.as<c128>(), ctx->d_sigma.as<double>()};
+        return FARCUR_OK;
+    };
+    KTimer kt;
+    RoundBufs fb{};
+    int fy = 0;
+    TRY(block_loop(ctx, io, *opts, alloc, ctx->d_plan, kt, info, &fb, &fy));
+    ctx->last_m = m;
+    ctx->last_n = n;
+    ctx->last_r = info->r;
+    ctx->last_f32 = f32;
+    ctx->last_y = fy;
+    ctx->has_result = true;
+    return FARCUR_OK;
+}
+
+namespace {
+int copy_panel(farcur_c128* dst, const void* src, size_t count, int f32, hipStream_t s) {
+    if (!dst) return FARCUR_OK;
+    if (f32) {
+        std::vector<c64> tmp(count);
+        HIPTRY(hipMemcpyAsync(tmp.data(), src, count * sizeof(c64), hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        for (size_t i = 0; i < count; ++i) dst[i] = farcur_c128{tmp[i].re, tmp[i].im};
+    } else {
+        HIPTRY(hipMemcpyAsync(dst, src, count * sizeof(c128), hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+    }
+    return FARCUR_OK;
+}
+}  // namespace
+
+extern "C" int farcur_block_result(farcur_ctx* ctx, farcur_c128* C, farcur_c128* U, farcur_c128* R,
+                                   int* row_samples, int* col_samples, farcur_c128* y) {
+    if (!ctx) return fail(FARCUR_EINVAL, "NULL context");
+    if (!ctx->has_result) return fail(FARCUR_ESTATE, "no compress_block result");
+    TRY(ctx->select());
+    const int m = ctx->last_m, n = ctx->last_n, r = ctx->last_r, f32 = ctx->last_f32;
+    hipStream_t s = ctx->stream;
+    TRY(copy_panel(C, ctx->d_C.p, (size_t)m * r, f32, s));
+    TRY(copy_panel(R, ctx->d_R.p, (size_t)r * n, f32, s));
+    TRY(copy_panel(U, ctx->d_U.p, (size_t)r * r, 0, s));
+    TRY(copy_panel(y, ctx->last_y ? ctx->d_y1.p : ctx->d_y0.p, (size_t)m, 0, s));
+    if (row_samples) HIPTRY(hipMemcpyAsync(row_samples, ctx->d_rs.p, r * 4, hipMemcpyDeviceToHost, s));
+    if (col_samples) HIPTRY(hipMemcpyAsync(col_samples, ctx->d_cs.p, r * 4, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    return FARCUR_OK;
+}
+
+// ------------------------------------------------------------------ drop-in
+namespace {
+std::mutex g_compat_mu;
+std::vector<farcur_ctx*> g_compat_ctx;  // one per device, kept for the process lifetime
+
+int compat_ctx(farcur_ctx** out) {
+    int dev = 0;
+    HIPTRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_compat_mu);
+    if ((int)g_compat_ctx.size() <= dev) g_compat_ctx.resize(dev + 1, nullptr);
+    if (!g_compat_ctx[dev]) TRY(farcur_create(dev, &g_compat_ctx[dev]));
+    *out = g_compat_ctx[dev];
+    return FARCUR_OK;
+}
+
+int to_ids(const double* src, int n, int ne, std::vector<int32_t>& out, const char* what) {
+    out.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const double x = src[i];
+        if (!(x >= 1 && x <= ne) || x != std::floor(x))
+            return fail(FARCUR_EINVAL, "%s[%d] = %g is not a 1-based RWG id in [1, %d]", what, i, x, ne);
+        out[i] = (int32_t)x - 1;
+    }
+    return FARCUR_OK;
+}
+
+int compat_impl(farcur_c128* Cc, farcur_c128* Uc, farcur_c128* Rc, int r, double* vec_time, double pinv_tol,
+                farcur_c128* y_old, farcur_c128* y_new, farcur_c128* vtest, double* col_s, double* row_s,
+                int field, double k, double eta, double* r1, double* r2, double* vertex, double* topol,
+                double* edges, double* ln, int n_r1, int n_r2, int rows_vertex, int cols_vertex, int rows_topol,
+                int cols_topol, int rows_edges, int cols_edges, int n_ln, double* err_out) {
+    if (!Cc || !Uc || !Rc || !vec_time || !y_old || !y_new || !vtest || !col_s || !row_s || !r1 || !r2 || !vertex ||
+        !topol || !edges || !ln)
+        return fail(FARCUR_EINVAL, "NULL argument");
+    if (field != 1) return fail(FARCUR_EINVAL, "field = %d: only the EFIE (field = 1) is implemented", field);
+    if (rows_vertex != 3 || rows_topol != 3 || rows_edges != 4)
+        return fail(FARCUR_EINVAL, "table shapes must be vertex 3xNv, topol 3xNt, edges 4xNe");
+    if (cols_edges != n_ln) return fail(FARCUR_EINVAL, "edges has %d columns but ln has %d entries", cols_edges, n_ln);
+    if (n_r1 <= 0 || n_r2 <= 0 || r <= 0) return fail(FARCUR_EINVAL, "empty block or cur_order");
+    if (r > std::min(n_r1, n_r2)) return fail(FARCUR_EINVAL, "cur_order %d exceeds min(n_r1, n_r2)", r);
+    if (r > 1024) return fail(FARCUR_EINVAL, "cur_order %d exceeds 1024", r);
+    farcur_ctx* ctx;
+    TRY(compat_ctx(&ctx));
+    TRY(ctx->select());
+    // Geometry cache keyed by the table contents (the reference re-uploads per call).
+    uint64_t h = 0xcbf29ce484222325ull;
+    h = fnv1a(h, vertex, sizeof(double) * 3 * (size_t)cols_vertex);
+    h = fnv1a(h, topol, sizeof(double) * 3 * (size_t)cols_topol);
+    h = fnv1a(h, edges, sizeof(double) * 4 * (size_t)cols_edges);
+    h = fnv1a(h, ln, sizeof(double) * (size_t)n_ln);
+    if (h != ctx->geom_hash || ctx->ne != n_ln) {
+        TRY(farcur_set_geometry(ctx, vertex, cols_vertex, topol, cols_topol, edges, cols_edges, ln));
+        ctx->geom_hash = h;
+    }
+    TRY(farcur_set_em(ctx, k, eta));
+    std::vector<int32_t> rows, cols, cs, rs;
+    TRY(to_ids(r1, n_r1, ctx->ne, rows, "r1"));
+    TRY(to_ids(r2, n_r2, ctx->ne, cols, "r2"));
+    TRY(to_ids(col_s, r, ctx->ne, cs, "col_samples"));
+    TRY(to_ids(row_s, r, ctx->ne, rs, "row_samples"));
+    hipStream_t s = ctx->stream;
+    const int m = n_r1, n = n_r2;
+    TRY(ctx->d_rows.ensure(m * 4));
+    TRY(ctx->d_cols.ensure(n * 4));
+    TRY(ctx->d_cs.ensure(r * 4));
+    TRY(ctx->d_rs.ensure(r * 4));
+    TRY(ctx->d_C.ensure((size_t)m * r * 16));
+    TRY(ctx->d_R.ensure((size_t)r * n * 16));
+    TRY(ctx->d_W.ensure((size_t)r * r * 16));
+    TRY(ctx->d_V.ensure((size_t)r * r * 16));
+    TRY(ctx->d_U.ensure((size_t)r * r * 16));
+    TRY(ctx->d_sigma.ensure(r * 8));
+    TRY(ctx->d_v.ensure((size_t)n * 16));
+    TRY(ctx->d_y0.ensure((size_t)m * 16));
+    TRY(ctx->d_y1.ensure((size_t)m * 16));
+    HIPTRY(hipMemcpyAsync(ctx->d_rows.p, rows.data(), m * 4, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(ctx->d_cols.p, cols.data(), n * 4, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(ctx->d_cs.p, cs.data(), r * 4, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(ctx->d_rs.p, rs.data(), r * 4, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(ctx->d_v.p, vtest, (size_t)n * 16, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(ctx->d_y0.p, y_old, (size_t)m * 16, hipMemcpyHostToDevice, s));
+    BlockJob J{};
+    J.m = m;
+    J.n = n;
+    J.r = r;
+    J.sample_mode = 1;
+    J.rows = ctx->d_rows.as<int32_t>();
+    J.cols = ctx->d_cols.as<int32_t>();
+    J.cs = ctx->d_cs.as<int32_t>();
+    J.rs = ctx->d_rs.as<int32_t>();
+    J.C = ctx->d_C.p;
+    J.R = ctx->d_R.p;
+    J.W = ctx->d_W.as<c128>();
+    J.V = ctx->d_V.as<c128>();
+    J.U = ctx->d_U.as<c128>();
+    J.sigma = ctx->d_sigma.as<double>();
+    J.v = ctx->d_v.as<c128>();
+    J.y_old = ctx->d_y0.as<c128>();
+    J.y_new = ctx->d_y1.as<c128>();
+    std::vector<BlockJob> jobs{J};
+    PackedRound pk;
+    pack_round(jobs, false, pk);
+    TRY(ctx->d_plan.ensure(pk.bytes.size()));
+    HIPTRY(hipMemcpyAsync(ctx->d_plan.p, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice, s));
+    RoundPlan P = plan_from(pk, ctx->d_plan.p, 1);
+    P.atol = pinv_tol;
+    P.rtol = 0;
+    P.pinv_threads_global = r > 128 ? 1024 : 256;
+    P.resid_threads = ((size_t)m * r > (size_t)256 * 1024 || r > 128) ? 1024 : 256;
+    KTimer kt;
+    HIPTRY(hipEventRecord(ctx->ev0, s));
+    run_round(ctx, P, kt);
+    HIPTRY(hipEventRecord(ctx->ev1, s));
+    HIPTRY(hipGetLastError());
+    JobOut res{};
+    HIPTRY(hipMemcpyAsync(&res, P.out, sizeof res, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(Cc, ctx->d_C.p, (size_t)m * r * 16, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(Uc, ctx->d_U.p, (size_t)r * r * 16, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(Rc, ctx->d_R.p, (size_t)r * n * 16, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(y_new, ctx->d_y1.p, (size_t)m * 16, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    vec_time[0] = ms;
+    *err_out = res.err;
+    return FARCUR_OK;
+}
+}  // namespace
+
+extern "C" double cuda_complete_compression(
+    farcur_c128* Ccompressed, farcur_c128* Ucompressed, farcur_c128* Rcompressed, int cur_order, double* vec_time_cuda,
+    double pinv_tol, farcur_c128* Avtest_old, farcur_c128* Avtest_new, farcur_c128* vtest, double* col_samples,
+    double* row_samples, int, int, int, int, int, int, int, int, int, int, int, int, farcur_c128* /*Ztotal*/,
+    int field_arg, double k_arg, double eta_arg, double /*Rinteg_s*/, double /*Ranal_s*/, double /*Rinteg_f*/,
+    int /*cor_solid*/, int /*flag*/, double* r1_arg, double* r2_arg, double* vertex_arg, double* topol_arg,
+    double* /*trian*/, double* edges_arg, double* /*un*/, double* /*ds*/, double* ln_arg, double* /*cent*/,
+    int /*N_arg*/, int n_r1_arg, int n_r2_arg, int rows_vertex_arg, int cols_vertex_arg, int rows_topol_arg,
+    int cols_topol_arg, int /*rows_trian*/, int /*cols_trian*/, int rows_edges_arg, int cols_edges_arg,
+    int /*rows_un*/, int /*cols_un*/, int /*rows_ds*/, int /*cols_ds*/, int n_ln_arg, int /*rows_cent*/,
+    int /*cols_cent*/, int, int, int, int, int, int) {
+    double err = NAN;
+    const int rc = compat_impl(Ccompressed, Ucompressed, Rcompressed, cur_order, vec_time_cuda, pinv_tol, Avtest_old,
+                               Avtest_new, vtest, col_samples, row_samples, field_arg, k_arg, eta_arg, r1_arg, r2_arg,
+                               vertex_arg, topol_arg, edges_arg, ln_arg, n_r1_arg, n_r2_arg, rows_vertex_arg,
+                               cols_vertex_arg, rows_topol_arg, cols_topol_arg, rows_edges_arg, cols_edges_arg,
+                               n_ln_arg, &err);
+    if (rc != FARCUR_OK) {
+        if (vec_time_cuda) vec_time_cuda[0] = -1.0;
+        return NAN;
+    }
+    return err;
+}
+
+// ------------------------------------------------------------------ batches
+namespace {
+
+struct BState {
+    int m = 0, n = 0, cap = 0, r = 0, round = 0;
+    int status = 0;  // 0 active, 1 converged, 2 stopped at the cap
+    int ycur = 0;
+    int kept = 0;
+    double err = INFINITY, smax = 0;
+    RoundBufs fin{};
+    int fin_r = 0;
+};
+
+// Distinct triangles touched by a set of RWGs.
+int distinct_triangles(const std::vector<int32_t>& tri_of_rwg, const int32_t* ids, int n,
+                       std::unordered_set<int32_t>& scratch) {
+    scratch.clear();
+    for (int i = 0; i < n; ++i) {
+        scratch.insert(tri_of_rwg[2 * (size_t)ids[i]]);
+        scratch.insert(tri_of_rwg[2 * (size_t)ids[i] + 1]);
+    }
+    return (int)scratch.size();
+}
+
+}  // namespace
+
+struct farcur_batch {
+    farcur_ctx* ctx = nullptr;
+    int nb = 0;
+    std::vector<int32_t> row_off, col_off, row_ids, col_ids;
+    DBuf d_rows, d_cols, d_v, d_ya, d_yb, d_vo, d_plan;
+    Arena arena;
+    std::vector<BState> st;
+    KTimer kt;
+    bool profile = false;
+    int f32 = 0;
+    bool has_result = false;
+    // Jobs of every round of the last compress (host copies), for accounting.
+    std::vector<std::vector<BlockJob>> rounds;
+    double work[KTimer::kClasses] = {0, 0, 0, 0};
+};
+
+namespace {
+
+int batch_alloc_round(farcur_batch* B, int r, int m, int n, RoundBufs& rb) {
+    const size_t ps = psize(B->f32);
+    void* p;
+    TRY(B->arena.alloc((size_t)r * 4, &p));
+    rb.cs = (int32_t*)p;
+    TRY(B->arena.alloc((size_t)r * 4, &p));
+    rb.rs = (int32_t*)p;
+    TRY(B->arena.alloc((size_t)m * r * ps, &p));
+    rb.C = p;
+    TRY(B->arena.alloc((size_t)r * n * ps, &p));
+    rb.R = p;
+    TRY(B->arena.alloc((size_t)r * r * 16, &p));
+    rb.W = (c128*)p;
+    if (r > kPinvLdsMaxR) {
+        TRY(B->arena.alloc((size_t)r * r * 16, &p));
+        rb.V = (c128*)p;
+    } else {
+        rb.V = nullptr;
+    }
+    TRY(B->arena.alloc((size_t)r * r * 16, &p));
+    rb.U = (c128*)p;
+    TRY(B->arena.alloc((size_t)r * 8, &p));
+    rb.sigma = (double*)p;
+    return FARCUR_OK;
+}
+
+BlockJob make_job(farcur_batch* B, int b, const RoundBufs& rb, const farcur_opts& o) {
+    const BState& s = B->st[b];
+    BlockJob J{};
+    J.b = b;
+    J.m = s.m;
+    J.n = s.n;
+    J.r = s.r;
+    J.round = s.round;
+    J.sample_mode = o.row_mode == FARCUR_ROWS_PIVOT ? 2 : 0;
+    J.rows = B->d_rows.as<int32_t>() + B->row_off[b];
+    J.cols = B->d_cols.as<int32_t>() + B->col_off[b];
+    J.cs = rb.cs;
+    J.rs = rb.rs;
+    J.C = rb.C;
+    J.R = rb.R;
+    J.W = rb.W;
+    J.V = rb.V;
+    J.U = rb.U;
+    J.sigma = rb.sigma;
+    J.v = B->d_v.as<c128>() + B->col_off[b];
+    c128* ya = B->d_ya.as<c128>() + B->row_off[b];
+    c128* yb = B->d_yb.as<c128>() + B->row_off[b];
+    J.y_old = s.ycur ? yb : ya;
+    J.y_new = s.ycur ? ya : yb;
+    return J;
+}
+
+// Applies one round's result to a block's state (cuda_cur_compress3 control flow).
+void advance(BState& s, const JobOut& out, const RoundBufs& rb, double tol) {
+    s.ycur ^= 1;
+    s.fin = rb;
+    s.fin_r = s.r;
+    s.kept = out.kept;
+    s.smax = out.sigma_max;
+    if (s.round == 0) {
+        int rn;
+        next_r(s.r, s.cap, 0, &rn);
+        s.r = rn;
+        s.round = 1;
+        return;
+    }
+    s.err = out.err;
+    s.round += 1;
+    if (!(s.err > tol)) {
+        s.status = 1;
+        return;
+    }
+    int rn;
+    if (!next_r(s.r, s.cap, s.round, &rn)) {
+        s.status = 2;
+        return;
+    }
+    s.r = rn;
+}
+
+int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_opts& o, std::vector<RoundBufs>& bufs,
+                std::vector<JobOut>& outs) {
+    farcur_ctx* ctx = B->ctx;
+    std::vector<BlockJob> jobs;
+    jobs.reserve(active.size());
+    bufs.resize(active.size());
+    for (size_t i = 0; i < active.size(); ++i) {
+        const int b = active[i];
+        TRY(batch_alloc_round(B, B->st[b].r, B->st[b].m, B->st[b].n, bufs[i]));
+        jobs.push_back(make_job(B, b, bufs[i], o));
+    }
+    PackedRound pk;
+    pack_round(jobs, o.row_mode == FARCUR_ROWS_PIVOT, pk);
+    TRY(B->d_plan.ensure(pk.bytes.size()));
+    HIPTRY(hipMemcpyAsync(B->d_plan.p, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
+    RoundPlan P = plan_from(pk, B->d_plan.p, (int)jobs.size());
+    P.f32 = B->f32;
+    P.pivot = o.row_mode == FARCUR_ROWS_PIVOT;
+    P.atol = o.pinv_tol;
+    P.rtol = o.pinv_rtol;
+    P.seed = o.seed;
+    P.fill_grid = 256 * 16;
+    P.pinv_threads_global = 256;
+    P.resid_threads = 256;
+    run_round(ctx, P, B->kt);
+    HIPTRY(hipGetLastError());
+    outs.resize(jobs.size());
+    HIPTRY(hipMemcpyAsync(outs.data(), P.out, jobs.size() * sizeof(JobOut), hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    B->kt.collect();
+    if (B->profile) B->rounds.push_back(std::move(jobs));
+    return FARCUR_OK;
+}
+
+// Algorithmic work of the recorded rounds: Green evaluations of the fill
+// (9 x distinct test/source triangle pairs of each panel), 96 r^3 flops per
+// core pinv, bytes of the residual's panel reads.
+int batch_account(farcur_batch* B) {
+    farcur_ctx* ctx = B->ctx;
+    std::unordered_set<int32_t> scratch;
+    std::vector<int> trows(B->nb), tcols(B->nb);
+    for (int b = 0; b < B->nb; ++b) {
+        trows[b] = distinct_triangles(ctx->tri_of_rwg, &B->row_ids[B->row_off[b]], B->row_off[b + 1] - B->row_off[b],
+                                      scratch);
+        tcols[b] = distinct_triangles(ctx->tri_of_rwg, &B->col_ids[B->col_off[b]], B->col_off[b + 1] - B->col_off[b],
+                                      scratch);
+    }
+    for (double& w : B->work) w = 0;
+    const double ps = (double)psize(B->f32);
+    for (auto& jobs : B->rounds) {
+        for (const BlockJob& J : jobs) {
+            std::vector<int32_t> cs(J.r), rs(J.r);
+            HIPTRY(hipMemcpy(cs.data(), J.cs, J.r * 4, hipMemcpyDeviceToHost));
+            HIPTRY(hipMemcpy(rs.data(), J.rs, J.r * 4, hipMemcpyDeviceToHost));
+            const double tcs = distinct_triangles(ctx->tri_of_rwg, cs.data(), J.r, scratch);
+            const double trs = distinct_triangles(ctx->tri_of_rwg, rs.data(), J.r, scratch);
+            B->work[1] += 9.0 * (trows[J.b] * tcs + trs * tcs + trs * tcols[J.b]);
+            B->work[2] += 96.0 * (double)J.r * J.r * J.r;
+            B->work[3] += ps * ((double)J.m * J.r + (double)J.r * J.n) + 16.0 * J.r * J.r + 16.0 * (J.n + 2.0 * J.m);
+        }
+    }
+    return FARCUR_OK;
+}
+
+}  // namespace
+
+extern "C" int farcur_batch_create(farcur_ctx* ctx, int nblocks, const int* row_off, const int* row_ids,
+                                   const int* col_off, const int* col_ids, farcur_batch** out) {
+    if (!ctx || !row_off || !col_off || !out) return fail(FARCUR_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (nblocks <= 0) return fail(FARCUR_EINVAL, "nblocks must be > 0");
+    if (ctx->ne == 0) return fail(FARCUR_ESTATE, "no geometry");
+    if (row_off[0] != 0 || col_off[0] != 0) return fail(FARCUR_EINVAL, "offsets must start at 0");
+    for (int b = 0; b < nblocks; ++b)
+        if (row_off[b + 1] <= row_off[b] || col_off[b + 1] <= col_off[b])
+            return fail(FARCUR_EINVAL, "block %d is empty or offsets decrease", b);
+    const int nr = row_off[nblocks], nc = col_off[nblocks];
+    if (!row_ids || !col_ids) return fail(FARCUR_EINVAL, "NULL ids");
+    TRY(check_ids(row_ids, nr, ctx->ne, "row_ids"));
+    TRY(check_ids(col_ids, nc, ctx->ne, "col_ids"));
+    TRY(ctx->select());
+    auto B = std::make_unique<farcur_batch>();
+    B->ctx = ctx;
+    B->nb = nblocks;
+    B->row_off.assign(row_off, row_off + nblocks + 1);
+    B->col_off.assign(col_off, col_off + nblocks + 1);
+    B->row_ids.assign(row_ids, row_ids + nr);
+    B->col_ids.assign(col_ids, col_ids + nc);
+    TRY(B->d_rows.ensure((size_t)nr * 4));
+    TRY(B->d_cols.ensure((size_t)nc * 4));
+    TRY(B->d_v.ensure((size_t)nc * 16));
+    TRY(B->d_vo.ensure((size_t)(nblocks + 1) * 4));
+    TRY(B->d_ya.ensure((size_t)nr * 16));
+    TRY(B->d_yb.ensure((size_t)nr * 16));
+    HIPTRY(hipMemcpyAsync(B->d_rows.p, row_ids, (size_t)nr * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPTRY(hipMemcpyAsync(B->d_cols.p, col_ids, (size_t)nc * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPTRY(hipMemcpyAsync(B->d_vo.p, col_off, (size_t)(nblocks + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    B->st.resize(nblocks);
+    *out = B.release();
+    return FARCUR_OK;
+}
+
+extern "C" void farcur_batch_destroy(farcur_batch* B) {
+    if (!B) return;
+    hipSetDevice(B->ctx->device);
+    hipStreamSynchronize(B->ctx->stream);
+    DBuf* bufs[] = {&B->d_rows, &B->d_cols, &B->d_v, &B->d_ya, &B->d_yb, &B->d_vo, &B->d_plan};
+    for (DBuf* b : bufs) b->release();
+    B->arena.release();
+    delete B;
+}
+
+extern "C" int farcur_batch_profile(farcur_batch* B, int enable) {
+    if (!B) return fail(FARCUR_EINVAL, "NULL batch");
+    B->profile = enable != 0;
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, int mode, double* wall_ms) {
+    if (!B) return fail(FARCUR_EINVAL, "NULL batch");
+    TRY(validate_opts(opts));
+    if (mode != FARCUR_MODE_PLAIN && mode != FARCUR_MODE_BATCHED) return fail(FARCUR_EINVAL, "bad mode %d", mode);
+    farcur_ctx* ctx = B->ctx;
+    TRY(ctx->select());
+    const farcur_opts o = *opts;
+    B->f32 = o.precision == FARCUR_F32;
+    B->has_result = false;
+    B->rounds.clear();
+    B->kt.clear();
+    B->kt.on = B->profile;
+    for (double& w : B->work) w = 0;
+    for (int b = 0; b < B->nb; ++b) {
+        BState& s = B->st[b];
+        s = BState{};
+        s.m = B->row_off[b + 1] - B->row_off[b];
+        s.n = B->col_off[b + 1] - B->col_off[b];
+        const int D = std::min(s.m, s.n);
+        s.cap = o.max_rank > 0 ? std::min(D, o.max_rank) : D;
+        if (s.cap > 1024) return fail(FARCUR_EINVAL, "block %d: sample cap %d exceeds 1024 (set max_rank)", b, s.cap);
+        s.r = std::min(o.r_start > 0 ? o.r_start : r_start_rule(D), s.cap);
+    }
+    B->arena.reset();
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = ctx->stream;
+    launch_vtest(B->d_v.as<c128>(), B->d_vo.as<int32_t>(), B->nb, o.seed, st);
+    launch_ones(B->d_ya.as<c128>(), (size_t)B->row_off[B->nb], st);
+    HIPTRY(hipGetLastError());
+    if (mode == FARCUR_MODE_BATCHED) {
+        std::vector<int> active(B->nb);
+        for (int b = 0; b < B->nb; ++b) active[b] = b;
+        std::vector<RoundBufs> bufs;
+        std::vector<JobOut> outs;
+        while (!active.empty()) {
+            TRY(batch_round(B, active, o, bufs, outs));
+            std::vector<int> next;
+            next.reserve(active.size());
+            for (size_t i = 0; i < active.size(); ++i) {
+                BState& s = B->st[active[i]];
+                advance(s, outs[i], bufs[i], o.tol);
+                if (s.status == 0) next.push_back(active[i]);
+            }
+            active.swap(next);
+        }
+    } else {
+        // One block at a time, each round synchronised: the reference's per-block loop.
+        for (int b = 0; b < B->nb; ++b) {
+            std::vector<int> one{b};
+            std::vector<RoundBufs> bufs;
+            std::vector<JobOut> outs;
+            while (B->st[b].status == 0) {
+                TRY(batch_round(B, one, o, bufs, outs));
+                advance(B->st[b], outs[0], bufs[0], o.tol);
+            }
+        }
+    }
+    HIPTRY(hipStreamSynchronize(st));
+    const auto t1 = std::chrono::steady_clock::now();
+    if (wall_ms) *wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    if (B->profile) TRY(batch_account(B));
+    B->has_result = true;
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_batch_info(farcur_batch* B, farcur_block_info* infos) {
+    if (!B || !infos) return fail(FARCUR_EINVAL, "NULL argument");
+    if (!B->has_result) return fail(FARCUR_ESTATE, "no batch result");
+    for (int b = 0; b < B->nb; ++b) {
+        const BState& s = B->st[b];
+        farcur_block_info& I = infos[b];
+        I.r = s.fin_r;
+        I.rounds = s.round;
+        I.kept = s.kept;
+        I.converged = s.status == 1;
+        I.err = s.err;
+        I.heldout_err = -1;
+        I.sigma_max = s.smax;
+        I.time_ms = 0;
+    }
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_batch_block(farcur_batch* B, int b, farcur_c128* C, farcur_c128* U, farcur_c128* R,
+                                  int* row_samples, int* col_samples) {
+    if (!B) return fail(FARCUR_EINVAL, "NULL batch");
+    if (!B->has_result) return fail(FARCUR_ESTATE, "no batch result");
+    if (b < 0 || b >= B->nb) return fail(FARCUR_EINVAL, "block %d out of range", b);
+    farcur_ctx* ctx = B->ctx;
+    TRY(ctx->select());
+    const BState& s = B->st[b];
+    const int r = s.fin_r;
+    hipStream_t st = ctx->stream;
+    TRY(copy_panel(C, s.fin.C, (size_t)s.m * r, B->f32, st));
+    TRY(copy_panel(R, s.fin.R, (size_t)r * s.n, B->f32, st));
+    TRY(copy_panel(U, s.fin.U, (size_t)r * r, 0, st));
+    if (row_samples) HIPTRY(hipMemcpyAsync(row_samples, s.fin.rs, r * 4, hipMemcpyDeviceToHost, st));
+    if (col_samples) HIPTRY(hipMemcpyAsync(col_samples, s.fin.cs, r * 4, hipMemcpyDeviceToHost, st));
+    HIPTRY(hipStreamSynchronize(st));
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_batch_kernel_times(farcur_batch* B, const char** names, double* ms, int* launches,
+                                         double* work, int cap) {
+    if (!B) return fail(FARCUR_EINVAL, "NULL batch");
+    static const char* kNames[KTimer::kClasses] = {"sample", "fill", "pinv", "residual"};
+    int n = 0;
+    for (int i = 0; i < KTimer::kClasses && n < cap; ++i, ++n) {
+        if (names) names[n] = kNames[i];
+        if (ms) ms[n] = B->kt.ms[i];
+        if (launches) launches[n] = B->kt.launches[i];
+        if (work) work[n] = B->work[i];
+    }
+    return n;
+}

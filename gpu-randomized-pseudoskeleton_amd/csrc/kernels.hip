@@ -1,0 +1,287 @@
+// Far-field CUR kernels for gfx950: sampling, pivoting, panel fill, core pinv
+// and the CUR residual, all driven by per-round BlockJob lists so that one
+// launch of each serves one block (the per-block drop-in) or every active block
+// of a batch (persistent workgroups striding over the work).
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "curmv.hpp"
+#include "element.hpp"
+#include "jacobi.hpp"
+#include "kernels.hpp"
+#include "philox.hpp"
+
+namespace farcur {
+
+// ---------------------------------------------------------------- sampling
+// Thread (job, kind): kind 0 draws r of the n block columns, kind 1 r of the m
+// block rows, by selection sampling; output is sorted by position in the block
+// and mapped to RWG ids (compres_single_block_spheres_for_sweep2.jl:492-497:
+// sample(1:NN, r, replace=false), sort, r2[samples]).
+__global__ void k_sample(const BlockJob* __restrict__ jobs, int njobs, uint64_t seed) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = t >> 1, kind = t & 1;
+    if (j >= njobs) return;
+    const BlockJob& J = jobs[j];
+    if (J.sample_mode == 1) return;
+    if (kind == 1 && J.sample_mode == 2) return;
+    const uint32_t N = kind ? J.m : J.n;
+    const int32_t* ids = kind ? J.rows : J.cols;
+    int32_t* out = kind ? J.rs : J.cs;
+    const uint32_t purpose = kind ? kStreamRows : kStreamCols;
+    const uint32_t need = J.r;
+    uint32_t got = 0;
+    for (uint32_t t4 = 0; t4 < N && got < need; t4 += 4) {
+        U4 c;
+        c.x = t4 >> 2;
+        c.y = (uint32_t)J.b;
+        c.z = ((uint32_t)J.round << 8) | purpose;
+        c.w = 0x46435552u;
+        const U4 o = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const uint32_t w[4] = {o.x, o.y, o.z, o.w};
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            const uint32_t tt = t4 + l;
+            if (tt < N && got < need && select_step(w[l], N - tt, need - got)) out[got++] = ids[tt];
+        }
+    }
+}
+
+void launch_sample(const BlockJob* jobs, int njobs, uint64_t seed, hipStream_t s) {
+    if (njobs <= 0) return;
+    const int threads = 256;
+    const int grid = (2 * njobs + threads - 1) / threads;
+    hipLaunchKernelGGL(k_sample, dim3(grid), dim3(threads), 0, s, jobs, njobs, seed);
+}
+
+// Rows by pivoting: for every sampled column, the block row of largest |C|
+// (first one on ties), as cur_smart_pinv2
+// (codes/GPU_implementation/seed/compres_single_block_spheres.jl:2870-2887).
+template <typename PC>
+__global__ __launch_bounds__(256) void k_pivot_rows(const BlockJob* __restrict__ jobs, int njobs) {
+    for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
+        const BlockJob& J = jobs[jb];
+        if (J.sample_mode != 2) continue;
+        const PC* C = (const PC*)J.C;
+        const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+        for (int col = w; col < J.r; col += nw) {
+            double best = -1.0;
+            int besti = 0x7fffffff;
+            for (int i = lane; i < J.m; i += kWave) {
+                const c128 a = to_c128(C[(size_t)col * J.m + i]);
+                const double v = a.re * a.re + a.im * a.im;
+                if (v > best) {
+                    best = v;
+                    besti = i;
+                }
+            }
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                const double ob = __shfl_xor(best, o);
+                const int oi = __shfl_xor(besti, o);
+                if (ob > best || (ob == best && oi < besti)) {
+                    best = ob;
+                    besti = oi;
+                }
+            }
+            if (lane == 0) J.rs[col] = J.rows[besti];
+        }
+    }
+}
+
+void launch_pivot_rows(const BlockJob* jobs, int njobs, int f32, hipStream_t s) {
+    if (njobs <= 0) return;
+    const int grid = njobs < 4096 ? njobs : 4096;
+    if (f32)
+        hipLaunchKernelGGL(k_pivot_rows<c64>, dim3(grid), dim3(256), 0, s, jobs, njobs);
+    else
+        hipLaunchKernelGGL(k_pivot_rows<c128>, dim3(grid), dim3(256), 0, s, jobs, njobs);
+}
+
+// ---------------------------------------------------------------- fill
+__host__ __device__ static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+int fill_tiles(int m, int n, int r, int panels) {
+    int t = 0;
+    if (panels & kPanelC) t += cdiv(m, kTileRows) * cdiv(r, kTileCols);
+    if (panels & kPanelW) t += cdiv(r, kTileRows) * cdiv(r, kTileCols);
+    if (panels & kPanelR) t += cdiv(r, kTileRows) * cdiv(n, kTileCols);
+    return t;
+}
+
+template <typename T>
+__device__ __forceinline__ FillConsts<T> make_consts(double k, double eta);
+template <>
+__device__ __forceinline__ FillConsts<double> make_consts<double>(double k, double eta) {
+    return FillConsts<double>{k, 4.0 / (k * k), k * eta / (144.0 * M_PI)};
+}
+template <>
+__device__ __forceinline__ FillConsts<float> make_consts<float>(double k, double eta) {
+    return FillConsts<float>{(float)(k / (2.0 * M_PI)), (float)(4.0 / (k * k)), (float)(k * eta / (144.0 * M_PI))};
+}
+
+// Persistent fill over the tiles of all jobs. T = double: C, W, R in c128.
+// T = float: C, R in c64 and W widened to c128 for the FP64 pinv.
+template <typename T, typename PC>
+__global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restrict__ jobs, int njobs,
+                                                       const int32_t* __restrict__ prefix, int total, int panels,
+                                                       const T* __restrict__ rec, int ne, double k, double eta) {
+    __shared__ T s_col[kTileCols][kRecFields];
+    const FillConsts<T> fc = make_consts<T>(k, eta);
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        int lo = 0, hi = njobs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        const BlockJob& J = jobs[lo];
+        int local = t - prefix[lo];
+        const int m = J.m, n = J.n, r = J.r;
+        if (panels & kPanelC) {
+            const int mt = cdiv(m, kTileRows), nC = mt * cdiv(r, kTileCols);
+            if (local < nC) {
+                fill_tile<T, PC>((PC*)J.C, m, J.rows, m, J.cs, r, (local % mt) * kTileRows,
+                                 (local / mt) * kTileCols, rec, ne, fc, s_col);
+                continue;
+            }
+            local -= nC;
+        }
+        const int rt = cdiv(r, kTileRows);
+        if (panels & kPanelW) {
+            const int nW = rt * cdiv(r, kTileCols);
+            if (local < nW) {
+                fill_tile<T, c128>(J.W, r, J.rs, r, J.cs, r, (local % rt) * kTileRows, (local / rt) * kTileCols,
+                                   rec, ne, fc, s_col);
+                continue;
+            }
+            local -= nW;
+        }
+        fill_tile<T, PC>((PC*)J.R, r, J.rs, r, J.cols, n, (local % rt) * kTileRows, (local / rt) * kTileCols, rec,
+                         ne, fc, s_col);
+    }
+}
+
+void launch_fill(const BlockJob* jobs, int njobs, const int32_t* tile_prefix, int total_tiles, int panels,
+                 const FillParams& fp, int grid, hipStream_t s) {
+    if (njobs <= 0 || total_tiles <= 0) return;
+    if (grid <= 0 || grid > total_tiles) grid = total_tiles;
+    if (fp.f32)
+        hipLaunchKernelGGL((k_fill<float, c64>), dim3(grid), dim3(kFillThreads), 0, s, jobs, njobs, tile_prefix,
+                           total_tiles, panels, (const float*)fp.rec, fp.ne, fp.k, fp.eta);
+    else
+        hipLaunchKernelGGL((k_fill<double, c128>), dim3(grid), dim3(kFillThreads), 0, s, jobs, njobs, tile_prefix,
+                           total_tiles, panels, (const double*)fp.rec, fp.ne, fp.k, fp.eta);
+}
+
+template <typename T, typename PC>
+__global__ __launch_bounds__(kFillThreads) void k_fill_panel(PC* __restrict__ Z, int ld,
+                                                             const int32_t* __restrict__ rows, int nr,
+                                                             const int32_t* __restrict__ cols, int nc,
+                                                             const T* __restrict__ rec, int ne, double k, double eta) {
+    __shared__ T s_col[kTileCols][kRecFields];
+    const FillConsts<T> fc = make_consts<T>(k, eta);
+    fill_tile<T, PC>(Z, ld, rows, nr, cols, nc, blockIdx.x * kTileRows, blockIdx.y * kTileCols, rec, ne, fc,
+                     s_col);
+}
+
+void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32_t* cols, int nc,
+                       const FillParams& fp, hipStream_t s) {
+    if (nr <= 0 || nc <= 0) return;
+    const dim3 grid(cdiv(nr, kTileRows), cdiv(nc, kTileCols));
+    if (fp.f32)
+        hipLaunchKernelGGL((k_fill_panel<float, c64>), grid, dim3(kFillThreads), 0, s, (c64*)Z, ld, rows, nr, cols,
+                           nc, (const float*)fp.rec, fp.ne, fp.k, fp.eta);
+    else
+        hipLaunchKernelGGL((k_fill_panel<double, c128>), grid, dim3(kFillThreads), 0, s, (c128*)Z, ld, rows, nr,
+                           cols, nc, (const double*)fp.rec, fp.ne, fp.k, fp.eta);
+}
+
+// ---------------------------------------------------------------- pinv
+template <bool kLds>
+__global__ void k_pinv(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist, double atol,
+                       double rtol, JobOut* __restrict__ out) {
+    constexpr int kMaxR = kLds ? kPinvLdsMaxR : 1;
+    __shared__ c128 s_a[kMaxR * kMaxR];
+    __shared__ c128 s_v[kMaxR * kMaxR];
+    __shared__ double s_w[kLds ? kPinvLdsMaxR : 1024];
+    __shared__ int s_flag;
+    for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
+        const int jb = list[li];
+        const BlockJob& J = jobs[jb];
+        const int r = J.r;
+        c128* A;
+        c128* V;
+        if (kLds) {
+            for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) s_a[idx] = J.W[idx];
+            A = s_a;
+            V = s_v;
+        } else {
+            A = J.W;
+            V = J.V;
+        }
+        int kept;
+        double smax;
+        jacobi_pinv(A, V, r, atol, rtol, J.U, r, J.sigma, &s_flag, s_w, &kept, &smax);
+        if (threadIdx.x == 0) {
+            out[jb].kept = kept;
+            out[jb].sigma_max = smax;
+        }
+        __syncthreads();
+    }
+}
+
+void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, int lds, int threads,
+                 JobOut* out, hipStream_t s) {
+    if (nlist <= 0) return;
+    const int grid = nlist < 8192 ? nlist : 8192;
+    if (lds)
+        hipLaunchKernelGGL(k_pinv<true>, dim3(grid), dim3(threads), 0, s, jobs, list, nlist, atol, rtol, out);
+    else
+        hipLaunchKernelGGL(k_pinv<false>, dim3(grid), dim3(threads), 0, s, jobs, list, nlist, atol, rtol, out);
+}
+
+// ---------------------------------------------------------------- residual
+constexpr int kResidMaxR = 1024;
+
+template <typename PC>
+__global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, JobOut* __restrict__ out) {
+    __shared__ c128 s_t1[kResidMaxR];
+    __shared__ c128 s_t2[kResidMaxR];
+    __shared__ c128 s_part[1024];
+    __shared__ double s_red[16];
+    for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
+        const BlockJob& J = jobs[jb];
+        const double err = cur_residual<PC>((const PC*)J.C, J.m, J.r, J.U, (const PC*)J.R, J.n, J.v, J.y_old,
+                                            J.y_new, s_t1, s_t2, s_part, s_red);
+        if (threadIdx.x == 0) out[jb].err = err;
+        __syncthreads();
+    }
+}
+
+void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, JobOut* out, hipStream_t s) {
+    if (njobs <= 0) return;
+    const int grid = njobs < 8192 ? njobs : 8192;
+    if (f32)
+        hipLaunchKernelGGL(k_residual<c64>, dim3(grid), dim3(threads), 0, s, jobs, njobs, out);
+    else
+        hipLaunchKernelGGL(k_residual<c128>, dim3(grid), dim3(threads), 0, s, jobs, njobs, out);
+}
+
+// ---------------------------------------------------------------- test vectors
+__global__ void k_vtest(c128* __restrict__ v, const int32_t* __restrict__ offsets, int nblocks, uint64_t seed) {
+    for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const int o = offsets[b], n = offsets[b + 1] - o;
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            double re, im;
+            vtest_entry(seed, (uint32_t)b, (uint32_t)j, &re, &im);
+            v[o + j] = c128{re, im};
+        }
+    }
+}
+
+void launch_vtest(c128* v, const int32_t* offsets, int nblocks, uint64_t seed, hipStream_t s) {
+    if (nblocks <= 0) return;
+    const int grid = nblocks < 4096 ? nblocks : 4096;
+    hipLaunchKernelGGL(k_vtest, dim3(grid), dim3(256), 0, s, v, offsets, nblocks, seed);
+}
+
+}  // namespace farcur

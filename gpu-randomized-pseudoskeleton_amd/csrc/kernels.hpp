@@ -1,0 +1,74 @@
+// Work descriptors and launchers of the far-field CUR kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace farcur {
+
+// One block in one round of the adaptive CUR. Every pointer is device memory.
+// Layouts (column-major, as the reference's panels, CU:1738-1871):
+//   C  m x r (ld m): rows = block rows, cols = sampled cols
+//   W  r x r (ld r): rows = sampled rows, cols = sampled cols (destroyed by the pinv)
+//   R  r x n (ld r): rows = sampled rows, cols = block cols
+//   U  r x r (ld r): W^+
+struct BlockJob {
+    int32_t b;      // block id (selects the RNG streams)
+    int32_t m, n, r, round;
+    int32_t sample_mode;  // 0: draw cols+rows, 1: samples given, 2: draw cols, rows by argmax |C|
+    int32_t pad0, pad1;
+    const int32_t* rows;  // m RWG ids
+    const int32_t* cols;  // n RWG ids
+    int32_t* cs;          // r sampled col ids (RWG ids)
+    int32_t* rs;          // r sampled row ids (RWG ids)
+    void* C;              // c128 or c64 (panel precision)
+    void* R;
+    c128* W;
+    c128* V;  // Jacobi scratch when r exceeds the LDS variant
+    c128* U;
+    double* sigma;        // r
+    const c128* v;        // n test vector
+    const c128* y_old;    // m
+    c128* y_new;          // m
+};
+
+// Per-job results written by the kernels.
+struct JobOut {
+    double err;       // successive residual ||y_old - y|| / ||y||
+    double sigma_max; // largest singular value of W
+    int32_t kept;     // singular values kept by the threshold
+    int32_t pad;
+};
+
+struct FillParams {
+    const void* rec;  // RWG records (double or float SoA, kRecFields x ne)
+    int32_t ne;
+    int32_t f32;      // 1: FP32 fill, panels c64; 0: FP64 fill, panels c128
+    double k, eta;
+};
+
+// Tiles of the fill for a job of size (m, n, r) restricted to `panels`.
+int fill_tiles(int m, int n, int r, int panels);
+
+void launch_sample(const BlockJob* jobs, int njobs, uint64_t seed, hipStream_t s);
+void launch_pivot_rows(const BlockJob* jobs, int njobs, int f32, hipStream_t s);
+// Fill of C and R (and W unless the rows were pivoted: then W = C[rows, :] is gathered).
+// panels: bit 0 = C, bit 1 = W, bit 2 = R. tile_prefix: njobs + 1 device ints
+// (tiles of job j are [tile_prefix[j], tile_prefix[j+1])).
+enum : int { kPanelC = 1, kPanelW = 2, kPanelR = 4 };
+void launch_fill(const BlockJob* jobs, int njobs, const int32_t* tile_prefix, int total_tiles, int panels,
+                 const FillParams& fp, int grid, hipStream_t s);
+// Pinv of W for the listed jobs (indices into jobs). lds = 1 needs r <= kPinvLdsMaxR.
+constexpr int kPinvLdsMaxR = 44;
+void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, int lds,
+                 int threads, JobOut* out, hipStream_t s);
+void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, JobOut* out, hipStream_t s);
+
+// Plain panel fill Z[i + j*ld] = Z(rows[i], cols[j]) (standalone, one panel).
+void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32_t* cols, int nc,
+                       const FillParams& fp, hipStream_t s);
+// v[offsets[b] + j] = test-vector entry j of block b, j < offsets[b+1] - offsets[b].
+void launch_vtest(c128* v, const int32_t* offsets, int nblocks, uint64_t seed, hipStream_t s);
+
+}  // namespace farcur

@@ -1,0 +1,504 @@
+"""Python host of libfarcur.so (ctypes), mirroring the reference's Julia driver.
+
+The reference drives its GPU CUR call from Julia
+(codes/GPU_implementation/seed/compres_single_block_spheres_for_sweep2.jl):
+``pure_cuda_compression2`` (:484-670) samples, converts the geometry tables and
+``ccall``s ``cuda_complete_compression``; ``cuda_cur_compress3`` (:878-973)
+runs the adaptive loop. Julia is not available here, so this module restates
+those two functions over the same C ABI (``cuda_cur_compress3`` and
+``pure_cuda_compression2`` below) and wraps the native API (meshes, contexts,
+batches). It needs the compiled library and raises if it is missing: there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import pathlib
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libfarcur.so"
+
+F64, F32 = 0, 1
+ROWS_UNIFORM, ROWS_PIVOT = 0, 1
+MODE_PLAIN, MODE_BATCHED, MODE_LEARNED = 0, 1, 2
+
+_lib = None
+
+
+class FarcurError(RuntimeError):
+    pass
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FarcurError(f"{LIB_PATH} is not built (run __graft_entry__.build() or make -C {HERE})")
+        _lib = ctypes.CDLL(str(LIB_PATH))
+        _setup(_lib)
+    return _lib
+
+
+vp, ci, cd, cu64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint64
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [
+        ("tol", cd),
+        ("pinv_tol", cd),
+        ("pinv_rtol", cd),
+        ("seed", cu64),
+        ("precision", ci),
+        ("row_mode", ci),
+        ("r_start", ci),
+        ("max_rank", ci),
+        ("block_id", ci),
+        ("heldout", ci),
+    ]
+
+
+class BlockInfo(ctypes.Structure):
+    _fields_ = [
+        ("r", ci),
+        ("rounds", ci),
+        ("kept", ci),
+        ("converged", ci),
+        ("err", cd),
+        ("heldout_err", cd),
+        ("sigma_max", cd),
+        ("time_ms", cd),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def _setup(L: ctypes.CDLL) -> None:
+    L.farcur_last_error.restype = ctypes.c_char_p
+    L.farcur_version.restype = ctypes.c_char_p
+    L.farcur_opts_default.argtypes = [vp]
+    L.farcur_mesh_plate.argtypes = [cd, cd, ci, ci, cd, cd, cd, vp]
+    L.farcur_mesh_octasphere.argtypes = [cd, ci, vp]
+    L.farcur_mesh_icosphere.argtypes = [cd, cd, cd, ci, cd, cu64, vp]
+    L.farcur_mesh_append.argtypes = [vp, vp, cd, cd, cd]
+    L.farcur_mesh_build_rwg.argtypes = [vp]
+    L.farcur_mesh_sizes.argtypes = [vp, vp, vp, vp]
+    L.farcur_mesh_tables.argtypes = [vp] * 9
+    L.farcur_mesh_destroy.argtypes = [vp]
+    L.farcur_mesh_destroy.restype = None
+    L.farcur_partition_build.argtypes = [vp, cd, cd, vp]
+    L.farcur_partition_sizes.argtypes = [vp, vp, vp, vp, vp]
+    L.farcur_partition_tables.argtypes = [vp] * 7
+    L.farcur_partition_destroy.argtypes = [vp]
+    L.farcur_partition_destroy.restype = None
+    L.farcur_create.argtypes = [ci, vp]
+    L.farcur_destroy.argtypes = [vp]
+    L.farcur_destroy.restype = None
+    L.farcur_set_em.argtypes = [vp, cd, cd]
+    L.farcur_set_geometry.argtypes = [vp, vp, ci, vp, ci, vp, ci, vp]
+    L.farcur_set_mesh.argtypes = [vp, vp]
+    L.farcur_fill.argtypes = [vp, vp, ci, vp, ci, ci, vp]
+    L.farcur_pinv.argtypes = [vp, vp, ci, cd, cd, vp, vp, vp]
+    L.farcur_cur_residual.argtypes = [vp, vp, ci, ci, vp, vp, ci, vp, vp, vp, vp]
+    L.farcur_sample.argtypes = [vp, cu64, ci, ci, ci, ci, ci, vp]
+    L.farcur_vtest.argtypes = [cu64, ci, ci, vp]
+    L.farcur_pivot_rows.argtypes = [vp, vp, ci, ci, vp]
+    L.farcur_compress_block.argtypes = [vp, vp, ci, vp, ci, vp, vp]
+    L.farcur_block_result.argtypes = [vp] * 7
+    L.cuda_complete_compression.restype = cd
+    L.cuda_complete_compression.argtypes = (
+        [vp, vp, vp, ci, vp, cd, vp, vp, vp, vp, vp] + [ci] * 12 + [vp, ci, cd, cd, cd, cd, cd, ci, ci] + [vp] * 10
+        + [ci] * 18 + [ci] * 6
+    )
+    if hasattr(L, "farcur_batch_create"):
+        L.farcur_batch_create.argtypes = [vp, ci, vp, vp, vp, vp, vp]
+        L.farcur_batch_destroy.argtypes = [vp]
+        L.farcur_batch_destroy.restype = None
+        L.farcur_batch_compress.argtypes = [vp, vp, ci, vp]
+        L.farcur_batch_info.argtypes = [vp, vp]
+        L.farcur_batch_block.argtypes = [vp, ci, vp, vp, vp, vp, vp]
+        L.farcur_batch_profile.argtypes = [vp, ci]
+        L.farcur_batch_kernel_times.argtypes = [vp, vp, vp, vp, vp, ci]
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise FarcurError(f"farcur error {rc}: {lib().farcur_last_error().decode()}")
+
+
+def _p(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    lib().farcur_opts_default(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise TypeError(f"unknown option {k}")
+        setattr(o, k, v)
+    return o
+
+
+# ---------------------------------------------------------------- geometry
+class Mesh:
+    """Surface mesh + RWG tables (get_edge.m layout)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def plate(cls, lx, ly, nx, ny, x0=0.0, y0=0.0, z0=0.0):
+        h = vp()
+        check(lib().farcur_mesh_plate(lx, ly, nx, ny, x0, y0, z0, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def octasphere(cls, radius, n_levels):
+        h = vp()
+        check(lib().farcur_mesh_octasphere(radius, n_levels, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def icosphere(cls, ax, ay, az, level, jitter=0.0, seed=1):
+        h = vp()
+        check(lib().farcur_mesh_icosphere(ax, ay, az, level, jitter, seed, ctypes.byref(h)))
+        return cls(h)
+
+    def append(self, other: "Mesh", dx=0.0, dy=0.0, dz=0.0) -> "Mesh":
+        check(lib().farcur_mesh_append(self._h, other._h, dx, dy, dz))
+        return self
+
+    def build_rwg(self) -> "Mesh":
+        check(lib().farcur_mesh_build_rwg(self._h))
+        return self
+
+    def sizes(self):
+        nv, nt, ne = ci(), ci(), ci()
+        check(lib().farcur_mesh_sizes(self._h, ctypes.byref(nv), ctypes.byref(nt), ctypes.byref(ne)))
+        return nv.value, nt.value, ne.value
+
+    def tables(self) -> dict:
+        """Reference layout: 1-based ids as float64, column-major 3 x N etc."""
+        nv, nt, ne = self.sizes()
+        out = {
+            "vertex": np.empty((nv, 3)),
+            "topol": np.empty((nt, 3)),
+            "trian": np.empty((nt, 3)),
+            "edges": np.empty((ne, 4)),
+            "un": np.empty((nt, 3)),
+            "ds": np.empty(nt),
+            "ln": np.empty(ne),
+            "cent": np.empty((nt, 3)),
+        }
+        check(lib().farcur_mesh_tables(self._h, *[_p(out[k]) for k in
+                                                   ("vertex", "topol", "trian", "edges", "un", "ds", "ln", "cent")]))
+        for k in ("vertex", "topol", "trian", "edges", "un", "cent"):
+            out[k] = np.asfortranarray(out[k].T)
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.farcur_mesh_destroy(self._h)
+            self._h = None
+
+
+def plates_config(side=1.0, nsq=20, zsep=4.0) -> Mesh:
+    """Two parallel square plates (normal +z) centred at z = 0 and z = zsep."""
+    m = Mesh.plate(side, side, nsq, nsq, 0, 0, 0)
+    m.append(Mesh.plate(side, side, nsq, nsq, 0, 0, 0), 0, 0, zsep)
+    return m.build_rwg()
+
+
+def two_octaspheres(radius, n_levels, d) -> Mesh:
+    """object_wrapper_spheres_interaction2.m: two sphere.m spheres, the second shifted by d along x."""
+    m = Mesh.octasphere(radius, n_levels)
+    m.append(Mesh.octasphere(radius, n_levels), d, 0, 0)
+    return m.build_rwg()
+
+
+class Partition:
+    def __init__(self, mesh: Mesh, leaf_size=0.5, eta_adm=2.0):
+        self._h = vp()
+        check(lib().farcur_partition_build(mesh._h, leaf_size, eta_adm, ctypes.byref(self._h)))
+        ng, nr, nf, nn = ci(), ci(), ci(), ci()
+        check(lib().farcur_partition_sizes(self._h, ctypes.byref(ng), ctypes.byref(nr), ctypes.byref(nf),
+                                           ctypes.byref(nn)))
+        self.group_offsets = np.empty(ng.value + 1, dtype=np.int32)
+        self.group_rwg = np.empty(nr.value, dtype=np.int32)
+        self.centre = np.empty((ng.value, 3))
+        self.diam = np.empty(ng.value)
+        self.far_pairs = np.empty((nf.value, 2), dtype=np.int32)
+        self.near_pairs = np.empty((nn.value, 2), dtype=np.int32)
+        check(lib().farcur_partition_tables(self._h, _p(self.group_offsets), _p(self.group_rwg), _p(self.centre),
+                                            _p(self.diam), _p(self.far_pairs), _p(self.near_pairs)))
+
+    @property
+    def ngroups(self):
+        return self.diam.size
+
+    def group(self, g):
+        return self.group_rwg[self.group_offsets[g]:self.group_offsets[g + 1]]
+
+    def far_blocks(self):
+        """(row_off, row_ids, col_off, col_ids) of the admissible pairs (a, b), a < b."""
+        rows, cols, ro, co = [], [], [0], [0]
+        for a, b in self.far_pairs:
+            ra, cb = self.group(a), self.group(b)
+            rows.append(ra)
+            cols.append(cb)
+            ro.append(ro[-1] + ra.size)
+            co.append(co[-1] + cb.size)
+        cat = lambda xs: np.concatenate(xs).astype(np.int32) if xs else np.zeros(0, np.int32)
+        return np.array(ro, np.int32), cat(rows), np.array(co, np.int32), cat(cols)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.farcur_partition_destroy(self._h)
+            self._h = None
+
+
+# ---------------------------------------------------------------- context
+def _cvec(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _cmat(a) -> np.ndarray:
+    return np.asfortranarray(a, dtype=np.complex128)
+
+
+def _ivec(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Context:
+    def __init__(self, device=0):
+        self._h = vp()
+        check(lib().farcur_create(device, ctypes.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().farcur_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def set_em(self, k, eta=120 * math.pi):
+        check(lib().farcur_set_em(self._h, k, eta))
+
+    def set_mesh(self, mesh: Mesh):
+        check(lib().farcur_set_mesh(self._h, mesh._h))
+
+    def set_geometry(self, vertex, topol, edges, ln):
+        v = np.asfortranarray(vertex, dtype=np.float64)
+        t = np.asfortranarray(topol, dtype=np.float64)
+        e = np.asfortranarray(edges, dtype=np.float64)
+        l = np.ascontiguousarray(ln, dtype=np.float64)
+        check(lib().farcur_set_geometry(self._h, _p(v), v.shape[1], _p(t), t.shape[1], _p(e), e.shape[1], _p(l)))
+
+    def fill(self, rows, cols, precision=F64) -> np.ndarray:
+        rows, cols = _ivec(rows), _ivec(cols)
+        Z = np.empty((rows.size, cols.size), dtype=np.complex128, order="F")
+        check(lib().farcur_fill(self._h, _p(rows), rows.size, _p(cols), cols.size, precision, _p(Z)))
+        return Z
+
+    def pinv(self, W, atol=1e-10, rtol=0.0):
+        W = _cmat(W)
+        r = W.shape[0]
+        U = np.empty((r, r), dtype=np.complex128, order="F")
+        s = np.empty(r)
+        kept = ci()
+        check(lib().farcur_pinv(self._h, _p(W), r, atol, rtol, _p(U), _p(s), ctypes.byref(kept)))
+        return U, s, kept.value
+
+    def cur_residual(self, C, U, R, v, y_old):
+        C, U, R, v, y_old = _cmat(C), _cmat(U), _cmat(R), _cvec(v), _cvec(y_old)
+        m, r = C.shape
+        n = R.shape[1]
+        y = np.empty(m, dtype=np.complex128)
+        err = cd()
+        check(lib().farcur_cur_residual(self._h, _p(C), m, r, _p(U), _p(R), n, _p(v), _p(y_old), _p(y),
+                                        ctypes.byref(err)))
+        return y, err.value
+
+    def sample(self, seed, block, rnd, purpose, N, r) -> np.ndarray:
+        out = np.empty(r, dtype=np.int32)
+        check(lib().farcur_sample(self._h, seed, block, rnd, purpose, N, r, _p(out)))
+        return out
+
+    def pivot_rows(self, C) -> np.ndarray:
+        C = _cmat(C)
+        m, r = C.shape
+        out = np.empty(r, dtype=np.int32)
+        check(lib().farcur_pivot_rows(self._h, _p(C), m, r, _p(out)))
+        return out
+
+    def compress_block(self, rows, cols, **opts):
+        rows, cols = _ivec(rows), _ivec(cols)
+        o = default_opts(**opts)
+        info = BlockInfo()
+        check(lib().farcur_compress_block(self._h, _p(rows), rows.size, _p(cols), cols.size, ctypes.byref(o),
+                                          ctypes.byref(info)))
+        m, n, r = rows.size, cols.size, info.r
+        C = np.empty((m, r), dtype=np.complex128, order="F")
+        U = np.empty((r, r), dtype=np.complex128, order="F")
+        R = np.empty((r, n), dtype=np.complex128, order="F")
+        rs = np.empty(r, dtype=np.int32)
+        cs = np.empty(r, dtype=np.int32)
+        y = np.empty(m, dtype=np.complex128)
+        check(lib().farcur_block_result(self._h, _p(C), _p(U), _p(R), _p(rs), _p(cs), _p(y)))
+        return {"C": C, "U": U, "R": R, "rs": rs, "cs": cs, "y": y, **info.as_dict()}
+
+
+def vtest(seed, block, n) -> np.ndarray:
+    v = np.empty(n, dtype=np.complex128)
+    check(lib().farcur_vtest(seed, block, n, _p(v)))
+    return v
+
+
+# ---------------------------------------------------------------- batches
+class Batch:
+    def __init__(self, ctx: Context, row_off, row_ids, col_off, col_ids):
+        self.ctx = ctx
+        self.row_off, self.row_ids = _ivec(row_off), _ivec(row_ids)
+        self.col_off, self.col_ids = _ivec(col_off), _ivec(col_ids)
+        self.nblocks = self.row_off.size - 1
+        self._h = vp()
+        check(lib().farcur_batch_create(ctx._h, self.nblocks, _p(self.row_off), _p(self.row_ids), _p(self.col_off),
+                                        _p(self.col_ids), ctypes.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().farcur_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def compress(self, mode=MODE_BATCHED, **opts) -> float:
+        o = default_opts(**opts)
+        wall = cd()
+        check(lib().farcur_batch_compress(self._h, ctypes.byref(o), mode, ctypes.byref(wall)))
+        return wall.value
+
+    def info(self) -> list[dict]:
+        arr = (BlockInfo * self.nblocks)()
+        check(lib().farcur_batch_info(self._h, arr))
+        return [a.as_dict() for a in arr]
+
+    def info_arrays(self) -> dict:
+        infos = self.info()
+        return {k: np.array([d[k] for d in infos]) for k in infos[0]} if infos else {}
+
+    def block(self, b):
+        m = int(self.row_off[b + 1] - self.row_off[b])
+        n = int(self.col_off[b + 1] - self.col_off[b])
+        r = self.info()[b]["r"]
+        C = np.empty((m, r), dtype=np.complex128, order="F")
+        U = np.empty((r, r), dtype=np.complex128, order="F")
+        R = np.empty((r, n), dtype=np.complex128, order="F")
+        rs = np.empty(r, dtype=np.int32)
+        cs = np.empty(r, dtype=np.int32)
+        check(lib().farcur_batch_block(self._h, b, _p(C), _p(U), _p(R), _p(rs), _p(cs)))
+        return {"C": C, "U": U, "R": R, "rs": rs, "cs": cs}
+
+    def profile(self, enable=True):
+        check(lib().farcur_batch_profile(self._h, 1 if enable else 0))
+
+    def kernel_times(self) -> dict:
+        cap = 8
+        names = (ctypes.c_char_p * cap)()
+        ms = np.zeros(cap)
+        launches = np.zeros(cap, dtype=np.int32)
+        evals = np.zeros(cap)
+        n = lib().farcur_batch_kernel_times(self._h, names, _p(ms), _p(launches), _p(evals), cap)
+        return {names[i].decode(): {"ms": float(ms[i]), "launches": int(launches[i]), "work": float(evals[i])}
+                for i in range(n)}
+
+
+# ---------------------------------------------------------------- reference driver, restated
+@dataclass
+class EMData:
+    """EM_data_struct (compres_single_block_spheres_for_sweep2.jl:28-39)."""
+    k: float = 2 * math.pi
+    eta: float = 120 * math.pi
+    field: int = 1
+    Rint_s: float = 0.0
+    Rint_f: float = 0.0
+    Ranal_s: float = 0.0
+    corr_solid: int = 0
+    flag: int = 0
+
+
+def pure_cuda_compression2(vtest_, Avtest_old, current_size, pinv_tol, r1, r2, obj: dict, EM: EMData, rng):
+    """One CUR round through the drop-in symbol, as the reference's
+    pure_cuda_compression2 (compres_single_block_spheres_for_sweep2.jl:484-670):
+    uniform sorted samples without replacement (StatsBase.sample, :492-495),
+    mapped to the block's global 1-based ids (:496), tables as Float64 (:513-545).
+    r1, r2 are 1-based RWG ids."""
+    r1 = np.asarray(r1)
+    r2 = np.asarray(r2)
+    MM, NN, cur = r1.size, r2.size, int(current_size)
+    col_samples = np.sort(rng.choice(NN, cur, replace=False))
+    row_samples = np.sort(rng.choice(MM, cur, replace=False))
+    Ccol = np.ascontiguousarray(r2[col_samples], dtype=np.float64)
+    Crow = np.ascontiguousarray(r1[row_samples], dtype=np.float64)
+    tabs = {k: np.asfortranarray(obj[k], dtype=np.float64) for k in
+            ("vertex", "topol", "trian", "edges", "un", "cent")}
+    ds = np.ascontiguousarray(obj["ds"], dtype=np.float64)
+    ln = np.ascontiguousarray(obj["ln"], dtype=np.float64)
+    r1d = np.ascontiguousarray(r1, dtype=np.float64)
+    r2d = np.ascontiguousarray(r2, dtype=np.float64)
+    C = np.zeros((MM, cur), dtype=np.complex128, order="F")
+    U = np.zeros((cur, cur), dtype=np.complex128, order="F")
+    R = np.zeros((cur, NN), dtype=np.complex128, order="F")
+    v = _cvec(vtest_)
+    yo = _cvec(Avtest_old)
+    yn = np.zeros(MM, dtype=np.complex128)
+    t = np.zeros(1)
+    Z = np.zeros(1, dtype=np.complex128)
+    d = 8  # divfactor
+    V, T, Tr, E, Un, Ce = (tabs[k] for k in ("vertex", "topol", "trian", "edges", "un", "cent"))
+    err = lib().cuda_complete_compression(
+        _p(C), _p(U), _p(R), cur, _p(t), pinv_tol, _p(yo), _p(yn), _p(v), _p(Ccol), _p(Crow),
+        -(-MM // d), -(-cur // d), d, d, -(-cur // d), -(-cur // d), d, d, -(-cur // d), -(-NN // d), d, d,
+        _p(Z), EM.field, EM.k, EM.eta, EM.Rint_s, EM.Ranal_s, EM.Rint_f, EM.corr_solid, EM.flag,
+        _p(r1d), _p(r2d), _p(V), _p(T), _p(Tr), _p(E), _p(Un), _p(ds), _p(ln), _p(Ce),
+        E.shape[1], MM, NN, V.shape[0], V.shape[1], T.shape[0], T.shape[1], T.shape[0], T.shape[1],
+        E.shape[0], E.shape[1], Un.shape[0], Un.shape[1], 1, ds.size, ln.size, Ce.shape[0], Ce.shape[1],
+        MM, NN, 1, 1, MM, NN)
+    if math.isnan(err):
+        raise FarcurError(lib().farcur_last_error().decode())
+    return C, U, R, Crow, Ccol, yn, err, float(t[0])
+
+
+def cuda_cur_compress3(r1, r2, tol, obj: dict, EM: EMData, rng, pinv_tol=1e-10):
+    """The reference's adaptive loop (compres_single_block_spheres_for_sweep2.jl:878-973)."""
+    M, N = len(r1), len(r2)
+    vt = rng.random(N) + 1j * rng.random(N)
+    D = min(M, N)
+    cur = 1 if D < 10 else D // 10
+    y_old = np.ones(M, dtype=np.complex128)
+    C, U, R, rs, cs, y_new, _, t = pure_cuda_compression2(vt, y_old, cur, pinv_tol, r1, r2, obj, EM, rng)
+    time_total = t
+    y_old = y_new.copy()
+    cur = 2 * cur if 2 * cur < D else D
+    err_app, flag_size = math.inf, 0
+    while err_app > tol and flag_size == 0:
+        C, U, R, rs, cs, y_new, err_app, t = pure_cuda_compression2(vt, y_old, cur, pinv_tol, r1, r2, obj, EM, rng)
+        time_total += t
+        y_old = y_new.copy()
+        if cur > D:
+            raise FarcurError("current_size larger than D")
+        elif cur == D:
+            flag_size = 1
+        else:
+            cur = 2 * cur if 2 * cur <= D else D
+    return C, U, R, rs, cs, err_app, time_total

@@ -1,0 +1,241 @@
+/*
+ * farcur — MI355X (gfx950) randomized pseudoskeleton (CUR) compression of the
+ * far-field blocks of the RWG/EFIE impedance matrix. C ABI.
+ *
+ * Every function returns FARCUR_OK (0) on success or a nonzero status, with a
+ * message available from farcur_last_error() (per host thread), except where
+ * stated. Host pointers unless stated. Matrices are column-major complex128
+ * (interleaved re, im), the layout of Julia's Matrix{ComplexF64}.
+ *
+ * Reference interfaces replaced (paths under the reference repository):
+ *   CU  = codes/GPU_implementation/seed/user_impedance_for_julia_cuda.cu
+ *   SW2 = codes/GPU_implementation/seed/compres_single_block_spheres_for_sweep2.jl
+ */
+#ifndef FARCUR_H
+#define FARCUR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    double re, im;
+} farcur_c128;
+
+enum {
+    FARCUR_OK = 0,
+    FARCUR_EINVAL = 1, /* bad argument (sizes, ids out of range, ...) */
+    FARCUR_EHIP = 2,   /* HIP runtime error or no usable GPU */
+    FARCUR_ENOMEM = 3, /* device allocation failed */
+    FARCUR_ESTATE = 4  /* call out of order (no geometry, no result, ...) */
+};
+
+/* Message of the last failed call on this host thread ("" if none). */
+const char* farcur_last_error(void);
+const char* farcur_version(void);
+
+/* ------------------------------------------------------------------ drop-in
+ * Same symbol, argument list and semantics as the reference's
+ * cuda_complete_compression (CU:1564-1629, body CU:1630-2012), as called by
+ * pure_cuda_compression2 (SW2:591-660): one CUR round of the block r1 x r2 with
+ * the caller's samples. Fills C = Z[r1, col_samples] (n_r1 x cur_order),
+ * W = Z[row_samples, col_samples], R = Z[row_samples, r2] (cur_order x n_r2),
+ * U = pinv(W) with absolute threshold pinv_tol, Avtest_new = C (U (R vtest)),
+ * and returns ||Avtest_old - Avtest_new|| / ||Avtest_new||.
+ * col_samples, row_samples, r1, r2 hold 1-based RWG ids stored as double
+ * (the reference declares int* but receives Float64, CU:1696-1697, SW2:593).
+ * vertex/topol/trian/edges/un/ds/ln/cent are the get_edge.m tables (1-based
+ * ids as double). The launch-shape arguments (C/U/R blocks and threads,
+ * blocks_x.., limx, limy) and Ztotal are accepted and ignored: this library
+ * chooses its own launch shapes. field must be 1 (EFIE); Rinteg_s, Ranal_s,
+ * Rinteg_f, cor_solid and flag are ignored (far-field blocks use quadrature
+ * only). vec_time_cuda[0] receives the device time in ms of the round.
+ * On error returns NaN, sets vec_time_cuda[0] = -1 and farcur_last_error().
+ */
+double cuda_complete_compression(farcur_c128* Ccompressed, farcur_c128* Ucompressed, farcur_c128* Rcompressed,
+                                 int cur_order, double* vec_time_cuda, double pinv_tol, farcur_c128* Avtest_old,
+                                 farcur_c128* Avtest_new, farcur_c128* vtest, double* col_samples,
+                                 double* row_samples, int C_blocks_x, int C_blocks_y, int C_threads_x,
+                                 int C_threads_y, int U_blocks_x, int U_blocks_y, int U_threads_x, int U_threads_y,
+                                 int R_blocks_x, int R_blocks_y, int R_threads_x, int R_threads_y,
+                                 farcur_c128* Ztotal, int field_arg, double k_arg, double eta_arg,
+                                 double Rinteg_s_arg, double Ranal_s_arg, double Rinteg_f_arg, int cor_solid_arg,
+                                 int flag_arg, double* r1_arg, double* r2_arg, double* vertex_arg,
+                                 double* topol_arg, double* trian_arg, double* edges_arg, double* un_arg,
+                                 double* ds_arg, double* ln_arg, double* cent_arg, int N_arg, int n_r1_arg,
+                                 int n_r2_arg, int rows_vertex_arg, int cols_vertex_arg, int rows_topol_arg,
+                                 int cols_topol_arg, int rows_trian_arg, int cols_trian_arg, int rows_edges_arg,
+                                 int cols_edges_arg, int rows_un_arg, int cols_un_arg, int rows_ds_arg,
+                                 int cols_ds_arg, int n_ln_arg, int rows_cent_arg, int cols_cent_arg, int blocks_x,
+                                 int blocks_y, int threads_x, int threads_y, int limx, int limy);
+
+/* ------------------------------------------------------------------ meshes
+ * Host-side geometry producing the get_edge.m tables
+ * (codes/GPU_implementation/seed/get_edge.m:16-121), which the reference builds
+ * in Matlab (sphere.m, sq_plate.m) and passes through obj_struct (SW2:12-26).
+ */
+typedef struct farcur_mesh farcur_mesh;
+
+/* Plate lx x ly in the plane z = z0 centred at (x0, y0), nx x ny squares
+ * split into right triangles along one diagonal, normal +z. */
+int farcur_mesh_plate(double lx, double ly, int nx, int ny, double x0, double y0, double z0, farcur_mesh** out);
+/* Octahedral sphere, 12 * 4^n_levels edges, numbering of sphere.m
+ * (codes/ACA_vs_CUR/seed/objects/sphere.m:13-53). */
+int farcur_mesh_octasphere(double radius, int n_levels, farcur_mesh** out);
+/* Icosphere (20 * 4^level triangles) scaled to semi-axes (ax, ay, az), radial
+ * vertex jitter U(-jitter, jitter) x mean edge length from `seed`. */
+int farcur_mesh_icosphere(double ax, double ay, double az, int level, double jitter, uint64_t seed,
+                          farcur_mesh** out);
+/* Appends src translated by (dx, dy, dz) to dst (RWG tables must be rebuilt). */
+int farcur_mesh_append(farcur_mesh* dst, const farcur_mesh* src, double dx, double dy, double dz);
+/* Builds the RWG tables (get_edge.m semantics; junctions are rejected). */
+int farcur_mesh_build_rwg(farcur_mesh* mesh);
+int farcur_mesh_sizes(const farcur_mesh* mesh, int* nv, int* nt, int* ne);
+/* Copies the tables in the reference layout, ids 1-based stored as double:
+ * vertex 3 x nv, topol 3 x nt, trian 3 x nt, edges 4 x ne, un 3 x nt, ds nt,
+ * ln ne, cent 3 x nt. Any pointer may be NULL. */
+int farcur_mesh_tables(const farcur_mesh* mesh, double* vertex, double* topol, double* trian, double* edges,
+                       double* un, double* ds, double* ln, double* cent);
+void farcur_mesh_destroy(farcur_mesh* mesh);
+
+/* Octree leaf groups of side leaf_size and the admissible (far) pairs
+ * |c_a - c_b| >= eta_adm * max(diam_a, diam_b); the Matlab Z_comp partition
+ * of the reference multi-block loop (codes/GPU_implementation/seed/
+ * compres_single_block_spheres.jl:2539-2696, comp/self flags) is not shipped,
+ * this builds the equivalent block list. */
+typedef struct farcur_partition farcur_partition;
+int farcur_partition_build(const farcur_mesh* mesh, double leaf_size, double eta_adm, farcur_partition** out);
+int farcur_partition_sizes(const farcur_partition* p, int* ngroups, int* nrwg, int* nfar, int* nnear);
+/* group_offsets ngroups+1, group_rwg nrwg (0-based RWG ids), centre 3 x ngroups,
+ * diam ngroups, far_pairs 2 x nfar, near_pairs 2 x nnear (0-based group ids). */
+int farcur_partition_tables(const farcur_partition* p, int* group_offsets, int* group_rwg, double* centre,
+                            double* diam, int* far_pairs, int* near_pairs);
+void farcur_partition_destroy(farcur_partition* p);
+
+/* ------------------------------------------------------------------ context
+ * A context owns one GPU stream and the device-resident geometry (uploaded
+ * once, unlike the per-call upload of CU:1711-1720). Calls on different
+ * contexts may run concurrently; calls on one context must not overlap. */
+typedef struct farcur_ctx farcur_ctx;
+
+int farcur_create(int device, farcur_ctx** out);
+void farcur_destroy(farcur_ctx* ctx);
+/* Wavenumber and impedance (EM_data_struct k, eta, SW2:28-39). Default k = 2 pi, eta = 120 pi. */
+int farcur_set_em(farcur_ctx* ctx, double k, double eta);
+/* Geometry from the reference tables (1-based ids as double, as SW2:513-545
+ * converts them): vertex 3 x nv, topol 3 x nt, edges 4 x ne, ln ne. */
+int farcur_set_geometry(farcur_ctx* ctx, const double* vertex, int nv, const double* topol, int nt,
+                        const double* edges, int ne, const double* ln);
+int farcur_set_mesh(farcur_ctx* ctx, const farcur_mesh* mesh);
+
+enum { FARCUR_F64 = 0, FARCUR_F32 = 1 };
+
+/* Panel fill: Z[i + j*nr] = Z(rows[i], cols[j]) for 0-based RWG ids, the role of
+ * impedance_matrix_compute_elbyel3 (CU:353-439) / impedance_matrix_cuda_elbyel4
+ * (CU:1022-1267). precision FARCUR_F32 computes in complex FP32 and widens. */
+int farcur_fill(farcur_ctx* ctx, const int* rows, int nr, const int* cols, int nc, int precision, farcur_c128* Z);
+
+/* Pseudo-inverse of an r x r matrix W with singular values below
+ * max(atol, rtol * sigma_max) dropped (CU:1873-1924 with kernel_Sinv CU:1435,
+ * my_pinv SW2:1013-1022). sigma (r, descending) may be NULL; *kept receives the
+ * number of singular values kept (may be NULL). */
+int farcur_pinv(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, double rtol, farcur_c128* U,
+                double* sigma, int* kept);
+
+/* y_new = C (U (R v)), returns err = ||y_old - y_new|| / ||y_new|| in *err
+ * (CU:1928-1966). C m x r, U r x r, R r x n. */
+int farcur_cur_residual(farcur_ctx* ctx, const farcur_c128* C, int m, int r, const farcur_c128* U,
+                        const farcur_c128* R, int n, const farcur_c128* v, const farcur_c128* y_old,
+                        farcur_c128* y_new, double* err);
+
+/* Uniform sorted sample of r of the indices 0..N-1 for stream
+ * (seed, block, round, purpose) (SW2:492-497, StatsBase.sample replace=false
+ * then sort); purpose 1 = columns, 2 = rows. */
+int farcur_sample(farcur_ctx* ctx, uint64_t seed, int block, int round, int purpose, int N, int r, int* out);
+
+/* Test vector entries j < n of stream (seed, block): U[0,1) + i U[0,1)
+ * (rand(N) + im*rand(N), SW2:888). Host computation, no GPU needed. */
+int farcur_vtest(uint64_t seed, int block, int n, farcur_c128* v);
+
+/* Row pivots of an m x r panel C: rows[j] = argmax_i |C[i, j]| (first on ties),
+ * cur_smart_pinv2 (codes/GPU_implementation/seed/compres_single_block_spheres.jl:2870-2887). */
+int farcur_pivot_rows(farcur_ctx* ctx, const farcur_c128* C, int m, int r, int* rows);
+
+/* ------------------------------------------------------------------ adaptive CUR */
+enum {
+    FARCUR_ROWS_UNIFORM = 0, /* rows sampled like the columns (reference) */
+    FARCUR_ROWS_PIVOT = 1    /* rows = argmax |C| per sampled column */
+};
+
+typedef struct {
+    double tol;      /* successive-residual tolerance (reference experiments: 1e-1, SW2:1155) */
+    double pinv_tol; /* absolute singular value threshold (reference: 1e-10, SW2:884) */
+    double pinv_rtol;/* relative threshold (x sigma_max); 0 = reference behaviour */
+    uint64_t seed;   /* RNG key for samples and test vectors */
+    int precision;   /* FARCUR_F64 or FARCUR_F32 fill */
+    int row_mode;    /* FARCUR_ROWS_UNIFORM or FARCUR_ROWS_PIVOT */
+    int r_start;     /* 0: reference rule r0 = D < 10 ? 1 : D / 10 (SW2:889-894) */
+    int max_rank;    /* cap on the sample count (0: min(m, n), the reference cap) */
+    int block_id;    /* stream id of a single-block call */
+    int heldout;     /* held-out rows checked after convergence (0: none) */
+} farcur_opts;
+
+void farcur_opts_default(farcur_opts* opts);
+
+typedef struct {
+    int r;          /* sample count of the final round */
+    int rounds;     /* rounds run */
+    int kept;       /* singular values kept in the final core */
+    int converged;  /* 1 if err <= tol, 0 if stopped at r = D or max_rank */
+    double err;     /* final successive residual */
+    double heldout_err; /* relative Frobenius error on held-out rows (-1 if not checked) */
+    double sigma_max;   /* largest singular value of the final W */
+    double time_ms;     /* device time of all rounds */
+} farcur_block_info;
+
+/* The adaptive loop of cuda_cur_compress3 (SW2:878-973) on one block, rows x
+ * cols (0-based RWG ids), every round on the GPU with device sampling. The
+ * result stays on the device until farcur_block_result. */
+int farcur_compress_block(farcur_ctx* ctx, const int* rows, int m, const int* cols, int n, const farcur_opts* opts,
+                          farcur_block_info* info);
+/* Copies the last compress_block result: C m x r, U r x r, R r x n, samples
+ * (0-based RWG ids, r each), y (m) = C U R v. Any pointer may be NULL. */
+int farcur_block_result(farcur_ctx* ctx, farcur_c128* C, farcur_c128* U, farcur_c128* R, int* row_samples,
+                        int* col_samples, farcur_c128* y);
+
+/* ------------------------------------------------------------------ batches
+ * Many blocks (the far-field block list) compressed together. Block b has rows
+ * row_ids[row_off[b] .. row_off[b+1]) and cols col_ids[col_off[b] .. ). */
+typedef struct farcur_batch farcur_batch;
+
+enum {
+    FARCUR_MODE_PLAIN = 0,   /* per-block host loop, one block at a time (drop-in semantics) */
+    FARCUR_MODE_BATCHED = 1  /* every active block advances one round per batched launch set */
+};
+
+int farcur_batch_create(farcur_ctx* ctx, int nblocks, const int* row_off, const int* row_ids, const int* col_off,
+                        const int* col_ids, farcur_batch** out);
+void farcur_batch_destroy(farcur_batch* batch);
+/* Compresses every block; *wall_ms receives the host wall time of the call
+ * (synchronised), may be NULL. */
+int farcur_batch_compress(farcur_batch* batch, const farcur_opts* opts, int mode, double* wall_ms);
+/* Per-block results (nblocks entries). */
+int farcur_batch_info(farcur_batch* batch, farcur_block_info* infos);
+/* Copies block b's C, U, R (any may be NULL). */
+int farcur_batch_block(farcur_batch* batch, int b, farcur_c128* C, farcur_c128* U, farcur_c128* R, int* row_samples,
+                       int* col_samples);
+
+/* Device time (ms) and launch count of each kernel class over the last
+ * batch_compress call with profiling on: names "sample", "fill", "pinv",
+ * "residual". Returns the number of classes written (<= cap). */
+int farcur_batch_profile(farcur_batch* batch, int enable);
+int farcur_batch_kernel_times(farcur_batch* batch, const char** names, double* ms, int* launches,
+                              double* evals, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FARCUR_H */

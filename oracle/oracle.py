@@ -1,0 +1,245 @@
+"""ORACLE — test infrastructure only (tests/, __graft_entry__.smoke(), bench.py's
+cpu_baseline leg). ctypes front end of oracle.c, the CPU FP64 restatement of
+the reference's far-field CUR path; see oracle.c's header for the reference
+file:line each function follows and the pins that check it.
+
+Geometry is passed as the tables of the reference's object struct
+(codes/GPU_implementation/seed/compres_single_block_spheres_for_sweep2.jl:12-26):
+``vertex`` 3 x Nv, ``topol`` 3 x Nt and ``edges`` 4 x Ne (1-based ids as
+float64, the layout Julia hands to the C library) and ``ln``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "liborcfarcur.so"
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        _lib = ctypes.CDLL(str(LIB_PATH))
+        _setup(_lib)
+    return _lib
+
+
+class _Geom(ctypes.Structure):
+    _fields_ = [
+        ("vertex", ctypes.c_void_p),
+        ("topol", ctypes.c_void_p),
+        ("edges", ctypes.c_void_p),
+        ("ln", ctypes.c_void_p),
+        ("ne", ctypes.c_int),
+        ("k", ctypes.c_double),
+        ("eta", ctypes.c_double),
+    ]
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [
+        ("tol", ctypes.c_double),
+        ("pinv_tol", ctypes.c_double),
+        ("pinv_rtol", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("row_mode", ctypes.c_int),
+        ("r_start", ctypes.c_int),
+        ("max_rank", ctypes.c_int),
+        ("block_id", ctypes.c_int),
+    ]
+
+
+class _Info(ctypes.Structure):
+    _fields_ = [
+        ("r", ctypes.c_int),
+        ("rounds", ctypes.c_int),
+        ("kept", ctypes.c_int),
+        ("converged", ctypes.c_int),
+        ("err", ctypes.c_double),
+    ]
+
+
+class _Block(ctypes.Structure):
+    _fields_ = [
+        ("g", ctypes.c_void_p),
+        ("rows", ctypes.c_void_p),
+        ("cols", ctypes.c_void_p),
+        ("dense", ctypes.c_void_p),
+        ("m", ctypes.c_int),
+        ("n", ctypes.c_int),
+    ]
+
+
+def _setup(L: ctypes.CDLL) -> None:
+    vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    L.orc_element.argtypes = [vp, i, i, vp, vp]
+    L.orc_fill.argtypes = [vp, vp, i, vp, i, vp]
+    L.orc_block_matvec.argtypes = [vp, vp, i, vp, i, vp, vp]
+    L.orc_pinv.argtypes = [vp, i, d, d, vp, vp]
+    L.orc_pinv.restype = i
+    L.orc_residual.argtypes = [vp, i, i, vp, vp, i, vp, vp, vp]
+    L.orc_residual.restype = d
+    L.orc_philox.argtypes = [vp, vp, vp]
+    L.orc_sample.argtypes = [ctypes.c_uint64, i, i, i, i, i, vp]
+    L.orc_vtest.argtypes = [ctypes.c_uint64, i, i, vp]
+    L.orc_pivot_rows.argtypes = [vp, i, i, vp]
+    L.orc_compress_block.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.orc_compress_block.restype = i
+    L.orc_num_threads.restype = i
+    L.orc_set_threads.argtypes = [i]
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class Geometry:
+    """Reference tables -> oracle geometry (0-based int32 ids internally)."""
+
+    def __init__(self, vertex, topol, edges, ln, k=2 * np.pi, eta=120 * np.pi):
+        self.vertex = np.ascontiguousarray(np.asarray(vertex, dtype=np.float64).T)  # nv x 3
+        self.topol = np.ascontiguousarray(np.asarray(topol).T.astype(np.int32) - 1)
+        self.edges = np.ascontiguousarray(np.asarray(edges).T.astype(np.int32) - 1)
+        self.ln = np.ascontiguousarray(np.asarray(ln, dtype=np.float64).ravel())
+        self.ne = self.ln.size
+        self.k = float(k)
+        self.eta = float(eta)
+        self._g = _Geom(_ptr(self.vertex), _ptr(self.topol), _ptr(self.edges), _ptr(self.ln), self.ne, self.k, self.eta)
+
+    @property
+    def ptr(self):
+        return ctypes.addressof(self._g)
+
+    def element(self, m: int, n: int) -> complex:
+        re, im = ctypes.c_double(), ctypes.c_double()
+        lib().orc_element(self.ptr, int(m), int(n), ctypes.byref(re), ctypes.byref(im))
+        return complex(re.value, im.value)
+
+    def fill(self, rows, cols) -> np.ndarray:
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        Z = np.empty((cols.size, rows.size), dtype=np.complex128)  # column-major storage
+        lib().orc_fill(self.ptr, _ptr(rows), rows.size, _ptr(cols), cols.size, _ptr(Z))
+        return Z.T
+
+    def block_matvec(self, rows, cols, v) -> np.ndarray:
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        v = np.ascontiguousarray(v, dtype=np.complex128)
+        y = np.empty(rows.size, dtype=np.complex128)
+        lib().orc_block_matvec(self.ptr, _ptr(rows), rows.size, _ptr(cols), cols.size, _ptr(v), _ptr(y))
+        return y
+
+
+def _fortran(a: np.ndarray) -> np.ndarray:
+    return np.asfortranarray(a, dtype=np.complex128)
+
+
+def pinv(W: np.ndarray, atol: float = 1e-10, rtol: float = 0.0):
+    """Returns (U, sigma (column order), kept)."""
+    W = _fortran(W)
+    r = W.shape[0]
+    U = np.empty((r, r), dtype=np.complex128, order="F")
+    sig = np.empty(r, dtype=np.float64)
+    kept = lib().orc_pinv(_ptr(W), r, atol, rtol, _ptr(U), _ptr(sig))
+    return U, sig, kept
+
+
+def residual(C, U, R, v, y_old):
+    C, U, R = _fortran(C), _fortran(U), _fortran(R)
+    v = np.ascontiguousarray(v, dtype=np.complex128)
+    y_old = np.ascontiguousarray(y_old, dtype=np.complex128)
+    m, r = C.shape
+    n = R.shape[1]
+    y = np.empty(m, dtype=np.complex128)
+    err = lib().orc_residual(_ptr(C), m, r, _ptr(U), _ptr(R), n, _ptr(v), _ptr(y_old), _ptr(y))
+    return y, err
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    o = np.empty(4, dtype=np.uint32)
+    lib().orc_philox(_ptr(c), _ptr(k), _ptr(o))
+    return o
+
+
+def sample(seed: int, block: int, rnd: int, purpose: int, N: int, r: int) -> np.ndarray:
+    out = np.empty(r, dtype=np.int32)
+    lib().orc_sample(seed, block, rnd, purpose, N, r, _ptr(out))
+    return out
+
+
+def vtest(seed: int, block: int, n: int) -> np.ndarray:
+    v = np.empty(n, dtype=np.complex128)
+    lib().orc_vtest(seed, block, n, _ptr(v))
+    return v
+
+
+def pivot_rows(C) -> np.ndarray:
+    C = _fortran(C)
+    m, r = C.shape
+    out = np.empty(r, dtype=np.int32)
+    lib().orc_pivot_rows(_ptr(C), m, r, _ptr(out))
+    return out
+
+
+def compress_block(geom: Geometry | None, rows=None, cols=None, dense=None, *, tol=1e-4, pinv_tol=1e-10,
+                   pinv_rtol=0.0, seed=12345, row_mode=0, r_start=0, max_rank=0, block_id=0):
+    """cuda_cur_compress3 restated. Returns dict with C, U, R, rs, cs (block positions), y, info."""
+    if geom is not None:
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        m, n = rows.size, cols.size
+        blk = _Block(geom.ptr, _ptr(rows), _ptr(cols), None, m, n)
+    else:
+        dense = _fortran(dense)
+        m, n = dense.shape
+        blk = _Block(None, None, None, _ptr(dense), m, n)
+    D = min(m, n)
+    cap = max_rank if 0 < max_rank < D else D
+    C = np.empty(m * cap, dtype=np.complex128)
+    U = np.empty(cap * cap, dtype=np.complex128)
+    R = np.empty(cap * n, dtype=np.complex128)
+    rs = np.empty(cap, dtype=np.int32)
+    cs = np.empty(cap, dtype=np.int32)
+    y = np.empty(m, dtype=np.complex128)
+    o = _Opts(tol, pinv_tol, pinv_rtol, seed, row_mode, r_start, max_rank, block_id)
+    info = _Info()
+    lib().orc_compress_block(ctypes.addressof(blk), ctypes.addressof(o), _ptr(C), _ptr(U), _ptr(R), _ptr(rs),
+                             _ptr(cs), _ptr(y), ctypes.addressof(info))
+    r = info.r
+    return {
+        "C": C[: m * r].reshape((r, m)).T,
+        "U": U[: r * r].reshape((r, r)).T,
+        "R": R[: r * n].reshape((n, r)).T,
+        "rs": rs[:r].copy(),
+        "cs": cs[:r].copy(),
+        "y": y,
+        "r": r,
+        "rounds": info.rounds,
+        "kept": info.kept,
+        "converged": bool(info.converged),
+        "err": info.err,
+    }
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))

@@ -1,0 +1,90 @@
+"""Batched far-field compression (all admissible octree blocks of a sphere)
+against the per-block path and the oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+K, ETA = 2 * np.pi, 120 * np.pi
+
+
+@pytest.fixture(scope="module")
+def sphere(farcur, oracle):
+    # Radius 1 lambda icosphere, level 3 (1920 RWGs), 0.5 lambda leaves.
+    mesh = farcur.Mesh.icosphere(1.0, 1.0, 1.0, 3, 0.01, 1).build_rwg()
+    part = farcur.Partition(mesh, 0.5, 2.0)
+    t = mesh.tables()
+    ctx = farcur.Context(0)
+    ctx.set_mesh(mesh)
+    g = oracle.Geometry(t["vertex"], t["topol"], t["edges"], t["ln"], K, ETA)
+    blocks = part.far_blocks()
+    yield ctx, g, part, blocks
+    ctx.close()
+
+
+def _relfro(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def test_partition_has_far_blocks(sphere):
+    _, _, part, (ro, ri, co, cix) = sphere
+    assert len(part.far_pairs) > 20
+
+
+@pytest.mark.parametrize("row_mode", [0, 1])
+def test_batched_equals_plain(farcur, sphere, row_mode):
+    ctx, g, part, (ro, ri, co, cix) = sphere
+    b1 = farcur.Batch(ctx, ro, ri, co, cix)
+    b2 = farcur.Batch(ctx, ro, ri, co, cix)
+    b1.compress(farcur.MODE_PLAIN, tol=1e-4, seed=9, row_mode=row_mode)
+    b2.compress(farcur.MODE_BATCHED, tol=1e-4, seed=9, row_mode=row_mode)
+    i1, i2 = b1.info_arrays(), b2.info_arrays()
+    for k in ("r", "rounds", "kept", "converged"):
+        np.testing.assert_array_equal(i1[k], i2[k])
+    np.testing.assert_array_equal(i1["err"], i2["err"])
+    for b in range(0, b1.nblocks, max(1, b1.nblocks // 7)):
+        x, y = b1.block(b), b2.block(b)
+        for k in ("C", "U", "R", "rs", "cs"):
+            np.testing.assert_array_equal(x[k], y[k])
+
+
+def test_batch_matches_oracle_and_dense(farcur, sphere, oracle):
+    ctx, g, part, (ro, ri, co, cix) = sphere
+    bt = farcur.Batch(ctx, ro, ri, co, cix)
+    bt.compress(farcur.MODE_BATCHED, tol=1e-4, seed=21)
+    info = bt.info_arrays()
+    worst = 0.0
+    for b in range(0, bt.nblocks, max(1, bt.nblocks // 12)):
+        rows, cols = ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]]
+        ref = oracle.compress_block(g, rows, cols, tol=1e-4, seed=21, block_id=b)
+        out = bt.block(b)
+        assert (info["r"][b], info["rounds"][b]) == (ref["r"], ref["rounds"])
+        np.testing.assert_array_equal(out["cs"], cols[ref["cs"]])
+        np.testing.assert_array_equal(out["rs"], rows[ref["rs"]])
+        A = out["C"] @ out["U"] @ out["R"]
+        assert _relfro(A, ref["C"] @ ref["U"] @ ref["R"]) <= 1e-8
+        Z = g.fill(rows, cols)
+        worst = max(worst, _relfro(A, Z))
+    assert worst <= 1e-3
+
+
+def test_batch_f32_accuracy(farcur, sphere):
+    ctx, g, part, (ro, ri, co, cix) = sphere
+    bt = farcur.Batch(ctx, ro, ri, co, cix)
+    bt.compress(farcur.MODE_BATCHED, tol=1e-4, seed=5, precision=farcur.F32, pinv_rtol=1e-6)
+    errs = []
+    for b in range(0, bt.nblocks, max(1, bt.nblocks // 10)):
+        rows, cols = ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]]
+        out = bt.block(b)
+        errs.append(_relfro(out["C"] @ out["U"] @ out["R"], g.fill(rows, cols)))
+    assert max(errs) <= 1e-3
+
+
+def test_batch_profile_accounting(farcur, sphere):
+    ctx, g, part, (ro, ri, co, cix) = sphere
+    bt = farcur.Batch(ctx, ro, ri, co, cix)
+    bt.profile(True)
+    bt.compress(farcur.MODE_BATCHED, tol=1e-4, seed=5)
+    kt = bt.kernel_times()
+    assert set(kt) == {"sample", "fill", "pinv", "residual"}
+    assert kt["fill"]["ms"] > 0 and kt["fill"]["work"] > 0 and kt["fill"]["launches"] >= 2
