@@ -83,6 +83,7 @@ __global__ __launch_bounds__(256) void k_pivot_rows(const BlockJob* __restrict__
                     besti = oi;
                 }
             }
+            if (besti >= J.m) besti = 0;  // all entries NaN: keep a valid row
             if (lane == 0) J.rs[col] = J.rows[besti];
         }
     }
