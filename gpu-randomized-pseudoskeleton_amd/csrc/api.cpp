@@ -906,8 +906,10 @@ int block_loop(farcur_ctx* ctx, const BlockLoopIO& io, const farcur_opts& o, All
         }
         err = res.err;
         ++round;
+        // The reference loop runs while err > tol: a NaN residual also ends it,
+        // but is not reported as converged.
         if (!(err > o.tol)) {
-            converged = true;
+            converged = err <= o.tol;
             break;
         }
         int rn;
@@ -1169,7 +1171,7 @@ namespace {
 
 struct BState {
     int m = 0, n = 0, cap = 0, r = 0, round = 0;
-    int status = 0;  // 0 active, 1 converged, 2 stopped at the cap
+    int status = 0;  // 0 active, 1 converged, 2 stopped at the cap, 3 NaN residual
     int ycur = 0;
     int kept = 0;
     double err = INFINITY, smax = 0;
@@ -1278,7 +1280,7 @@ void advance(BState& s, const JobOut& out, const RoundBufs& rb, double tol) {
     s.err = out.err;
     s.round += 1;
     if (!(s.err > tol)) {
-        s.status = 1;
+        s.status = s.err <= tol ? 1 : 3;  // 3: stopped on a NaN residual
         return;
     }
     int rn;

@@ -450,7 +450,7 @@ int orc_compress_block(const orc_block* b, const orc_opts* o, double* Cout, doub
         err = e;
         ++round;
         if (!(err > o->tol)) {
-            converged = 1;
+            converged = err <= o->tol;
             break;
         }
         if (r >= cap) break;
