@@ -195,7 +195,7 @@ struct farcur_ctx {
     double eta = 120.0 * M_PI;
     int ne = 0, nt = 0;
     std::vector<int32_t> tri_of_rwg;  // 2 * ne: T+, T- (host, for accounting)
-    DBuf rec64, rec32;
+    DBuf rec64, rec32, cen;
     uint64_t geom_hash = 0;
     // single-block scratch
     DBuf d_rows, d_cols, d_cs, d_rs, d_C, d_R, d_W, d_V, d_U, d_sigma, d_v, d_y0, d_y1, d_plan, d_out;
@@ -210,6 +210,7 @@ struct farcur_ctx {
     FillParams fill_params(int f32) const {
         FillParams fp;
         fp.rec = f32 ? rec32.p : rec64.p;
+        fp.cen = cen.as<double>();
         fp.ne = ne;
         fp.f32 = f32;
         fp.k = k;
@@ -222,7 +223,8 @@ namespace {
 
 // RWG records (kRecFields x ne SoA) from the reference tables, 0-based ids.
 void build_records(const std::vector<double>& vertex, const std::vector<int32_t>& topol,
-                   const std::vector<int32_t>& edges, const std::vector<double>& ln, std::vector<double>& rec) {
+                   const std::vector<int32_t>& edges, const std::vector<double>& ln, std::vector<double>& rec,
+                   std::vector<float>& rec32, std::vector<double>& cen) {
     const int ne = (int)ln.size();
     rec.assign((size_t)kRecFields * ne, 0.0);
     const double b[3][3] = {{2.0 / 3, 1.0 / 6, 1.0 / 6}, {1.0 / 6, 2.0 / 3, 1.0 / 6}, {1.0 / 6, 1.0 / 6, 2.0 / 3}};
@@ -243,6 +245,22 @@ void build_records(const std::vector<double>& vertex, const std::vector<int32_t>
             }
         }
         rec[(size_t)36 * ne + e] = ln[e];
+    }
+    // FP32 records: quadrature points as offsets from the RWG centre (mean of
+    // the six points), centre kept in FP64.
+    rec32.assign(rec.begin(), rec.end());
+    cen.assign(3 * (size_t)ne, 0.0);
+    for (int e = 0; e < ne; ++e) {
+        for (int c = 0; c < 3; ++c) {
+            double m = 0;
+            for (int h = 0; h < 2; ++h)
+                for (int i = 0; i < 3; ++i) m += rec[(size_t)(h * 9 + i * 3 + c) * ne + e];
+            m /= 6.0;
+            cen[(size_t)c * ne + e] = m;
+            for (int h = 0; h < 2; ++h)
+                for (int i = 0; i < 3; ++i)
+                    rec32[(size_t)(h * 9 + i * 3 + c) * ne + e] = (float)(rec[(size_t)(h * 9 + i * 3 + c) * ne + e] - m);
+        }
     }
 }
 
@@ -281,12 +299,14 @@ int set_geometry_impl(farcur_ctx* ctx, const std::vector<double>& vertex, const 
     }
     for (double x : vertex)
         if (!std::isfinite(x)) return fail(FARCUR_EINVAL, "non-finite vertex coordinate");
-    std::vector<double> rec;
-    build_records(vertex, topol, edges, ln, rec);
-    std::vector<float> rec32(rec.begin(), rec.end());
+    std::vector<double> rec, cen;
+    std::vector<float> rec32;
+    build_records(vertex, topol, edges, ln, rec, rec32, cen);
     TRY(ctx->select());
     TRY(ctx->rec64.ensure(rec.size() * sizeof(double)));
     TRY(ctx->rec32.ensure(rec32.size() * sizeof(float)));
+    TRY(ctx->cen.ensure(cen.size() * sizeof(double)));
+    HIPTRY(hipMemcpyAsync(ctx->cen.p, cen.data(), cen.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIPTRY(hipMemcpyAsync(ctx->rec64.p, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIPTRY(hipMemcpyAsync(ctx->rec32.p, rec32.data(), rec32.size() * sizeof(float), hipMemcpyHostToDevice,
                           ctx->stream));
@@ -588,7 +608,7 @@ void farcur_destroy(farcur_ctx* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    DBuf* bufs[] = {&ctx->rec64, &ctx->rec32, &ctx->d_rows, &ctx->d_cols, &ctx->d_cs,    &ctx->d_rs,
+    DBuf* bufs[] = {&ctx->rec64, &ctx->rec32, &ctx->cen, &ctx->d_rows, &ctx->d_cols, &ctx->d_cs,    &ctx->d_rs,
                     &ctx->d_C,   &ctx->d_R,   &ctx->d_W,    &ctx->d_V,    &ctx->d_U,     &ctx->d_sigma,
                     &ctx->d_v,   &ctx->d_y0,  &ctx->d_y1,   &ctx->d_plan, &ctx->d_out};
     for (DBuf* b : bufs) b->release();
@@ -656,12 +676,12 @@ int farcur_fill(farcur_ctx* ctx, const int* rows, int nr, const int* cols, int n
 int farcur_pinv(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, double rtol, farcur_c128* U,
                 double* sigma, int* kept) {
     if (!ctx || !W || !U) return fail(FARCUR_EINVAL, "NULL argument");
-    if (r <= 0 || r > 1024) return fail(FARCUR_EINVAL, "r = %d outside [1, 1024]", r);
+    if (r <= 0 || r > kMaxSamples) return fail(FARCUR_EINVAL, "r = %d outside [1, %d]", r, kMaxSamples);
     if (!(atol >= 0) || !(rtol >= 0)) return fail(FARCUR_EINVAL, "thresholds must be >= 0");
     TRY(ctx->select());
     const size_t rr = (size_t)r * r * sizeof(c128);
     TRY(ctx->d_W.ensure(rr));
-    TRY(ctx->d_V.ensure(rr));
+    TRY(ctx->d_V.ensure(pinv_v_elems(r) * sizeof(c128)));
     TRY(ctx->d_U.ensure(rr));
     TRY(ctx->d_sigma.ensure(r * sizeof(double)));
     HIPTRY(hipMemcpyAsync(ctx->d_W.p, W, rr, hipMemcpyHostToDevice, ctx->stream));
@@ -698,7 +718,7 @@ int farcur_cur_residual(farcur_ctx* ctx, const farcur_c128* C, int m, int r, con
                         const farcur_c128* R, int n, const farcur_c128* v, const farcur_c128* y_old,
                         farcur_c128* y_new, double* err) {
     if (!ctx || !C || !U || !R || !v || !y_old || !y_new || !err) return fail(FARCUR_EINVAL, "NULL argument");
-    if (m <= 0 || n <= 0 || r <= 0 || r > 1024) return fail(FARCUR_EINVAL, "bad sizes m=%d n=%d r=%d", m, n, r);
+    if (m <= 0 || n <= 0 || r <= 0 || r > kMaxSamples) return fail(FARCUR_EINVAL, "bad sizes m=%d n=%d r=%d", m, n, r);
     TRY(ctx->select());
     TRY(ctx->d_C.ensure((size_t)m * r * 16));
     TRY(ctx->d_U.ensure((size_t)r * r * 16));
@@ -706,6 +726,7 @@ int farcur_cur_residual(farcur_ctx* ctx, const farcur_c128* C, int m, int r, con
     TRY(ctx->d_v.ensure((size_t)n * 16));
     TRY(ctx->d_y0.ensure((size_t)m * 16));
     TRY(ctx->d_y1.ensure((size_t)m * 16));
+    TRY(ctx->d_W.ensure((size_t)std::max(r * r, 2 * r) * 16));
     hipStream_t s = ctx->stream;
     HIPTRY(hipMemcpyAsync(ctx->d_C.p, C, (size_t)m * r * 16, hipMemcpyHostToDevice, s));
     HIPTRY(hipMemcpyAsync(ctx->d_U.p, U, (size_t)r * r * 16, hipMemcpyHostToDevice, s));
@@ -722,6 +743,7 @@ int farcur_cur_residual(farcur_ctx* ctx, const farcur_c128* C, int m, int r, con
     plan.job.C = ctx->d_C.p;
     plan.job.R = ctx->d_R.p;
     plan.job.U = ctx->d_U.as<c128>();
+    plan.job.W = ctx->d_W.as<c128>();
     plan.job.v = ctx->d_v.as<c128>();
     plan.job.y_old = ctx->d_y0.as<c128>();
     plan.job.y_new = ctx->d_y1.as<c128>();
@@ -837,7 +859,7 @@ int block_loop(farcur_ctx* ctx, const BlockLoopIO& io, const farcur_opts& o, All
                KTimer& kt, farcur_block_info* info, RoundBufs* final_bufs, int* final_y) {
     const int D = std::min(io.m, io.n);
     const int cap = o.max_rank > 0 ? std::min(D, o.max_rank) : D;
-    if (cap > 1024) return fail(FARCUR_EINVAL, "sample count cap %d exceeds 1024 (set max_rank)", cap);
+    if (cap > kMaxSamples) return fail(FARCUR_EINVAL, "sample count cap %d exceeds %d (set max_rank)", cap, kMaxSamples);
     int r = o.r_start > 0 ? std::min(o.r_start, cap) : std::min(r_start_rule(D), cap);
     const int f32 = o.precision == FARCUR_F32;
     int round = 0, ycur = 0;
@@ -960,7 +982,7 @@ extern "C" int farcur_compress_block(farcur_ctx* ctx, const int* rows, int m, co
         TRY(ctx->d_C.ensure((size_t)m * r * psize(f32)));
         TRY(ctx->d_R.ensure((size_t)r * n * psize(f32)));
         TRY(ctx->d_W.ensure((size_t)r * r * 16));
-        TRY(ctx->d_V.ensure((size_t)r * r * 16));
+        TRY(ctx->d_V.ensure(pinv_v_elems(r) * 16));
         TRY(ctx->d_U.ensure((size_t)r * r * 16));
         TRY(ctx->d_sigma.ensure(r * 8));
         rb = RoundBufs{ctx->d_cs.as<int32_t>(), ctx->d_rs.as<int32_t>(), ctx->d_C.p,         ctx->d_R.p,
@@ -1053,7 +1075,7 @@ int compat_impl(farcur_c128* Cc, farcur_c128* Uc, farcur_c128* Rc, int r, double
     if (cols_edges != n_ln) return fail(FARCUR_EINVAL, "edges has %d columns but ln has %d entries", cols_edges, n_ln);
     if (n_r1 <= 0 || n_r2 <= 0 || r <= 0) return fail(FARCUR_EINVAL, "empty block or cur_order");
     if (r > std::min(n_r1, n_r2)) return fail(FARCUR_EINVAL, "cur_order %d exceeds min(n_r1, n_r2)", r);
-    if (r > 1024) return fail(FARCUR_EINVAL, "cur_order %d exceeds 1024", r);
+    if (r > kMaxSamples) return fail(FARCUR_EINVAL, "cur_order %d exceeds %d", r, kMaxSamples);
     farcur_ctx* ctx;
     TRY(compat_ctx(&ctx));
     TRY(ctx->select());
@@ -1082,7 +1104,7 @@ int compat_impl(farcur_c128* Cc, farcur_c128* Uc, farcur_c128* Rc, int r, double
     TRY(ctx->d_C.ensure((size_t)m * r * 16));
     TRY(ctx->d_R.ensure((size_t)r * n * 16));
     TRY(ctx->d_W.ensure((size_t)r * r * 16));
-    TRY(ctx->d_V.ensure((size_t)r * r * 16));
+    TRY(ctx->d_V.ensure(pinv_v_elems(r) * 16));
     TRY(ctx->d_U.ensure((size_t)r * r * 16));
     TRY(ctx->d_sigma.ensure(r * 8));
     TRY(ctx->d_v.ensure((size_t)n * 16));
@@ -1224,7 +1246,7 @@ int batch_alloc_round(farcur_batch* B, int r, int m, int n, RoundBufs& rb) {
     TRY(B->arena.alloc((size_t)r * r * 16, &p));
     rb.W = (c128*)p;
     if (r > kPinvLdsMaxR) {
-        TRY(B->arena.alloc((size_t)r * r * 16, &p));
+        TRY(B->arena.alloc(pinv_v_elems(r) * 16, &p));
         rb.V = (c128*)p;
     } else {
         rb.V = nullptr;
@@ -1430,7 +1452,8 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
         s.n = B->col_off[b + 1] - B->col_off[b];
         const int D = std::min(s.m, s.n);
         s.cap = o.max_rank > 0 ? std::min(D, o.max_rank) : D;
-        if (s.cap > 1024) return fail(FARCUR_EINVAL, "block %d: sample cap %d exceeds 1024 (set max_rank)", b, s.cap);
+        if (s.cap > kMaxSamples)
+            return fail(FARCUR_EINVAL, "block %d: sample cap %d exceeds %d (set max_rank)", b, s.cap, kMaxSamples);
         s.r = std::min(o.r_start > 0 ? o.r_start : r_start_rule(D), s.cap);
     }
     B->arena.reset();

@@ -18,14 +18,18 @@
 namespace farcur {
 
 // Per-RWG record, structure of arrays: field f of RWG e at rec[f * ne + e].
-//   f = h*9 + i*3 + c        quadrature point i (xyz c) of triangle T+ (h=0) / T- (h=1)
+// FP64 records:
+//   f = h*9 + i*3 + c        quadrature point q_hi (xyz c) of triangle T+ (h=0) / T- (h=1)
 //   f = 18 + h*9 + i*3 + c   rho_hi = s_h (q_hi - v_h)
 //   f = 36                   edge length l
+// FP32 records hold u_hi = q_hi - c_e (offsets from the RWG centre c_e, the
+// mean of its six quadrature points) in place of q_hi; c_e itself is kept in
+// FP64 (cen[c * ne + e]) so that centre-to-centre distances keep full accuracy.
 constexpr int kRecFields = 37;
 
 template <typename T>
 struct RwgRec {
-    T q[2][3][3];
+    T q[2][3][3];  // q (FP64) or u (FP32)
     T p[2][3][3];
     T l;
 };
@@ -44,7 +48,8 @@ __device__ __forceinline__ void load_rec(const T* __restrict__ rec, int ne, int 
     r.l = rec[(size_t)36 * ne + e];
 }
 
-// Green-kernel sample: accumulates T (sin kR, cos kR) / R.
+// ---------------------------------------------------------------- FP64
+// Accumulates T (sin kR, cos kR) / R.
 __device__ __forceinline__ void green_acc(double dx, double dy, double dz, double term, double k, double& as,
                                           double& ac) {
     const double r2 = dx * dx + dy * dy + dz * dz;
@@ -56,64 +61,99 @@ __device__ __forceinline__ void green_acc(double dx, double dy, double dz, doubl
     ac += w * c;
 }
 
-// FP32: 1/R from the hardware reciprocal square root, and the phase in
-// revolutions (R / lambda) reduced to [-1/2, 1/2] so that the hardware sine and
-// cosine (which take revolutions) see a small argument.
-__device__ __forceinline__ void green_acc(float dx, float dy, float dz, float term, float inv_lambda, float& as,
-                                          float& ac) {
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    const float ir = __builtin_amdgcn_rsqf(r2);
-    const float t = r2 * ir * inv_lambda;
-    const float f = t - __builtin_rintf(t);
-    const float s = __builtin_amdgcn_sinf(f);
-    const float c = __builtin_amdgcn_cosf(f);
-    const float w = term * ir;
-    as += w * s;
-    ac += w * c;
-}
-
 // Source point (hb, j) against the 6 test points of `a`.
-template <typename T>
-__device__ __forceinline__ void source_point_sums(const RwgRec<T>& a, const T* __restrict__ b, int hb, int j, T kw,
-                                                  T c4k2, T& as, T& ac) {
-    const T bx = b[hb * 9 + j * 3 + 0], by = b[hb * 9 + j * 3 + 1], bz = b[hb * 9 + j * 3 + 2];
-    const T px = b[18 + hb * 9 + j * 3 + 0], py = b[18 + hb * 9 + j * 3 + 1], pz = b[18 + hb * 9 + j * 3 + 2];
+__device__ __forceinline__ void source_point_sums(const RwgRec<double>& a, const double* __restrict__ b, int hb, int j,
+                                                  double k, double c4k2, double& as, double& ac) {
+    const double bx = b[hb * 9 + j * 3 + 0], by = b[hb * 9 + j * 3 + 1], bz = b[hb * 9 + j * 3 + 2];
+    const double px = b[18 + hb * 9 + j * 3 + 0], py = b[18 + hb * 9 + j * 3 + 1], pz = b[18 + hb * 9 + j * 3 + 2];
 #pragma unroll
     for (int ha = 0; ha < 2; ++ha) {
-        const T ss = (ha == hb) ? c4k2 : -c4k2;
+        const double ss = (ha == hb) ? c4k2 : -c4k2;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            const T term = a.p[ha][i][0] * px + a.p[ha][i][1] * py + a.p[ha][i][2] * pz - ss;
-            green_acc(a.q[ha][i][0] - bx, a.q[ha][i][1] - by, a.q[ha][i][2] - bz, term, kw, as, ac);
+            const double term = a.p[ha][i][0] * px + a.p[ha][i][1] * py + a.p[ha][i][2] * pz - ss;
+            green_acc(a.q[ha][i][0] - bx, a.q[ha][i][1] - by, a.q[ha][i][2] - bz, term, k, as, ac);
         }
     }
 }
 
-// Sum over the 4 triangle pairs x 9 point pairs. `kw` is k for FP64 and
-// 1/lambda for FP32 (see green_acc). Returns sum T sin/R and sum T cos/R.
-// FP64 keeps the 6 source points as a loop (the FP64 sine/cosine is long and
-// a full unroll exhausts the register file); FP32 unrolls completely.
-template <typename T>
-__device__ __forceinline__ void entry_sums(const RwgRec<T>& a, const T* __restrict__ b, T kw, T c4k2, T& as,
-                                           T& ac) {
-    as = 0;
-    ac = 0;
-    if constexpr (sizeof(T) == 8) {
+// Z_mn in FP64. The 6 source points stay a loop: the FP64 sine/cosine is long
+// and a full unroll exhausts the register file.
+__device__ __forceinline__ void entry_f64(const RwgRec<double>& a, const double* __restrict__ b, double k,
+                                          double c4k2, double scale, double& zr, double& zi) {
+    double as = 0, ac = 0;
 #pragma unroll 1
-        for (int s = 0; s < 6; ++s) source_point_sums(a, b, s / 3, s % 3, kw, c4k2, as, ac);
-    } else {
-#pragma unroll
-        for (int s = 0; s < 6; ++s) source_point_sums(a, b, s / 3, s % 3, kw, c4k2, as, ac);
-    }
+    for (int s = 0; s < 6; ++s) source_point_sums(a, b, s / 3, s % 3, k, c4k2, as, ac);
+    const double kap = scale * a.l * b[36];
+    zr = -kap * as;
+    zi = -kap * ac;
 }
 
-// Constants of one fill: k, k*eta/(144 pi), 4/k^2, and the phase factor kw.
-template <typename T>
-struct FillConsts {
-    T kw;     // k (FP64) or 1/lambda = k / (2 pi) (FP32)
-    T c4k2;   // 4 / k^2
-    T scale;  // k eta / (144 pi)
+// ---------------------------------------------------------------- FP32
+// With R0 = |c_m - c_n| and delta = R - R0 for each point pair,
+//   e^{-jkR}/R = e^{-jkR0} g,   g = e^{-jk delta} / R,
+// and because sum_{a,b} s_a s_b = 0 the charge (div-div) term may use
+//   g - 1/R0 = [(cos k delta - 1) - j sin k delta] / R - delta / (R R0)
+// instead of g: the near-cancelling second difference of the scalar potential
+// is then formed from small quantities only. delta = X / (R + R0) with
+// X = 2 d.D + |D|^2 (d = c_m - c_n, D = u_a - u_b) is exact to rounding.
+// Sines take revolutions (hardware sine), arguments |delta / lambda| < 1.
+struct F32Consts {
+    float inv_lambda;  // k / (2 pi)
+    float c4k2;        // 4 / k^2
+    float scale;       // k eta / (144 pi)
+    double inv_lambda64;
 };
+
+__device__ __forceinline__ void entry_f32(const RwgRec<float>& a, const double* __restrict__ ac3,
+                                          const float* __restrict__ b, const double* __restrict__ bc3,
+                                          const F32Consts& fc, float& zr, float& zi) {
+    const double dx64 = ac3[0] - bc3[0], dy64 = ac3[1] - bc3[1], dz64 = ac3[2] - bc3[2];
+    const double r064 = sqrt(dx64 * dx64 + dy64 * dy64 + dz64 * dz64);
+    const double ph = r064 * fc.inv_lambda64;
+    const float f0 = (float)(ph - rint(ph));
+    const float s0 = __builtin_amdgcn_sinf(f0), c0 = __builtin_amdgcn_cosf(f0);
+    const float dx = (float)dx64, dy = (float)dy64, dz = (float)dz64;
+    const float r0 = (float)r064, r0sq = r0 * r0, ir0 = 1.0f / r0;
+    float ar = 0, ai = 0;
+#pragma unroll
+    for (int hb = 0; hb < 2; ++hb) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const float bx = b[hb * 9 + j * 3 + 0], by = b[hb * 9 + j * 3 + 1], bz = b[hb * 9 + j * 3 + 2];
+            const float px = b[18 + hb * 9 + j * 3 + 0], py = b[18 + hb * 9 + j * 3 + 1];
+            const float pz = b[18 + hb * 9 + j * 3 + 2];
+#pragma unroll
+            for (int ha = 0; ha < 2; ++ha) {
+                const float ss = (ha == hb) ? fc.c4k2 : -fc.c4k2;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const float Dx = a.q[ha][i][0] - bx, Dy = a.q[ha][i][1] - by, Dz = a.q[ha][i][2] - bz;
+                    const float X = 2.0f * (dx * Dx + dy * Dy + dz * Dz) + (Dx * Dx + Dy * Dy + Dz * Dz);
+                    const float r2 = r0sq + X;
+                    const float ir = __builtin_amdgcn_rsqf(r2);
+                    const float r = r2 * ir;
+                    const float delta = X * __builtin_amdgcn_rcpf(r + r0);
+                    const float t = delta * fc.inv_lambda;
+                    const float sn = __builtin_amdgcn_sinf(t);
+                    const float sh = __builtin_amdgcn_sinf(0.5f * t);
+                    const float cm1 = -2.0f * sh * sh;  // cos(k delta) - 1
+                    const float term = a.p[ha][i][0] * px + a.p[ha][i][1] * py + a.p[ha][i][2] * pz;
+                    // term * g - ss * (g - 1/R0)
+                    const float gr = (1.0f + cm1) * ir, gi = -sn * ir;
+                    const float hr = cm1 * ir - delta * ir * ir0;
+                    ar += term * gr - ss * hr;
+                    ai += (term - ss) * gi;
+                }
+            }
+        }
+    }
+    // Z = -j kappa e^{-jkR0} (ar + j ai)
+    const float pr = c0 * ar + s0 * ai, pi = c0 * ai - s0 * ar;
+    const float kap = fc.scale * a.l * b[36];
+    zr = kap * pi;
+    zi = -kap * pr;
+}
 
 // Tile geometry of the panel fill: a workgroup of 4 waves covers 64 rows x
 // (4 * kTileColsPerWave) columns; lane = row, wave = group of columns. The
@@ -123,37 +163,67 @@ constexpr int kTileColsPerWave = 4;
 constexpr int kTileCols = 4 * kTileColsPerWave;
 constexpr int kFillThreads = 256;
 
+template <typename T>
+struct FillTileLds {
+    T col[kTileCols][kRecFields];
+    double cen[kTileCols][3];
+};
+
 // Fills rows [row0, row0+64) x cols [col0, col0+16) of the panel
 // out[i + j*ld] = Z(rows[i], cols[j]). Must be called by all 256 threads.
-template <typename T, typename OutC>
+// F64: FillConsts (k, 4/k^2, k eta / (144 pi)); F32: F32Consts and centres.
+template <typename T, typename OutC, typename Consts>
 __device__ __forceinline__ void fill_tile(OutC* __restrict__ out, int ld, const int* __restrict__ rows, int nr,
                                           const int* __restrict__ cols, int nc, int row0, int col0,
-                                          const T* __restrict__ rec, int ne, const FillConsts<T>& fc,
-                                          T (*s_col)[kRecFields]) {
+                                          const T* __restrict__ rec, const double* __restrict__ cen, int ne,
+                                          const Consts& fc, FillTileLds<T>& lds) {
     const int tid = threadIdx.x;
-    __syncthreads();  // s_col may still be read by a previous tile
+    __syncthreads();  // the LDS tile may still be read by a previous tile
     for (int idx = tid; idx < kTileCols * kRecFields; idx += kFillThreads) {
         const int c = idx / kRecFields, f = idx - c * kRecFields;
         const int j = col0 + c;
-        s_col[c][f] = (j < nc) ? rec[(size_t)f * ne + cols[j]] : T(0);
+        lds.col[c][f] = (j < nc) ? rec[(size_t)f * ne + cols[j]] : T(0);
+    }
+    if constexpr (sizeof(T) == 4) {
+        if (tid < kTileCols * 3) {
+            const int c = tid / 3, x = tid - 3 * c;
+            const int j = col0 + c;
+            lds.cen[c][x] = (j < nc) ? cen[(size_t)x * ne + cols[j]] : 0.0;
+        }
     }
     __syncthreads();
     const int lane = tid & (kWave - 1);
     const int w = tid / kWave;
     const int i = row0 + lane;
     if (i >= nr) return;
+    const int e = rows[i];
     RwgRec<T> a;
-    load_rec(rec, ne, rows[i], a);
+    load_rec(rec, ne, e, a);
+    double ac3[3] = {0, 0, 0};
+    if constexpr (sizeof(T) == 4) {
+        ac3[0] = cen[e];
+        ac3[1] = cen[(size_t)ne + e];
+        ac3[2] = cen[2 * (size_t)ne + e];
+    }
 #pragma unroll 1
     for (int cc = 0; cc < kTileColsPerWave; ++cc) {
         const int c = w * kTileColsPerWave + cc;
         const int j = col0 + c;
         if (j >= nc) break;
-        T as, ac;
-        entry_sums<T>(a, s_col[c], fc.kw, fc.c4k2, as, ac);
-        const T kap = fc.scale * a.l * s_col[c][36];
-        store_c(&out[(size_t)j * ld + i], -kap * as, -kap * ac);
+        T zr, zi;
+        if constexpr (sizeof(T) == 8)
+            entry_f64(a, lds.col[c], fc.k, fc.c4k2, fc.scale, zr, zi);
+        else
+            entry_f32(a, ac3, lds.col[c], lds.cen[c], fc, zr, zi);
+        store_c(&out[(size_t)j * ld + i], zr, zi);
     }
 }
+
+// Constants of an FP64 fill.
+struct FillConsts {
+    double k;      // wavenumber
+    double c4k2;   // 4 / k^2
+    double scale;  // k eta / (144 pi)
+};
 
 }  // namespace farcur

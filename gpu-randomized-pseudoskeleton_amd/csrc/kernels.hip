@@ -110,24 +110,32 @@ int fill_tiles(int m, int n, int r, int panels) {
 }
 
 template <typename T>
-__device__ __forceinline__ FillConsts<T> make_consts(double k, double eta);
+struct ConstsOf;
 template <>
-__device__ __forceinline__ FillConsts<double> make_consts<double>(double k, double eta) {
-    return FillConsts<double>{k, 4.0 / (k * k), k * eta / (144.0 * M_PI)};
-}
+struct ConstsOf<double> {
+    using type = FillConsts;
+    static __device__ FillConsts make(double k, double eta) {
+        return FillConsts{k, 4.0 / (k * k), k * eta / (144.0 * M_PI)};
+    }
+};
 template <>
-__device__ __forceinline__ FillConsts<float> make_consts<float>(double k, double eta) {
-    return FillConsts<float>{(float)(k / (2.0 * M_PI)), (float)(4.0 / (k * k)), (float)(k * eta / (144.0 * M_PI))};
-}
+struct ConstsOf<float> {
+    using type = F32Consts;
+    static __device__ F32Consts make(double k, double eta) {
+        const double il = k / (2.0 * M_PI);
+        return F32Consts{(float)il, (float)(4.0 / (k * k)), (float)(k * eta / (144.0 * M_PI)), il};
+    }
+};
 
 // Persistent fill over the tiles of all jobs. T = double: C, W, R in c128.
 // T = float: C, R in c64 and W widened to c128 for the FP64 pinv.
 template <typename T, typename PC>
 __global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restrict__ jobs, int njobs,
                                                        const int32_t* __restrict__ prefix, int total, int panels,
-                                                       const T* __restrict__ rec, int ne, double k, double eta) {
-    __shared__ T s_col[kTileCols][kRecFields];
-    const FillConsts<T> fc = make_consts<T>(k, eta);
+                                                       const T* __restrict__ rec, const double* __restrict__ cen,
+                                                       int ne, double k, double eta) {
+    __shared__ FillTileLds<T> lds;
+    const auto fc = ConstsOf<T>::make(k, eta);
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         int lo = 0, hi = njobs - 1;
         while (lo < hi) {
@@ -141,7 +149,7 @@ __global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restric
             const int mt = cdiv(m, kTileRows), nC = mt * cdiv(r, kTileCols);
             if (local < nC) {
                 fill_tile<T, PC>((PC*)J.C, m, J.rows, m, J.cs, r, (local % mt) * kTileRows,
-                                 (local / mt) * kTileCols, rec, ne, fc, s_col);
+                                 (local / mt) * kTileCols, rec, cen, ne, fc, lds);
                 continue;
             }
             local -= nC;
@@ -151,13 +159,13 @@ __global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restric
             const int nW = rt * cdiv(r, kTileCols);
             if (local < nW) {
                 fill_tile<T, c128>(J.W, r, J.rs, r, J.cs, r, (local % rt) * kTileRows, (local / rt) * kTileCols,
-                                   rec, ne, fc, s_col);
+                                   rec, cen, ne, fc, lds);
                 continue;
             }
             local -= nW;
         }
         fill_tile<T, PC>((PC*)J.R, r, J.rs, r, J.cols, n, (local % rt) * kTileRows, (local / rt) * kTileCols, rec,
-                         ne, fc, s_col);
+                         cen, ne, fc, lds);
     }
 }
 
@@ -167,21 +175,23 @@ void launch_fill(const BlockJob* jobs, int njobs, const int32_t* tile_prefix, in
     if (grid <= 0 || grid > total_tiles) grid = total_tiles;
     if (fp.f32)
         hipLaunchKernelGGL((k_fill<float, c64>), dim3(grid), dim3(kFillThreads), 0, s, jobs, njobs, tile_prefix,
-                           total_tiles, panels, (const float*)fp.rec, fp.ne, fp.k, fp.eta);
+                           total_tiles, panels, (const float*)fp.rec, fp.cen, fp.ne, fp.k, fp.eta);
     else
         hipLaunchKernelGGL((k_fill<double, c128>), dim3(grid), dim3(kFillThreads), 0, s, jobs, njobs, tile_prefix,
-                           total_tiles, panels, (const double*)fp.rec, fp.ne, fp.k, fp.eta);
+                           total_tiles, panels, (const double*)fp.rec, fp.cen, fp.ne, fp.k, fp.eta);
 }
 
 template <typename T, typename PC>
 __global__ __launch_bounds__(kFillThreads) void k_fill_panel(PC* __restrict__ Z, int ld,
                                                              const int32_t* __restrict__ rows, int nr,
                                                              const int32_t* __restrict__ cols, int nc,
-                                                             const T* __restrict__ rec, int ne, double k, double eta) {
-    __shared__ T s_col[kTileCols][kRecFields];
-    const FillConsts<T> fc = make_consts<T>(k, eta);
-    fill_tile<T, PC>(Z, ld, rows, nr, cols, nc, blockIdx.x * kTileRows, blockIdx.y * kTileCols, rec, ne, fc,
-                     s_col);
+                                                             const T* __restrict__ rec,
+                                                             const double* __restrict__ cen, int ne, double k,
+                                                             double eta) {
+    __shared__ FillTileLds<T> lds;
+    const auto fc = ConstsOf<T>::make(k, eta);
+    fill_tile<T, PC>(Z, ld, rows, nr, cols, nc, blockIdx.x * kTileRows, blockIdx.y * kTileCols, rec, cen, ne, fc,
+                     lds);
 }
 
 void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32_t* cols, int nc,
@@ -190,10 +200,10 @@ void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32
     const dim3 grid(cdiv(nr, kTileRows), cdiv(nc, kTileCols));
     if (fp.f32)
         hipLaunchKernelGGL((k_fill_panel<float, c64>), grid, dim3(kFillThreads), 0, s, (c64*)Z, ld, rows, nr, cols,
-                           nc, (const float*)fp.rec, fp.ne, fp.k, fp.eta);
+                           nc, (const float*)fp.rec, fp.cen, fp.ne, fp.k, fp.eta);
     else
         hipLaunchKernelGGL((k_fill_panel<double, c128>), grid, dim3(kFillThreads), 0, s, (c128*)Z, ld, rows, nr,
-                           cols, nc, (const double*)fp.rec, fp.ne, fp.k, fp.eta);
+                           cols, nc, (const double*)fp.rec, fp.cen, fp.ne, fp.k, fp.eta);
 }
 
 // ---------------------------------------------------------------- pinv
@@ -221,7 +231,10 @@ __global__ void k_pinv(const BlockJob* __restrict__ jobs, const int32_t* __restr
         }
         int kept;
         double smax;
-        jacobi_pinv(A, V, r, atol, rtol, J.U, r, J.sigma, &s_flag, s_w, &kept, &smax);
+        // Weights live in LDS up to r = 1024, beyond that after V (V holds
+        // r*r + (r+1)/2 complex values, see pinv_v_elems).
+        double* w = (kLds || r <= 1024) ? s_w : (double*)(J.V + (size_t)r * r);
+        jacobi_pinv(A, V, r, atol, rtol, J.U, r, J.sigma, &s_flag, w, &kept, &smax);
         if (threadIdx.x == 0) {
             out[jb].kept = kept;
             out[jb].sigma_max = smax;
@@ -251,8 +264,14 @@ __global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, JobOut*
     __shared__ double s_red[16];
     for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
         const BlockJob& J = jobs[jb];
-        const double err = cur_residual<PC>((const PC*)J.C, J.m, J.r, J.U, (const PC*)J.R, J.n, J.v, J.y_old,
-                                            J.y_new, s_t1, s_t2, s_part, s_red);
+        // Beyond kResidMaxR the two r-vectors use the W panel (r*r >= 2r),
+        // which the pinv has already consumed.
+        const double err =
+            J.r <= kResidMaxR
+                ? cur_residual<PC>((const PC*)J.C, J.m, J.r, J.U, (const PC*)J.R, J.n, J.v, J.y_old, J.y_new, s_t1,
+                                   s_t2, s_part, s_red)
+                : cur_residual<PC>((const PC*)J.C, J.m, J.r, J.U, (const PC*)J.R, J.n, J.v, J.y_old, J.y_new, J.W,
+                                   J.W + J.r, s_part, s_red);
         if (threadIdx.x == 0) out[jb].err = err;
         __syncthreads();
     }

@@ -42,7 +42,8 @@ struct JobOut {
 };
 
 struct FillParams {
-    const void* rec;  // RWG records (double or float SoA, kRecFields x ne)
+    const void* rec;     // RWG records (double or float SoA, kRecFields x ne)
+    const double* cen;   // RWG centres, 3 x ne SoA (used by the FP32 fill)
     int32_t ne;
     int32_t f32;      // 1: FP32 fill, panels c64; 0: FP64 fill, panels c128
     double k, eta;
@@ -53,7 +54,6 @@ int fill_tiles(int m, int n, int r, int panels);
 
 void launch_sample(const BlockJob* jobs, int njobs, uint64_t seed, hipStream_t s);
 void launch_pivot_rows(const BlockJob* jobs, int njobs, int f32, hipStream_t s);
-// Fill of C and R (and W unless the rows were pivoted: then W = C[rows, :] is gathered).
 // panels: bit 0 = C, bit 1 = W, bit 2 = R. tile_prefix: njobs + 1 device ints
 // (tiles of job j are [tile_prefix[j], tile_prefix[j+1])).
 enum : int { kPanelC = 1, kPanelW = 2, kPanelR = 4 };
@@ -61,6 +61,11 @@ void launch_fill(const BlockJob* jobs, int njobs, const int32_t* tile_prefix, in
                  const FillParams& fp, int grid, hipStream_t s);
 // Pinv of W for the listed jobs (indices into jobs). lds = 1 needs r <= kPinvLdsMaxR.
 constexpr int kPinvLdsMaxR = 44;
+// Complex elements of the Jacobi scratch V for sample count r (V plus, past
+// r = 1024, the r weights).
+inline size_t pinv_v_elems(int r) { return (size_t)r * r + (size_t)(r + 1) / 2; }
+// Largest sample count any entry point accepts.
+constexpr int kMaxSamples = 8192;
 void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, int lds,
                  int threads, JobOut* out, hipStream_t s);
 void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, JobOut* out, hipStream_t s);

@@ -63,8 +63,9 @@ def test_fill_f32_vs_f64(farcur, plates):
     cols = np.arange(1160, 2320, 7)
     Z64 = ctx.fill(rows, cols, farcur.F64)
     Z32 = ctx.fill(rows, cols, farcur.F32)
-    assert _relfro(Z32, Z64) <= 2e-5
-    assert _relmax(Z32, Z64) <= 1e-4
+    print("f32 vs f64: relfro %.3e relmax %.3e" % (_relfro(Z32, Z64), _relmax(Z32, Z64)))
+    assert _relfro(Z32, Z64) <= 2e-6
+    assert _relmax(Z32, Z64) <= 5e-6
 
 
 @pytest.mark.parametrize("N,r", [(1, 1), (10, 3), (113, 11), (1160, 116), (3072, 614)])
@@ -106,7 +107,12 @@ def test_pinv_ill_conditioned_core(plates, oracle):
     Uo, so, kepto = oracle.pinv(W, 1e-10)
     assert kept == kepto
     np.testing.assert_allclose(np.sort(s)[::-1], np.sort(so)[::-1], rtol=1e-8, atol=1e-13 * s.max())
-    assert _relfro(W @ U @ W, W) <= 1e-8
+    # W U W = W up to the dropped singular values (below 1e-10).
+    sl = np.linalg.svd(W, compute_uv=False)
+    dropped = np.linalg.norm(sl[sl < 1e-10])
+    assert np.linalg.norm(W @ U @ W - W) <= 1.5 * dropped + 1e-9 * np.linalg.norm(W)
+    Ul, _, _ = _lapack_pinv(W, 1e-10)
+    assert _relfro(W @ U @ W, W @ Ul @ W) <= 1e-6
 
 
 def test_residual_matches_numpy(plates):
@@ -157,8 +163,9 @@ def test_compress_block_matches_oracle(plates, oracle, row_mode):
     A = out["C"] @ out["U"] @ out["R"]
     Ao = ref["C"] @ ref["U"] @ ref["R"]
     Z = g.fill(rows, cols)
-    assert _relfro(A, Ao) <= 1e-8
-    assert _relfro(A, Z) <= 1e-4  # north-star accuracy bar for the plain method
+    assert _relfro(A, Ao) <= 1e-6  # |U| ~ 1/sigma_min amplifies rounding (absolute 1e-10 threshold)
+    assert abs(_relfro(A, Z) - _relfro(Ao, Z)) <= 1e-6
+    assert _relfro(A, Z) <= 1e-4  # accuracy bar for the plain method
     assert out["converged"] == 1
 
 
