@@ -188,6 +188,13 @@ struct KTimer {
 
 }  // namespace
 
+// Launch shapes, chosen by the autotuner (farcur_autotune) or defaults.
+struct Tuning {
+    int pinv_threads[kPinvClasses] = {64, 64, 128, 256, 256};
+    int fill_grid = 256 * 16;
+    int resid_threads = 256;
+};
+
 struct farcur_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -202,6 +209,7 @@ struct farcur_ctx {
     int last_m = 0, last_n = 0, last_r = 0, last_f32 = 0, last_y = 0;
     bool has_result = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Tuning tune;
 
     int select() {
         HIPTRY(hipSetDevice(device));
@@ -337,7 +345,7 @@ struct RoundPlan {
     int f32 = 0;
     bool pivot = false;
     int fill_grid = 0;
-    int pinv_threads_global = 256;
+    int pinv_threads[kPinvClasses] = {64, 64, 128, 256, 256};
     int resid_threads = 256;
     double atol = 1e-10, rtol = 0;
     uint64_t seed = 0;
@@ -346,11 +354,14 @@ struct RoundPlan {
     const int32_t* prefix_all = nullptr;  // fill prefix: all panels (or C only when pivoting)
     const int32_t* prefix_wr = nullptr;   // fill prefix: W + R (pivot second pass)
     int total_all = 0, total_wr = 0;
-    const int32_t* list_lds = nullptr;
-    int nlds = 0;
-    const int32_t* list_glb = nullptr;
-    int nglb = 0;
+    const int32_t* list[kPinvClasses] = {};
+    int nlist[kPinvClasses] = {};
     JobOut* out = nullptr;
+    void tune(const Tuning& t) {
+        for (int c = 0; c < kPinvClasses; ++c) pinv_threads[c] = t.pinv_threads[c];
+        fill_grid = t.fill_grid;
+        resid_threads = t.resid_threads;
+    }
 };
 
 void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt) {
@@ -376,25 +387,28 @@ void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt) {
         kt.end(1, s, a);
     }
     kt.begin(2, s, &a);
-    launch_pinv(P.jobs, P.list_lds, P.nlds, P.atol, P.rtol, 1, 256, P.out, s);
-    launch_pinv(P.jobs, P.list_glb, P.nglb, P.atol, P.rtol, 0, P.pinv_threads_global, P.out, s);
+    for (int c = 0; c < kPinvClasses; ++c)
+        launch_pinv(P.jobs, P.list[c], P.nlist[c], P.atol, P.rtol, kPinvClassMax[c], P.pinv_threads[c], P.out, s);
     kt.end(2, s, a);
     kt.begin(3, s, &a);
     launch_residual(P.jobs, P.njobs, P.f32, P.resid_threads, P.out, s);
     kt.end(3, s, a);
 }
 
-// Host-side packing of a round: jobs, fill prefixes, pinv lists, into one
-// device upload.
+// Host-side packing of a round: jobs, fill prefixes, pinv lists by r class,
+// into one device upload.
 struct PackedRound {
     std::vector<char> bytes;
-    size_t off_jobs = 0, off_pa = 0, off_pwr = 0, off_ll = 0, off_lg = 0, off_out = 0;
-    int total_all = 0, total_wr = 0, nlds = 0, nglb = 0;
+    size_t off_jobs = 0, off_pa = 0, off_pwr = 0, off_out = 0;
+    size_t off_list[kPinvClasses] = {};
+    int total_all = 0, total_wr = 0;
+    int nlist[kPinvClasses] = {};
 };
 
 void pack_round(const std::vector<BlockJob>& jobs, bool pivot, PackedRound& pk) {
     const int nj = (int)jobs.size();
-    std::vector<int32_t> pa(nj + 1), pwr(nj + 1), ll, lg;
+    std::vector<int32_t> pa(nj + 1), pwr(nj + 1);
+    std::vector<int32_t> lists[kPinvClasses];
     pa[0] = 0;
     pwr[0] = 0;
     for (int j = 0; j < nj; ++j) {
@@ -402,7 +416,7 @@ void pack_round(const std::vector<BlockJob>& jobs, bool pivot, PackedRound& pk) 
         const int all = pivot ? fill_tiles(J.m, J.n, J.r, kPanelC) : fill_tiles(J.m, J.n, J.r, kPanelC | kPanelW | kPanelR);
         pa[j + 1] = pa[j] + all;
         pwr[j + 1] = pwr[j] + (pivot ? fill_tiles(J.m, J.n, J.r, kPanelW | kPanelR) : 0);
-        (J.r <= kPinvLdsMaxR ? ll : lg).push_back(j);
+        lists[pinv_class(J.r)].push_back(j);
     }
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t o = 0;
@@ -412,22 +426,21 @@ void pack_round(const std::vector<BlockJob>& jobs, bool pivot, PackedRound& pk) 
     o = align(o + pa.size() * 4);
     pk.off_pwr = o;
     o = align(o + pwr.size() * 4);
-    pk.off_ll = o;
-    o = align(o + std::max<size_t>(ll.size(), 1) * 4);
-    pk.off_lg = o;
-    o = align(o + std::max<size_t>(lg.size(), 1) * 4);
+    for (int c = 0; c < kPinvClasses; ++c) {
+        pk.off_list[c] = o;
+        o = align(o + std::max<size_t>(lists[c].size(), 1) * 4);
+        pk.nlist[c] = (int)lists[c].size();
+    }
     pk.off_out = o;
     o = align(o + nj * sizeof(JobOut));
     pk.bytes.assign(o, 0);
     std::memcpy(pk.bytes.data() + pk.off_jobs, jobs.data(), nj * sizeof(BlockJob));
     std::memcpy(pk.bytes.data() + pk.off_pa, pa.data(), pa.size() * 4);
     std::memcpy(pk.bytes.data() + pk.off_pwr, pwr.data(), pwr.size() * 4);
-    if (!ll.empty()) std::memcpy(pk.bytes.data() + pk.off_ll, ll.data(), ll.size() * 4);
-    if (!lg.empty()) std::memcpy(pk.bytes.data() + pk.off_lg, lg.data(), lg.size() * 4);
+    for (int c = 0; c < kPinvClasses; ++c)
+        if (!lists[c].empty()) std::memcpy(pk.bytes.data() + pk.off_list[c], lists[c].data(), lists[c].size() * 4);
     pk.total_all = pa[nj];
     pk.total_wr = pwr[nj];
-    pk.nlds = (int)ll.size();
-    pk.nglb = (int)lg.size();
 }
 
 RoundPlan plan_from(const PackedRound& pk, void* dev, int njobs) {
@@ -439,10 +452,10 @@ RoundPlan plan_from(const PackedRound& pk, void* dev, int njobs) {
     P.prefix_wr = (const int32_t*)(d + pk.off_pwr);
     P.total_all = pk.total_all;
     P.total_wr = pk.total_wr;
-    P.list_lds = (const int32_t*)(d + pk.off_ll);
-    P.nlds = pk.nlds;
-    P.list_glb = (const int32_t*)(d + pk.off_lg);
-    P.nglb = pk.nglb;
+    for (int c = 0; c < kPinvClasses; ++c) {
+        P.list[c] = (const int32_t*)(d + pk.off_list[c]);
+        P.nlist[c] = pk.nlist[c];
+    }
     P.out = (JobOut*)(d + pk.off_out);
     return P;
 }
@@ -700,8 +713,9 @@ int farcur_pinv(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, doubl
     TRY(ctx->d_plan.ensure(sizeof(Plan)));
     HIPTRY(hipMemcpyAsync(ctx->d_plan.p, &plan, sizeof(Plan), hipMemcpyHostToDevice, ctx->stream));
     Plan* dp = ctx->d_plan.as<Plan>();
-    const int lds = r <= kPinvLdsMaxR;
-    launch_pinv(&dp->job, dp->list, 1, atol, rtol, lds, lds ? 256 : (r > 128 ? 1024 : 256), &dp->out, ctx->stream);
+    const int cls = pinv_class(r);
+    const int threads = cls < kPinvClasses - 1 ? ctx->tune.pinv_threads[cls] : (r > 128 ? 1024 : 256);
+    launch_pinv(&dp->job, dp->list, 1, atol, rtol, kPinvClassMax[cls], threads, &dp->out, ctx->stream);
     HIPTRY(hipGetLastError());
     HIPTRY(hipMemcpyAsync(U, ctx->d_U.p, rr, hipMemcpyDeviceToHost, ctx->stream));
     std::vector<double> s(r);
@@ -903,8 +917,9 @@ int block_loop(farcur_ctx* ctx, const BlockLoopIO& io, const farcur_opts& o, All
         P.rtol = o.pinv_rtol;
         P.seed = o.seed;
         const bool big = (size_t)io.m * r > (size_t)256 * 1024 || r > 128;
-        P.pinv_threads_global = r > 128 ? 1024 : 256;
-        P.resid_threads = big ? 1024 : 256;
+        P.tune(ctx->tune);
+        if (r > kPinvLdsMaxR) P.pinv_threads[kPinvClasses - 1] = r > 128 ? 1024 : 256;
+        if (big) P.resid_threads = 1024;
         HIPTRY(hipEventRecord(ctx->ev0, ctx->stream));
         run_round(ctx, P, kt);
         HIPTRY(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1142,8 +1157,9 @@ int compat_impl(farcur_c128* Cc, farcur_c128* Uc, farcur_c128* Rc, int r, double
     RoundPlan P = plan_from(pk, ctx->d_plan.p, 1);
     P.atol = pinv_tol;
     P.rtol = 0;
-    P.pinv_threads_global = r > 128 ? 1024 : 256;
-    P.resid_threads = ((size_t)m * r > (size_t)256 * 1024 || r > 128) ? 1024 : 256;
+    P.tune(ctx->tune);
+    if (r > kPinvLdsMaxR) P.pinv_threads[kPinvClasses - 1] = r > 128 ? 1024 : 256;
+    if ((size_t)m * r > (size_t)256 * 1024 || r > 128) P.resid_threads = 1024;
     KTimer kt;
     HIPTRY(hipEventRecord(ctx->ev0, s));
     run_round(ctx, P, kt);
@@ -1334,9 +1350,7 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
     P.atol = o.pinv_tol;
     P.rtol = o.pinv_rtol;
     P.seed = o.seed;
-    P.fill_grid = 256 * 16;
-    P.pinv_threads_global = 256;
-    P.resid_threads = 256;
+    P.tune(ctx->tune);
     run_round(ctx, P, B->kt);
     HIPTRY(hipGetLastError());
     outs.resize(jobs.size());
@@ -1547,4 +1561,122 @@ extern "C" int farcur_batch_kernel_times(farcur_batch* B, const char** names, do
         if (work) work[n] = B->work[i];
     }
     return n;
+}
+
+// ------------------------------------------------------------------ autotuner
+namespace {
+
+// Times the pinv of nprob random r x r problems with the given launch shape
+// (min over reps, ms).
+int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* best_ms) {
+    hipStream_t s = ctx->stream;
+    const size_t rr = (size_t)r * r;
+    DBuf W0, W, V, U, sig, plan;
+    TRY(W0.ensure(nprob * rr * 16));
+    TRY(W.ensure(nprob * rr * 16));
+    TRY(V.ensure(nprob * pinv_v_elems(r) * 16));
+    TRY(U.ensure(nprob * rr * 16));
+    TRY(sig.ensure((size_t)nprob * r * 8));
+    std::vector<c128> h(nprob * rr);
+    uint64_t st = 0x1234567ull + r;
+    for (auto& z : h) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        z.re = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        z.im = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    HIPTRY(hipMemcpyAsync(W0.p, h.data(), h.size() * 16, hipMemcpyHostToDevice, s));
+    std::vector<BlockJob> jobs(nprob);
+    std::vector<int32_t> list(nprob);
+    for (int i = 0; i < nprob; ++i) {
+        BlockJob& J = jobs[i];
+        J = BlockJob{};
+        J.r = r;
+        J.W = W.as<c128>() + i * rr;
+        J.V = V.as<c128>() + i * pinv_v_elems(r);
+        J.U = U.as<c128>() + i * rr;
+        J.sigma = sig.as<double>() + (size_t)i * r;
+        list[i] = i;
+    }
+    const size_t bj = nprob * sizeof(BlockJob), bl = (size_t)nprob * 4, bo = (size_t)nprob * sizeof(JobOut);
+    TRY(plan.ensure(bj + bl + bo + 512));
+    char* d = (char*)plan.p;
+    HIPTRY(hipMemcpyAsync(d, jobs.data(), bj, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(d + bj, list.data(), bl, hipMemcpyHostToDevice, s));
+    *best_ms = 1e30;
+    for (int rep = 0; rep < 3; ++rep) {
+        HIPTRY(hipMemcpyAsync(W.p, W0.p, nprob * rr * 16, hipMemcpyDeviceToDevice, s));
+        HIPTRY(hipEventRecord(ctx->ev0, s));
+        launch_pinv((const BlockJob*)d, (const int32_t*)(d + bj), nprob, 1e-10, 0.0, rmax, threads,
+                    (JobOut*)(d + bj + bl + 256), s);
+        HIPTRY(hipEventRecord(ctx->ev1, s));
+        HIPTRY(hipGetLastError());
+        HIPTRY(hipStreamSynchronize(s));
+        float ms = 0;
+        HIPTRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        *best_ms = std::min(*best_ms, (double)ms);
+    }
+    return FARCUR_OK;
+}
+
+}  // namespace
+
+extern "C" int farcur_pinv_bench(farcur_ctx* ctx, int r, int nprob, int threads, int use_lds, double* ms) {
+    if (!ctx || !ms) return fail(FARCUR_EINVAL, "NULL argument");
+    if (r <= 0 || r > 1024 || nprob <= 0 || nprob > 65536) return fail(FARCUR_EINVAL, "bad sizes");
+    if (threads <= 0 || threads > 1024 || threads % 64) return fail(FARCUR_EINVAL, "threads must be a multiple of 64");
+    if (use_lds && r > kPinvLdsMaxR) return fail(FARCUR_EINVAL, "LDS variant needs r <= %d", kPinvLdsMaxR);
+    TRY(ctx->select());
+    return time_pinv(ctx, r, nprob, threads, use_lds ? r : 0, ms);
+}
+
+extern "C" int farcur_autotune(farcur_ctx* ctx, farcur_tuning* out) {
+    if (!ctx) return fail(FARCUR_EINVAL, "NULL context");
+    TRY(ctx->select());
+    Tuning t = ctx->tune;
+    farcur_tuning rep{};
+    const int cand[3] = {64, 128, 256};
+    for (int c = 0; c < kPinvClasses; ++c) {
+        const int r = c < kPinvClasses - 1 ? kPinvClassMax[c] : 96;
+        const int nprob = c < kPinvClasses - 1 ? 4096 : 1024;
+        double best = 1e30;
+        for (int k = 0; k < 3; ++k) {
+            const int th = c < kPinvClasses - 1 ? cand[k] : 2 * cand[k];
+            double ms;
+            TRY(time_pinv(ctx, r, nprob, th, kPinvClassMax[c], &ms));
+            if (ms < best) {
+                best = ms;
+                t.pinv_threads[c] = th;
+            }
+        }
+        rep.pinv_threads[c] = t.pinv_threads[c];
+        rep.pinv_ms[c] = best;
+    }
+    ctx->tune = t;
+    rep.fill_grid = t.fill_grid;
+    rep.resid_threads = t.resid_threads;
+    if (out) *out = rep;
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_get_tuning(farcur_ctx* ctx, farcur_tuning* out) {
+    if (!ctx || !out) return fail(FARCUR_EINVAL, "NULL argument");
+    *out = farcur_tuning{};
+    for (int c = 0; c < kPinvClasses; ++c) out->pinv_threads[c] = ctx->tune.pinv_threads[c];
+    out->fill_grid = ctx->tune.fill_grid;
+    out->resid_threads = ctx->tune.resid_threads;
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_set_tuning(farcur_ctx* ctx, const farcur_tuning* in) {
+    if (!ctx || !in) return fail(FARCUR_EINVAL, "NULL argument");
+    for (int c = 0; c < kPinvClasses; ++c)
+        if (in->pinv_threads[c] <= 0 || in->pinv_threads[c] > 1024 || in->pinv_threads[c] % 64)
+            return fail(FARCUR_EINVAL, "pinv_threads[%d] must be a multiple of 64 in [64, 1024]", c);
+    if (in->fill_grid <= 0 || in->resid_threads <= 0 || in->resid_threads > 1024 || in->resid_threads % 64)
+        return fail(FARCUR_EINVAL, "bad fill_grid / resid_threads");
+    for (int c = 0; c < kPinvClasses; ++c) ctx->tune.pinv_threads[c] = in->pinv_threads[c];
+    ctx->tune.fill_grid = in->fill_grid;
+    ctx->tune.resid_threads = in->resid_threads;
+    return FARCUR_OK;
 }

@@ -207,34 +207,23 @@ void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32
 }
 
 // ---------------------------------------------------------------- pinv
-template <bool kLds>
-__global__ void k_pinv(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist, double atol,
-                       double rtol, JobOut* __restrict__ out) {
-    constexpr int kMaxR = kLds ? kPinvLdsMaxR : 1;
-    __shared__ c128 s_a[kMaxR * kMaxR];
-    __shared__ c128 s_v[kMaxR * kMaxR];
-    __shared__ double s_w[kLds ? kPinvLdsMaxR : 1024];
-    __shared__ int s_flag;
+// LDS variant: A, V and the weights of one problem in dynamic LDS sized for
+// the largest r of the launch (r <= kPinvLdsMaxR); one workgroup per problem.
+__global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist, int rmax,
+                           double atol, double rtol, JobOut* __restrict__ out) {
+    extern __shared__ __align__(16) char smem[];
+    c128* s_a = (c128*)smem;
+    c128* s_v = s_a + (size_t)rmax * rmax;
+    double* s_w = (double*)(s_v + (size_t)rmax * rmax);
+    int* s_flag = (int*)(s_w + rmax);
     for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
         const int jb = list[li];
         const BlockJob& J = jobs[jb];
         const int r = J.r;
-        c128* A;
-        c128* V;
-        if (kLds) {
-            for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) s_a[idx] = J.W[idx];
-            A = s_a;
-            V = s_v;
-        } else {
-            A = J.W;
-            V = J.V;
-        }
+        for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) s_a[idx] = J.W[idx];
         int kept;
         double smax;
-        // Weights live in LDS up to r = 1024, beyond that after V (V holds
-        // r*r + (r+1)/2 complex values, see pinv_v_elems).
-        double* w = (kLds || r <= 1024) ? s_w : (double*)(J.V + (size_t)r * r);
-        jacobi_pinv(A, V, r, atol, rtol, J.U, r, J.sigma, &s_flag, w, &kept, &smax);
+        jacobi_pinv(s_a, s_v, r, atol, rtol, J.U, r, J.sigma, s_flag, s_w, &kept, &smax);
         if (threadIdx.x == 0) {
             out[jb].kept = kept;
             out[jb].sigma_max = smax;
@@ -243,14 +232,46 @@ __global__ void k_pinv(const BlockJob* __restrict__ jobs, const int32_t* __restr
     }
 }
 
-void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, int lds, int threads,
-                 JobOut* out, hipStream_t s) {
+// Global-memory variant (any r): A = the W panel, V = the job's scratch; the
+// weights stay in LDS up to r = 1024, beyond that after V (pinv_v_elems).
+__global__ void k_pinv_global(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist,
+                              double atol, double rtol, JobOut* __restrict__ out) {
+    __shared__ double s_w[1024];
+    __shared__ int s_flag;
+    for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
+        const int jb = list[li];
+        const BlockJob& J = jobs[jb];
+        const int r = J.r;
+        double* w = r <= 1024 ? s_w : (double*)(J.V + (size_t)r * r);
+        int kept;
+        double smax;
+        jacobi_pinv(J.W, J.V, r, atol, rtol, J.U, r, J.sigma, &s_flag, w, &kept, &smax);
+        if (threadIdx.x == 0) {
+            out[jb].kept = kept;
+            out[jb].sigma_max = smax;
+        }
+        __syncthreads();
+    }
+}
+
+size_t pinv_lds_bytes(int rmax) { return (size_t)32 * rmax * rmax + (size_t)8 * rmax + 16; }
+
+void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, int rmax,
+                 int threads, JobOut* out, hipStream_t s) {
     if (nlist <= 0) return;
-    const int grid = nlist < 8192 ? nlist : 8192;
-    if (lds)
-        hipLaunchKernelGGL(k_pinv<true>, dim3(grid), dim3(threads), 0, s, jobs, list, nlist, atol, rtol, out);
-    else
-        hipLaunchKernelGGL(k_pinv<false>, dim3(grid), dim3(threads), 0, s, jobs, list, nlist, atol, rtol, out);
+    const int grid = nlist < 16384 ? nlist : 16384;
+    if (rmax > 0 && rmax <= kPinvLdsMaxR) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            (void)hipFuncSetAttribute((const void*)k_pinv_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)pinv_lds_bytes(kPinvLdsMaxR));
+            attr_set = true;
+        }
+        hipLaunchKernelGGL(k_pinv_lds, dim3(grid), dim3(threads), pinv_lds_bytes(rmax), s, jobs, list, nlist, rmax,
+                           atol, rtol, out);
+    } else {
+        hipLaunchKernelGGL(k_pinv_global, dim3(grid), dim3(threads), 0, s, jobs, list, nlist, atol, rtol, out);
+    }
 }
 
 // ---------------------------------------------------------------- residual

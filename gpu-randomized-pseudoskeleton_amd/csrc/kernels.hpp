@@ -59,14 +59,20 @@ void launch_pivot_rows(const BlockJob* jobs, int njobs, int f32, hipStream_t s);
 enum : int { kPanelC = 1, kPanelW = 2, kPanelR = 4 };
 void launch_fill(const BlockJob* jobs, int njobs, const int32_t* tile_prefix, int total_tiles, int panels,
                  const FillParams& fp, int grid, hipStream_t s);
-// Pinv of W for the listed jobs (indices into jobs). lds = 1 needs r <= kPinvLdsMaxR.
-constexpr int kPinvLdsMaxR = 44;
+// Pinv of W for the listed jobs (indices into jobs). rmax in 1..kPinvLdsMaxR
+// selects the LDS variant sized for rmax (every listed job has r <= rmax);
+// rmax = 0 selects the global-memory variant.
+constexpr int kPinvLdsMaxR = 64;
+constexpr int kPinvClasses = 5;  // r in (0,16], (16,32], (32,48], (48,64], > 64
+constexpr int kPinvClassMax[kPinvClasses] = {16, 32, 48, 64, 0};
+inline int pinv_class(int r) { return r <= 16 ? 0 : r <= 32 ? 1 : r <= 48 ? 2 : r <= 64 ? 3 : 4; }
+size_t pinv_lds_bytes(int rmax);
 // Complex elements of the Jacobi scratch V for sample count r (V plus, past
 // r = 1024, the r weights).
 inline size_t pinv_v_elems(int r) { return (size_t)r * r + (size_t)(r + 1) / 2; }
 // Largest sample count any entry point accepts.
 constexpr int kMaxSamples = 8192;
-void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, int lds,
+void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, int rmax,
                  int threads, JobOut* out, hipStream_t s);
 void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, JobOut* out, hipStream_t s);
 

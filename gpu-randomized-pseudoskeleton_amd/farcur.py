@@ -77,6 +77,14 @@ class BlockInfo(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("pinv_threads", ci * 5), ("fill_grid", ci), ("resid_threads", ci), ("pinv_ms", cd * 5)]
+
+    def as_dict(self):
+        return {"pinv_threads": list(self.pinv_threads), "fill_grid": self.fill_grid,
+                "resid_threads": self.resid_threads, "pinv_ms": list(self.pinv_ms)}
+
+
 def _setup(L: ctypes.CDLL) -> None:
     L.farcur_last_error.restype = ctypes.c_char_p
     L.farcur_version.restype = ctypes.c_char_p
@@ -114,6 +122,10 @@ def _setup(L: ctypes.CDLL) -> None:
         [vp, vp, vp, ci, vp, cd, vp, vp, vp, vp, vp] + [ci] * 12 + [vp, ci, cd, cd, cd, cd, cd, ci, ci] + [vp] * 10
         + [ci] * 18 + [ci] * 6
     )
+    L.farcur_autotune.argtypes = [vp, vp]
+    L.farcur_get_tuning.argtypes = [vp, vp]
+    L.farcur_set_tuning.argtypes = [vp, vp]
+    L.farcur_pinv_bench.argtypes = [vp, ci, ci, ci, ci, vp]
     if hasattr(L, "farcur_batch_create"):
         L.farcur_batch_create.argtypes = [vp, ci, vp, vp, vp, vp, vp]
         L.farcur_batch_destroy.argtypes = [vp]
@@ -338,6 +350,33 @@ class Context:
         out = np.empty(r, dtype=np.int32)
         check(lib().farcur_pivot_rows(self._h, _p(C), m, r, _p(out)))
         return out
+
+    def autotune(self) -> dict:
+        t = Tuning()
+        check(lib().farcur_autotune(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        check(lib().farcur_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, pinv_threads=None, fill_grid=None, resid_threads=None):
+        t = Tuning()
+        check(lib().farcur_get_tuning(self._h, ctypes.byref(t)))
+        if pinv_threads is not None:
+            for i, v in enumerate(pinv_threads):
+                t.pinv_threads[i] = v
+        if fill_grid is not None:
+            t.fill_grid = fill_grid
+        if resid_threads is not None:
+            t.resid_threads = resid_threads
+        check(lib().farcur_set_tuning(self._h, ctypes.byref(t)))
+
+    def pinv_bench(self, r, nprob, threads, use_lds) -> float:
+        ms = cd()
+        check(lib().farcur_pinv_bench(self._h, r, nprob, threads, 1 if use_lds else 0, ctypes.byref(ms)))
+        return ms.value
 
     def compress_block(self, rows, cols, **opts):
         rows, cols = _ivec(rows), _ivec(cols)

@@ -234,6 +234,26 @@ int farcur_batch_profile(farcur_batch* batch, int enable);
 int farcur_batch_kernel_times(farcur_batch* batch, const char** names, double* ms, int* launches,
                               double* evals, int cap);
 
+/* ------------------------------------------------------------------ tuning
+ * Launch shapes of the batched kernels. The reference fixes its thread
+ * blocks from a `divfactor` sweep (codes/GPU_implementation/run_experiment.sh:8,
+ * compres_single_block_spheres_for_sweep2.jl:574-587); here farcur_autotune
+ * times candidate shapes on the device and keeps the fastest per context.
+ * pinv classes: sample counts (0,16], (16,32], (32,48], (48,64], > 64. */
+typedef struct {
+    int pinv_threads[5];
+    int fill_grid;
+    int resid_threads;
+    double pinv_ms[5]; /* autotune: best time of the class's probe (ms) */
+} farcur_tuning;
+
+int farcur_autotune(farcur_ctx* ctx, farcur_tuning* out);
+int farcur_get_tuning(farcur_ctx* ctx, farcur_tuning* out);
+int farcur_set_tuning(farcur_ctx* ctx, const farcur_tuning* in);
+/* Device time (ms, best of 3) of the pinv of nprob random r x r problems with
+ * `threads` per workgroup, LDS (use_lds = 1, r <= 64) or global variant. */
+int farcur_pinv_bench(farcur_ctx* ctx, int r, int nprob, int threads, int use_lds, double* ms);
+
 #ifdef __cplusplus
 }
 #endif
