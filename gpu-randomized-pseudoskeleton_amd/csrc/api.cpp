@@ -137,12 +137,13 @@ inline int r_start_rule(int D) { return D < 10 ? 1 : D / 10; }
 
 // Per-kernel-class device timing by event pairs.
 struct KTimer {
-    static constexpr int kClasses = 4;  // sample, fill, pinv, residual
+    // sample, fill, pinv (all), residual, then pinv per r class (4 + c)
+    static constexpr int kClasses = 4 + kPinvClasses;
     std::vector<hipEvent_t> pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
     size_t used = 0;
-    double ms[kClasses] = {0, 0, 0, 0};
-    int launches[kClasses] = {0, 0, 0, 0};
+    double ms[kClasses] = {};
+    int launches[kClasses] = {};
     bool on = false;
     hipEvent_t get() {
         if (used == pool.size()) {
@@ -171,6 +172,10 @@ struct KTimer {
             hipEventElapsedTime(&t, mk.second.first, mk.second.second);
             ms[mk.first] += t;
             launches[mk.first] += 1;
+            if (mk.first >= 4) {  // a pinv class also counts towards "pinv"
+                ms[2] += t;
+                launches[2] += 1;
+            }
         }
         marks.clear();
         used = 0;
@@ -386,10 +391,12 @@ void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt) {
         launch_fill(P.jobs, P.njobs, P.prefix_all, P.total_all, kPanelC | kPanelW | kPanelR, fp, P.fill_grid, s);
         kt.end(1, s, a);
     }
-    kt.begin(2, s, &a);
-    for (int c = 0; c < kPinvClasses; ++c)
+    for (int c = 0; c < kPinvClasses; ++c) {
+        if (P.nlist[c] == 0) continue;
+        kt.begin(4 + c, s, &a);
         launch_pinv(P.jobs, P.list[c], P.nlist[c], P.atol, P.rtol, kPinvClassMax[c], P.pinv_threads[c], P.out, s);
-    kt.end(2, s, a);
+        kt.end(4 + c, s, a);
+    }
     kt.begin(3, s, &a);
     launch_residual(P.jobs, P.njobs, P.f32, P.resid_threads, P.out, s);
     kt.end(3, s, a);
@@ -1243,7 +1250,7 @@ struct farcur_batch {
     bool has_result = false;
     // Jobs of every round of the last compress (host copies), for accounting.
     std::vector<std::vector<BlockJob>> rounds;
-    double work[KTimer::kClasses] = {0, 0, 0, 0};
+    double work[KTimer::kClasses] = {};
 };
 
 namespace {
@@ -1385,6 +1392,7 @@ int batch_account(farcur_batch* B) {
             const double trs = distinct_triangles(ctx->tri_of_rwg, rs.data(), J.r, scratch);
             B->work[1] += 9.0 * (trows[J.b] * tcs + trs * tcs + trs * tcols[J.b]);
             B->work[2] += 96.0 * (double)J.r * J.r * J.r;
+            B->work[4 + pinv_class(J.r)] += 96.0 * (double)J.r * J.r * J.r;
             B->work[3] += ps * ((double)J.m * J.r + (double)J.r * J.n) + 16.0 * J.r * J.r + 16.0 * (J.n + 2.0 * J.m);
         }
     }
@@ -1552,7 +1560,9 @@ extern "C" int farcur_batch_block(farcur_batch* B, int b, farcur_c128* C, farcur
 extern "C" int farcur_batch_kernel_times(farcur_batch* B, const char** names, double* ms, int* launches,
                                          double* work, int cap) {
     if (!B) return fail(FARCUR_EINVAL, "NULL batch");
-    static const char* kNames[KTimer::kClasses] = {"sample", "fill", "pinv", "residual"};
+    static const char* kNames[KTimer::kClasses] = {"sample",       "fill",         "pinv",         "residual",
+                                                   "pinv_r16",     "pinv_r32",     "pinv_r48",     "pinv_r64",
+                                                   "pinv_global"};
     int n = 0;
     for (int i = 0; i < KTimer::kClasses && n < cap; ++i, ++n) {
         if (names) names[n] = kNames[i];

@@ -452,7 +452,7 @@ class Batch:
         check(lib().farcur_batch_profile(self._h, 1 if enable else 0))
 
     def kernel_times(self) -> dict:
-        cap = 8
+        cap = 16
         names = (ctypes.c_char_p * cap)()
         ms = np.zeros(cap)
         launches = np.zeros(cap, dtype=np.int32)
