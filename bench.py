@@ -82,10 +82,39 @@ def cpu_baseline(mesh, blocks, opts, budget_s=15.0):
             "sample": f"first {done} of {len(ro) - 1} far blocks, FP64 oracle, {dt:.1f} s"}
 
 
+def accuracy(fc, ctx, batch, blocks, nsample, seed=0):
+    ro, ri, co, cix = blocks
+    nb = len(ro) - 1
+    rng = np.random.default_rng(seed)
+    errs = []
+    for b in rng.choice(nb, min(nsample, nb), replace=False):
+        rows, cols = ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]]
+        out = batch.block(int(b))
+        Z = ctx.fill(rows, cols, fc.F64)
+        errs.append(np.linalg.norm(out["C"] @ out["U"] @ out["R"] - Z) / np.linalg.norm(Z))
+    return float(np.max(errs)), float(np.median(errs))
+
+
+def rank_model(fc, ctx, log):
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("rank_model", ROOT / "gpu-randomized-pseudoskeleton_amd" / "rank_model.py")
+    rm = importlib.util.module_from_spec(spec)
+    sys.modules["rank_model"] = rm
+    spec.loader.exec_module(rm)
+    if rm.DEFAULT_MODEL.exists():
+        log(f"# rank model: {rm.DEFAULT_MODEL.relative_to(ROOT)}")
+        return rm.load(fc, ctx), str(rm.DEFAULT_MODEL.relative_to(ROOT))
+    log("# rank model: no trained model file, training a small one (40k blocks)")
+    X, y = rm.training_set(fc, ctx, 40_000, seed=1, levels=(4,), log=lambda s: None)
+    model, _ = rm.train(fc, ctx, X, y, epochs=40, log=lambda s: None)
+    return model, "trained in bench (40k blocks, level-4 ellipsoids)"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
     ap.add_argument("--level", type=int, default=5)
@@ -97,6 +126,7 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(far-field blocks shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    log = lambda s: print(s, flush=True)
 
     fc = _load_farcur()
     mesh, part = build_workload(fc, level=args.level)
@@ -107,55 +137,58 @@ def main():
     n = np.diff(co)
     represented = float(np.sum(m.astype(np.float64) * n))
     ctx = fc.Context(0)
+    tuning = ctx.autotune()
+    log(f"# autotune: {tuning}")
+    model, model_src = rank_model(fc, ctx, log)
     ctx.set_mesh(mesh)
     prec = fc.F32 if args.precision == "f32" else fc.F64
-    opts = dict(tol=1e-4, seed=12345, precision=prec, pinv_rtol=1e-6 if prec == fc.F32 else 0.0)
-    tuning = ctx.autotune()
-    print(f"# autotune: {tuning}", flush=True)
-    batch = fc.Batch(ctx, ro, ri, co, cix)
-    print(f"# workload: {mesh.sizes()[2]} RWGs, {part.ngroups} groups, {nb} far blocks "
-          f"(sizes {m.min()}-{max(m.max(), n.max())}), represented {represented:.3e} entries", flush=True)
+    opts = dict(tol=1e-4, seed=12345, precision=prec, pinv_rtol=1e-6 if prec == fc.F32 else 0.0, heldout=8,
+                oversample=10)
+    log(f"# workload: {mesh.sizes()[2]} RWGs, {part.ngroups} groups, {nb} far blocks "
+        f"(sizes {m.min()}-{max(m.max(), n.max())}), represented {represented:.3e} entries")
+
+    # headline: learned rank + batched rounds + complex-FP32 fill
+    learned = fc.Batch(ctx, ro, ri, co, cix)
+    learned.set_model(model)
     for _ in range(args.warmup):
-        batch.compress(fc.MODE_BATCHED, **opts)
-    walls = []
-    for _ in range(args.steps):
-        walls.append(batch.compress(fc.MODE_BATCHED, **opts))
+        learned.compress(fc.MODE_LEARNED, **opts)
+    walls = [learned.compress(fc.MODE_LEARNED, **opts) for _ in range(args.steps)]
     ms = float(np.mean(walls))
-    info = batch.info_arrays()
-    r = info["r"]
+    il = learned.info_arrays()
+    r = il["r"]
     filled = float(np.sum(m * r + r * r + r * n))
-    print(f"# batched: {ms:.2f} ms/step, rounds max {info['rounds'].max()}, r mean {r.mean():.1f}", flush=True)
+    misses = learned.misses()
+    log(f"# learned+batched: {ms:.2f} ms/step (steps {', '.join(f'{w:.1f}' for w in walls)}), "
+        f"rounds mean {il['rounds'].mean():.3f}, r mean {r.mean():.1f}, miss rate {misses / nb:.4f}")
 
-    # Kernel device times and algorithmic work (one extra, profiled step).
-    batch.profile(True)
-    prof_wall = batch.compress(fc.MODE_BATCHED, **opts)
-    kt = batch.kernel_times()
-    batch.profile(False)
+    learned.profile(True)
+    prof_wall = learned.compress(fc.MODE_LEARNED, **opts)
+    kt = learned.kernel_times()
+    learned.profile(False)
     for k, v in kt.items():
-        print(f"# kernel {k:12s} {v['ms']:9.3f} ms  launches {v['launches']:4d}  work {v['work']:.3e}", flush=True)
+        log(f"# kernel {k:12s} {v['ms']:9.3f} ms  launches {v['launches']:4d}  work {v['work']:.3e}")
+    err_max, err_med = accuracy(fc, ctx, learned, blocks, args.check_blocks)
+    log(f"# learned accuracy vs FP64 dense ({args.check_blocks} blocks): max {err_max:.2e} median {err_med:.2e}")
 
-    # Accuracy: sampled blocks against the FP64 dense block (GPU FP64 fill).
-    rng = np.random.default_rng(0)
-    check = rng.choice(nb, min(args.check_blocks, nb), replace=False)
-    errs = []
-    for b in check:
-        rows, cols = ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]]
-        out = batch.block(int(b))
-        Z = ctx.fill(rows, cols, fc.F64)
-        errs.append(np.linalg.norm(out["C"] @ out["U"] @ out["R"] - Z) / np.linalg.norm(Z))
+    # the same without the network: batched reference loop
+    batched = fc.Batch(ctx, ro, ri, co, cix)
+    batched.compress(fc.MODE_BATCHED, **opts)
+    bw = float(np.mean([batched.compress(fc.MODE_BATCHED, **opts) for _ in range(2)]))
+    bmax, bmed = accuracy(fc, ctx, batched, blocks, args.check_blocks)
+    log(f"# batched plain loop: {bw:.2f} ms/step, accuracy max {bmax:.2e} median {bmed:.2e}")
 
     plain = None
     if not args.no_plain:
-        t0 = time.perf_counter()
-        pw = batch.compress(fc.MODE_PLAIN, **opts)
+        pw = batched.compress(fc.MODE_PLAIN, **opts)
         plain = {"blocks_per_s": nb / (pw * 1e-3), "ms": pw}
-        print(f"# plain per-block loop: {pw:.1f} ms", flush=True)
+        log(f"# plain per-block loop: {pw:.1f} ms")
 
     cpu = None if args.no_cpu else cpu_baseline(mesh, blocks, opts)
 
+    value = nb / (ms * 1e-3)
     res = {
-        "metric": "far-field CUR blocks/s (all admissible blocks, relerr<=1e-4)",
-        "value": nb / (ms * 1e-3),
+        "metric": "far-field CUR blocks/s (all admissible blocks, relerr<=1e-4), learned+batched",
+        "value": value,
         "unit": "blocks/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -167,14 +200,17 @@ def main():
         "dtype": "c64 fill / f64 core" if prec == fc.F32 else "f64",
         "data": "synthetic (seeded jittered icosphere)",
         "config": {"workload": f"icosphere R=2 lambda level {args.level} ({mesh.sizes()[2]} RWG), 0.5 lambda "
-                               f"leaves, {nb} far blocks, tol 1e-4, batched rounds",
-                   "blocks": nb, "represented_entries": represented},
+                               f"leaves, {nb} far blocks, tol 1e-4, learned rank + batched rounds",
+                   "blocks": nb, "represented_entries": represented, "rank_model": model_src},
         "represented_entries_per_s": represented / (ms * 1e-3),
         "filled_entries_per_s": filled / (ms * 1e-3),
-        "max_block_relerr_sampled": float(np.max(errs)),
-        "median_block_relerr_sampled": float(np.median(errs)),
+        "miss_rate": misses / nb,
+        "max_block_relerr_sampled": err_max,
+        "median_block_relerr_sampled": err_med,
+        "batched_plain": {"blocks_per_s": nb / (bw * 1e-3), "ms": bw, "max_block_relerr_sampled": bmax},
         "plain_per_block": plain,
-        "speedup_vs_plain": (nb / (ms * 1e-3)) / plain["blocks_per_s"] if plain else None,
+        "speedup_vs_plain_per_block": value / plain["blocks_per_s"] if plain else None,
+        "speedup_vs_batched_plain": value / (nb / (bw * 1e-3)),
         "profiled_step_ms": prof_wall,
         "roofline": roofline(kt),
         "kernels": kt,

@@ -19,6 +19,7 @@
 #include "element.hpp"
 #include "geometry.hpp"
 #include "kernels.hpp"
+#include "mlp.hpp"
 #include "philox.hpp"
 
 using namespace farcur;
@@ -135,6 +136,18 @@ void launch_ones(c128* y, size_t n, hipStream_t s) {
 
 inline int r_start_rule(int D) { return D < 10 ? 1 : D / 10; }
 
+// splitmix64 for host-side shuffles and initialisation.
+struct SplitMixHost {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
 // Per-kernel-class device timing by event pairs.
 struct KTimer {
     // sample, fill, pinv (all), residual, then pinv per r class (4 + c)
@@ -207,6 +220,9 @@ struct farcur_ctx {
     double eta = 120.0 * M_PI;
     int ne = 0, nt = 0;
     std::vector<int32_t> tri_of_rwg;  // 2 * ne: T+, T- (host, for accounting)
+    std::vector<double> h_vertex;     // host copies for block features
+    std::vector<int32_t> h_topol, h_edges;
+    std::vector<double> h_ln;
     DBuf rec64, rec32, cen;
     uint64_t geom_hash = 0;
     // single-block scratch
@@ -326,6 +342,10 @@ int set_geometry_impl(farcur_ctx* ctx, const std::vector<double>& vertex, const 
     HIPTRY(hipStreamSynchronize(ctx->stream));
     ctx->ne = ne;
     ctx->nt = nt;
+    ctx->h_vertex = vertex;
+    ctx->h_topol = topol;
+    ctx->h_edges = edges;
+    ctx->h_ln = ln;
     ctx->tri_of_rwg.resize(2 * (size_t)ne);
     for (int e = 0; e < ne; ++e) {
         ctx->tri_of_rwg[2 * e] = edges[4 * e];
@@ -352,7 +372,7 @@ struct RoundPlan {
     int fill_grid = 0;
     int pinv_threads[kPinvClasses] = {64, 64, 128, 256, 256};
     int resid_threads = 256;
-    double atol = 1e-10, rtol = 0;
+    double atol = 1e-10, rtol = 0, label_eps = 1e-4;
     uint64_t seed = 0;
     // device pointers
     const BlockJob* jobs = nullptr;
@@ -384,17 +404,19 @@ void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt) {
         launch_pivot_rows(P.jobs, P.njobs, P.f32, s);
         kt.end(0, s, a);
         kt.begin(1, s, &a);
-        launch_fill(P.jobs, P.njobs, P.prefix_wr, P.total_wr, kPanelW | kPanelR, fp, P.fill_grid, s);
+        launch_fill(P.jobs, P.njobs, P.prefix_wr, P.total_wr, kPanelW | kPanelR | kPanelH, fp, P.fill_grid, s);
         kt.end(1, s, a);
     } else {
         kt.begin(1, s, &a);
-        launch_fill(P.jobs, P.njobs, P.prefix_all, P.total_all, kPanelC | kPanelW | kPanelR, fp, P.fill_grid, s);
+        launch_fill(P.jobs, P.njobs, P.prefix_all, P.total_all, kPanelC | kPanelW | kPanelR | kPanelH, fp,
+                    P.fill_grid, s);
         kt.end(1, s, a);
     }
     for (int c = 0; c < kPinvClasses; ++c) {
         if (P.nlist[c] == 0) continue;
         kt.begin(4 + c, s, &a);
-        launch_pinv(P.jobs, P.list[c], P.nlist[c], P.atol, P.rtol, kPinvClassMax[c], P.pinv_threads[c], P.out, s);
+        launch_pinv(P.jobs, P.list[c], P.nlist[c], P.atol, P.rtol, P.label_eps, kPinvClassMax[c], P.pinv_threads[c],
+                    P.out, s);
         kt.end(4 + c, s, a);
     }
     kt.begin(3, s, &a);
@@ -420,9 +442,10 @@ void pack_round(const std::vector<BlockJob>& jobs, bool pivot, PackedRound& pk) 
     pwr[0] = 0;
     for (int j = 0; j < nj; ++j) {
         const BlockJob& J = jobs[j];
-        const int all = pivot ? fill_tiles(J.m, J.n, J.r, kPanelC) : fill_tiles(J.m, J.n, J.r, kPanelC | kPanelW | kPanelR);
+        const int all = pivot ? fill_tiles(J.m, J.n, J.r, J.h, kPanelC)
+                              : fill_tiles(J.m, J.n, J.r, J.h, kPanelC | kPanelW | kPanelR | kPanelH);
         pa[j + 1] = pa[j] + all;
-        pwr[j + 1] = pwr[j] + (pivot ? fill_tiles(J.m, J.n, J.r, kPanelW | kPanelR) : 0);
+        pwr[j + 1] = pwr[j] + (pivot ? fill_tiles(J.m, J.n, J.r, J.h, kPanelW | kPanelR | kPanelH) : 0);
         lists[pinv_class(J.r)].push_back(j);
     }
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -474,6 +497,9 @@ int validate_opts(const farcur_opts* o) {
     if (o->precision != FARCUR_F64 && o->precision != FARCUR_F32) return fail(FARCUR_EINVAL, "bad precision");
     if (o->row_mode != FARCUR_ROWS_UNIFORM && o->row_mode != FARCUR_ROWS_PIVOT) return fail(FARCUR_EINVAL, "bad row_mode");
     if (o->r_start < 0 || o->max_rank < 0) return fail(FARCUR_EINVAL, "r_start and max_rank must be >= 0");
+    if (o->heldout < 0 || o->heldout > 256) return fail(FARCUR_EINVAL, "heldout must be in [0, 256]");
+    if (o->oversample < 0) return fail(FARCUR_EINVAL, "oversample must be >= 0");
+    if (!(o->label_eps >= 0)) return fail(FARCUR_EINVAL, "label_eps must be >= 0");
     return FARCUR_OK;
 }
 
@@ -507,6 +533,9 @@ void farcur_opts_default(farcur_opts* o) {
     o->seed = 12345;
     o->precision = FARCUR_F64;
     o->row_mode = FARCUR_ROWS_UNIFORM;
+    o->heldout = 8;
+    o->oversample = 10;
+    o->label_eps = 1e-4;
 }
 
 // ------------------------------------------------------------------ meshes
@@ -722,7 +751,7 @@ int farcur_pinv(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, doubl
     Plan* dp = ctx->d_plan.as<Plan>();
     const int cls = pinv_class(r);
     const int threads = cls < kPinvClasses - 1 ? ctx->tune.pinv_threads[cls] : (r > 128 ? 1024 : 256);
-    launch_pinv(&dp->job, dp->list, 1, atol, rtol, kPinvClassMax[cls], threads, &dp->out, ctx->stream);
+    launch_pinv(&dp->job, dp->list, 1, atol, rtol, 0.0, kPinvClassMax[cls], threads, &dp->out, ctx->stream);
     HIPTRY(hipGetLastError());
     HIPTRY(hipMemcpyAsync(U, ctx->d_U.p, rr, hipMemcpyDeviceToHost, ctx->stream));
     std::vector<double> s(r);
@@ -863,6 +892,8 @@ struct RoundBufs {
     void *C, *R;
     c128 *W, *V, *U;
     double* sigma;
+    int32_t *hs, *hpos;  // held-out rows (learned mode)
+    void* H;
 };
 
 struct BlockLoopIO {
@@ -925,6 +956,7 @@ int block_loop(farcur_ctx* ctx, const BlockLoopIO& io, const farcur_opts& o, All
         P.seed = o.seed;
         const bool big = (size_t)io.m * r > (size_t)256 * 1024 || r > 128;
         P.tune(ctx->tune);
+        P.label_eps = o.label_eps;
         if (r > kPinvLdsMaxR) P.pinv_threads[kPinvClasses - 1] = r > 128 ? 1024 : 256;
         if (big) P.resid_threads = 1024;
         HIPTRY(hipEventRecord(ctx->ev0, ctx->stream));
@@ -968,6 +1000,7 @@ int block_loop(farcur_ctx* ctx, const BlockLoopIO& io, const farcur_opts& o, All
     info->heldout_err = -1;
     info->sigma_max = res.sigma_max;
     info->time_ms = dev_ms;
+    info->rank_eps = res.rank_eps;
     *final_y = ycur;
     return FARCUR_OK;
 }
@@ -1008,7 +1041,8 @@ extern "C" int farcur_compress_block(farcur_ctx* ctx, const int* rows, int m, co
         TRY(ctx->d_U.ensure((size_t)r * r * 16));
         TRY(ctx->d_sigma.ensure(r * 8));
         rb = RoundBufs{ctx->d_cs.as<int32_t>(), ctx->d_rs.as<int32_t>(), ctx->d_C.p,         ctx->d_R.p,
-                       ctx->d_W.as<c128>(),    ctx->d_V.as<c128>(),    ctx->d_U.as<c128>(), ctx->d_sigma.as<double>()};
+                       ctx->d_W.as<c128>(),    ctx->d_V.as<c128>(),    ctx->d_U.as<c128>(), ctx->d_sigma.as<double>(),
+                       nullptr,                nullptr,                nullptr};
         return FARCUR_OK;
     };
     KTimer kt;
@@ -1212,14 +1246,80 @@ extern "C" double cuda_complete_compression(
 }
 
 // ------------------------------------------------------------------ batches
+int model_rank_predict(farcur_model* M, const float* d_X, const int32_t* d_cap, int n, int p, int32_t* d_r,
+                       hipStream_t s);
+
 namespace {
+
+// Rank-model features of the block rows x cols (see farcur_batch_features).
+void block_features(const farcur_ctx* ctx, const int32_t* rows, int m, const int32_t* cols, int n, float* f) {
+    const double lambda = 2.0 * M_PI / ctx->k;
+    auto group = [&](const int32_t* ids, int cnt, double c[3], double nrm[3], double* diam, double* hsum) {
+        c[0] = c[1] = c[2] = 0;
+        nrm[0] = nrm[1] = nrm[2] = 0;
+        *hsum = 0;
+        for (int i = 0; i < cnt; ++i) {
+            const int e = ids[i];
+            const int tp = ctx->h_edges[4 * e], vp = ctx->h_edges[4 * e + 2];
+            double mid[3] = {0, 0, 0};
+            for (int k = 0; k < 3; ++k) {
+                const int v = ctx->h_topol[3 * tp + k];
+                if (v == vp) continue;
+                for (int x = 0; x < 3; ++x) mid[x] += 0.5 * ctx->h_vertex[3 * v + x];
+            }
+            for (int x = 0; x < 3; ++x) c[x] += mid[x] / cnt;
+            *hsum += ctx->h_ln[e];
+            for (int hh = 0; hh < 2; ++hh) {
+                const int t = ctx->h_edges[4 * e + hh];
+                const double* v1 = &ctx->h_vertex[3 * ctx->h_topol[3 * t]];
+                const double* v2 = &ctx->h_vertex[3 * ctx->h_topol[3 * t + 1]];
+                const double* v3 = &ctx->h_vertex[3 * ctx->h_topol[3 * t + 2]];
+                const double a[3] = {v3[0] - v1[0], v3[1] - v1[1], v3[2] - v1[2]};
+                const double b[3] = {v2[0] - v1[0], v2[1] - v1[1], v2[2] - v1[2]};
+                nrm[0] += a[1] * b[2] - a[2] * b[1];
+                nrm[1] += a[2] * b[0] - a[0] * b[2];
+                nrm[2] += a[0] * b[1] - a[1] * b[0];
+            }
+        }
+        double r2 = 0;
+        for (int i = 0; i < cnt; ++i) {
+            for (int hh = 0; hh < 2; ++hh) {
+                const int t = ctx->h_edges[4 * ids[i] + hh];
+                for (int k = 0; k < 3; ++k) {
+                    const double* v = &ctx->h_vertex[3 * ctx->h_topol[3 * t + k]];
+                    const double d2 = (v[0] - c[0]) * (v[0] - c[0]) + (v[1] - c[1]) * (v[1] - c[1]) +
+                                      (v[2] - c[2]) * (v[2] - c[2]);
+                    r2 = std::max(r2, d2);
+                }
+            }
+        }
+        *diam = 2.0 * std::sqrt(r2);
+        const double nn = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+        for (int x = 0; x < 3; ++x) nrm[x] = nn > 0 ? nrm[x] / nn : 0.0;
+    };
+    double ca[3], na[3], da, ha, cb[3], nb[3], db, hb;
+    group(rows, m, ca, na, &da, &ha);
+    group(cols, n, cb, nb, &db, &hb);
+    double d[3] = {cb[0] - ca[0], cb[1] - ca[1], cb[2] - ca[2]};
+    const double dist = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    for (int x = 0; x < 3; ++x) d[x] = dist > 0 ? d[x] / dist : 0.0;
+    f[0] = (float)(da / lambda);
+    f[1] = (float)(db / lambda);
+    f[2] = (float)(dist / lambda);
+    f[3] = (float)(na[0] * d[0] + na[1] * d[1] + na[2] * d[2]);
+    f[4] = (float)(-(nb[0] * d[0] + nb[1] * d[1] + nb[2] * d[2]));
+    f[5] = (float)(ctx->k * (ha + hb) / (m + n));
+    f[6] = (float)std::log((double)m);
+    f[7] = (float)std::log((double)n);
+}
 
 struct BState {
     int m = 0, n = 0, cap = 0, r = 0, round = 0;
     int status = 0;  // 0 active, 1 converged, 2 stopped at the cap, 3 NaN residual
     int ycur = 0;
-    int kept = 0;
-    double err = INFINITY, smax = 0;
+    int kept = 0, rank_eps = 0;
+    int h = 0;       // held-out rows of the current round (learned mode)
+    double err = INFINITY, smax = 0, heldout_err = -1;
     RoundBufs fin{};
     int fin_r = 0;
 };
@@ -1251,6 +1351,12 @@ struct farcur_batch {
     // Jobs of every round of the last compress (host copies), for accounting.
     std::vector<std::vector<BlockJob>> rounds;
     double work[KTimer::kClasses] = {};
+    // Rank-model features (nb x 8) and the model used by the learned mode.
+    std::vector<float> feat;
+    DBuf d_feat, d_cap, d_rpred;
+    farcur_model* model = nullptr;
+    bool learned = false;
+    int first_round_misses = 0;
 };
 
 namespace {
@@ -1278,6 +1384,20 @@ int batch_alloc_round(farcur_batch* B, int r, int m, int n, RoundBufs& rb) {
     rb.U = (c128*)p;
     TRY(B->arena.alloc((size_t)r * 8, &p));
     rb.sigma = (double*)p;
+    rb.hs = rb.hpos = nullptr;
+    rb.H = nullptr;
+    return FARCUR_OK;
+}
+
+int batch_alloc_heldout(farcur_batch* B, int h, int n, RoundBufs& rb) {
+    if (h <= 0) return FARCUR_OK;
+    void* p;
+    TRY(B->arena.alloc((size_t)h * 4, &p));
+    rb.hs = (int32_t*)p;
+    TRY(B->arena.alloc((size_t)h * 4, &p));
+    rb.hpos = (int32_t*)p;
+    TRY(B->arena.alloc((size_t)h * n * psize(B->f32), &p));
+    rb.H = p;
     return FARCUR_OK;
 }
 
@@ -1289,7 +1409,11 @@ BlockJob make_job(farcur_batch* B, int b, const RoundBufs& rb, const farcur_opts
     J.n = s.n;
     J.r = s.r;
     J.round = s.round;
-    J.sample_mode = o.row_mode == FARCUR_ROWS_PIVOT ? 2 : 0;
+    J.sample_mode = B->learned ? 3 : (o.row_mode == FARCUR_ROWS_PIVOT ? 2 : 0);
+    J.h = B->learned ? s.h : 0;
+    J.hs = rb.hs;
+    J.hpos = rb.hpos;
+    J.H = rb.H;
     J.rows = B->d_rows.as<int32_t>() + B->row_off[b];
     J.cols = B->d_cols.as<int32_t>() + B->col_off[b];
     J.cs = rb.cs;
@@ -1308,12 +1432,40 @@ BlockJob make_job(farcur_batch* B, int b, const RoundBufs& rb, const farcur_opts
     return J;
 }
 
+// Learned mode: the predicted round is accepted when its held-out rows meet
+// tol; otherwise the sample count doubles (capped) and the block runs again.
+void advance_learned(BState& s, const JobOut& out, const RoundBufs& rb, double tol, int heldout) {
+    s.fin = rb;
+    s.fin_r = s.r;
+    s.kept = out.kept;
+    s.rank_eps = out.rank_eps;
+    s.smax = out.sigma_max;
+    s.err = s.h > 0 ? out.err : 0.0;
+    s.heldout_err = s.h > 0 ? out.err : -1.0;
+    s.round += 1;
+    if (s.h == 0 || s.err <= tol) {
+        s.status = 1;
+        return;
+    }
+    if (!(s.err == s.err)) {
+        s.status = 3;
+        return;
+    }
+    if (s.r >= s.cap) {
+        s.status = 2;
+        return;
+    }
+    s.r = std::min(2 * s.r, s.cap);
+    s.h = std::min(heldout, s.m - s.r);
+}
+
 // Applies one round's result to a block's state (cuda_cur_compress3 control flow).
 void advance(BState& s, const JobOut& out, const RoundBufs& rb, double tol) {
     s.ycur ^= 1;
     s.fin = rb;
     s.fin_r = s.r;
     s.kept = out.kept;
+    s.rank_eps = out.rank_eps;
     s.smax = out.sigma_max;
     if (s.round == 0) {
         int rn;
@@ -1345,6 +1497,7 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
     for (size_t i = 0; i < active.size(); ++i) {
         const int b = active[i];
         TRY(batch_alloc_round(B, B->st[b].r, B->st[b].m, B->st[b].n, bufs[i]));
+        if (B->learned) TRY(batch_alloc_heldout(B, B->st[b].h, B->st[b].n, bufs[i]));
         jobs.push_back(make_job(B, b, bufs[i], o));
     }
     PackedRound pk;
@@ -1357,6 +1510,7 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
     P.atol = o.pinv_tol;
     P.rtol = o.pinv_rtol;
     P.seed = o.seed;
+    P.label_eps = o.label_eps;
     P.tune(ctx->tune);
     run_round(ctx, P, B->kt);
     HIPTRY(hipGetLastError());
@@ -1434,6 +1588,15 @@ extern "C" int farcur_batch_create(farcur_ctx* ctx, int nblocks, const int* row_
     HIPTRY(hipMemcpyAsync(B->d_vo.p, col_off, (size_t)(nblocks + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
     HIPTRY(hipStreamSynchronize(ctx->stream));
     B->st.resize(nblocks);
+    B->feat.resize((size_t)nblocks * kMlpFeatures);
+    for (int b = 0; b < nblocks; ++b)
+        block_features(ctx, row_ids + row_off[b], row_off[b + 1] - row_off[b], col_ids + col_off[b],
+                       col_off[b + 1] - col_off[b], &B->feat[(size_t)b * kMlpFeatures]);
+    TRY(B->d_feat.ensure(B->feat.size() * 4));
+    TRY(B->d_cap.ensure((size_t)nblocks * 4));
+    TRY(B->d_rpred.ensure((size_t)nblocks * 4));
+    HIPTRY(hipMemcpyAsync(B->d_feat.p, B->feat.data(), B->feat.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
     *out = B.release();
     return FARCUR_OK;
 }
@@ -1442,7 +1605,8 @@ extern "C" void farcur_batch_destroy(farcur_batch* B) {
     if (!B) return;
     hipSetDevice(B->ctx->device);
     hipStreamSynchronize(B->ctx->stream);
-    DBuf* bufs[] = {&B->d_rows, &B->d_cols, &B->d_v, &B->d_ya, &B->d_yb, &B->d_vo, &B->d_plan};
+    DBuf* bufs[] = {&B->d_rows, &B->d_cols, &B->d_v,    &B->d_ya,  &B->d_yb,
+                    &B->d_vo,   &B->d_plan, &B->d_feat, &B->d_cap, &B->d_rpred};
     for (DBuf* b : bufs) b->release();
     B->arena.release();
     delete B;
@@ -1457,11 +1621,16 @@ extern "C" int farcur_batch_profile(farcur_batch* B, int enable) {
 extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, int mode, double* wall_ms) {
     if (!B) return fail(FARCUR_EINVAL, "NULL batch");
     TRY(validate_opts(opts));
-    if (mode != FARCUR_MODE_PLAIN && mode != FARCUR_MODE_BATCHED) return fail(FARCUR_EINVAL, "bad mode %d", mode);
+    if (mode != FARCUR_MODE_PLAIN && mode != FARCUR_MODE_BATCHED && mode != FARCUR_MODE_LEARNED)
+        return fail(FARCUR_EINVAL, "bad mode %d", mode);
+    if (mode == FARCUR_MODE_LEARNED && opts->row_mode != FARCUR_ROWS_UNIFORM)
+        return fail(FARCUR_EINVAL, "learned mode samples rows uniformly (row_mode must be FARCUR_ROWS_UNIFORM)");
     farcur_ctx* ctx = B->ctx;
     TRY(ctx->select());
     const farcur_opts o = *opts;
     B->f32 = o.precision == FARCUR_F32;
+    B->learned = mode == FARCUR_MODE_LEARNED;
+    B->first_round_misses = 0;
     B->has_result = false;
     B->rounds.clear();
     B->kt.clear();
@@ -1484,7 +1653,45 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
     launch_vtest(B->d_v.as<c128>(), B->d_vo.as<int32_t>(), B->nb, o.seed, st);
     launch_ones(B->d_ya.as<c128>(), (size_t)B->row_off[B->nb], st);
     HIPTRY(hipGetLastError());
-    if (mode == FARCUR_MODE_BATCHED) {
+    if (mode == FARCUR_MODE_LEARNED) {
+        // Sample counts from the rank model (device inference), then rounds of
+        // every block whose held-out rows still miss tol.
+        // Without a model the first round uses the reference's rule (the
+        // method without the network, for comparison).
+        std::vector<int32_t> cap(B->nb), rp(B->nb);
+        for (int b = 0; b < B->nb; ++b) cap[b] = B->st[b].cap;
+        if (B->model) {
+            HIPTRY(hipMemcpyAsync(B->d_cap.p, cap.data(), B->nb * 4, hipMemcpyHostToDevice, st));
+            TRY(model_rank_predict(B->model, B->d_feat.as<float>(), B->d_cap.as<int32_t>(), B->nb, o.oversample,
+                                   B->d_rpred.as<int32_t>(), st));
+            HIPTRY(hipMemcpyAsync(rp.data(), B->d_rpred.p, B->nb * 4, hipMemcpyDeviceToHost, st));
+            HIPTRY(hipStreamSynchronize(st));
+        } else {
+            for (int b = 0; b < B->nb; ++b) rp[b] = B->st[b].r;
+        }
+        std::vector<int> active(B->nb);
+        for (int b = 0; b < B->nb; ++b) {
+            BState& s = B->st[b];
+            s.r = std::max(1, std::min(rp[b], s.cap));
+            s.h = std::min(o.heldout, s.m - s.r);
+            active[b] = b;
+        }
+        std::vector<RoundBufs> bufs;
+        std::vector<JobOut> outs;
+        bool first = true;
+        while (!active.empty()) {
+            TRY(batch_round(B, active, o, bufs, outs));
+            std::vector<int> next;
+            for (size_t i = 0; i < active.size(); ++i) {
+                BState& s = B->st[active[i]];
+                advance_learned(s, outs[i], bufs[i], o.tol, o.heldout);
+                if (s.status == 0) next.push_back(active[i]);
+            }
+            if (first) B->first_round_misses = (int)next.size();
+            first = false;
+            active.swap(next);
+        }
+    } else if (mode == FARCUR_MODE_BATCHED) {
         std::vector<int> active(B->nb);
         for (int b = 0; b < B->nb; ++b) active[b] = b;
         std::vector<RoundBufs> bufs;
@@ -1531,9 +1738,11 @@ extern "C" int farcur_batch_info(farcur_batch* B, farcur_block_info* infos) {
         I.kept = s.kept;
         I.converged = s.status == 1;
         I.err = s.err;
-        I.heldout_err = -1;
+        I.heldout_err = s.heldout_err;
         I.sigma_max = s.smax;
         I.time_ms = 0;
+        I.rank_eps = s.rank_eps;
+        I.pad = 0;
     }
     return FARCUR_OK;
 }
@@ -1617,7 +1826,7 @@ int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* 
     for (int rep = 0; rep < 3; ++rep) {
         HIPTRY(hipMemcpyAsync(W.p, W0.p, nprob * rr * 16, hipMemcpyDeviceToDevice, s));
         HIPTRY(hipEventRecord(ctx->ev0, s));
-        launch_pinv((const BlockJob*)d, (const int32_t*)(d + bj), nprob, 1e-10, 0.0, rmax, threads,
+        launch_pinv((const BlockJob*)d, (const int32_t*)(d + bj), nprob, 1e-10, 0.0, 0.0, rmax, threads,
                     (JobOut*)(d + bj + bl + 256), s);
         HIPTRY(hipEventRecord(ctx->ev1, s));
         HIPTRY(hipGetLastError());
@@ -1690,3 +1899,252 @@ extern "C" int farcur_set_tuning(farcur_ctx* ctx, const farcur_tuning* in) {
     ctx->tune.resid_threads = in->resid_threads;
     return FARCUR_OK;
 }
+
+// ------------------------------------------------------------------ rank model
+struct farcur_model {
+    farcur_ctx* ctx = nullptr;
+    DBuf params, m, v, grad, loss, X, y, idx, partial, ploss;
+    MlpNorm norm{};
+    bool has_norm = false;
+    int step = 0;
+};
+
+int model_rank_predict(farcur_model* M, const float* d_X, const int32_t* d_cap, int n, int p, int32_t* d_r,
+                       hipStream_t s) {
+    if (!M->has_norm) return fail(FARCUR_ESTATE, "model has no normalisation (train or set it first)");
+    launch_rank_predict(M->params.as<float>(), M->norm, d_X, d_cap, n, p, d_r, s);
+    HIPTRY(hipGetLastError());
+    return FARCUR_OK;
+}
+
+namespace {
+
+void norm_from_data(const float* X, const float* y, int n, MlpNorm& nrm) {
+    for (int c = 0; c < kMlpFeatures; ++c) {
+        double s1 = 0, s2 = 0;
+        for (int i = 0; i < n; ++i) {
+            const double x = X[(size_t)i * kMlpFeatures + c];
+            s1 += x;
+            s2 += x * x;
+        }
+        const double mean = s1 / n, var = std::max(0.0, s2 / n - mean * mean);
+        nrm.xm[c] = (float)mean;
+        nrm.xs[c] = (float)std::max(1e-6, std::sqrt(var));
+    }
+    double s1 = 0, s2 = 0;
+    for (int i = 0; i < n; ++i) {
+        s1 += y[i];
+        s2 += (double)y[i] * y[i];
+    }
+    const double mean = s1 / n, var = std::max(0.0, s2 / n - mean * mean);
+    nrm.ym = (float)mean;
+    nrm.ys = (float)std::max(1e-6, std::sqrt(var));
+}
+
+int model_upload_data(farcur_model* M, const float* X, const float* y, int n) {
+    hipStream_t s = M->ctx->stream;
+    TRY(M->X.ensure((size_t)n * kMlpFeatures * 4));
+    TRY(M->y.ensure((size_t)n * 4));
+    HIPTRY(hipMemcpyAsync(M->X.p, X, (size_t)n * kMlpFeatures * 4, hipMemcpyHostToDevice, s));
+    if (y) HIPTRY(hipMemcpyAsync(M->y.p, y, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    return FARCUR_OK;
+}
+
+int check_data(const float* X, const float* y, int n) {
+    for (size_t i = 0; i < (size_t)n * kMlpFeatures; ++i)
+        if (!std::isfinite(X[i])) return fail(FARCUR_EINVAL, "non-finite feature at %zu", i);
+    if (y)
+        for (int i = 0; i < n; ++i)
+            if (!std::isfinite(y[i])) return fail(FARCUR_EINVAL, "non-finite target at %d", i);
+    return FARCUR_OK;
+}
+
+}  // namespace
+
+extern "C" int farcur_model_create(farcur_ctx* ctx, uint64_t seed, farcur_model** out) {
+    if (!ctx || !out) return fail(FARCUR_EINVAL, "NULL argument");
+    TRY(ctx->select());
+    auto M = std::make_unique<farcur_model>();
+    M->ctx = ctx;
+    // Glorot-uniform weights, zero biases.
+    std::vector<float> p(kMlpParams, 0.0f);
+    SplitMixHost rng{seed};
+    auto fill_w = [&](int off, int fan_out, int fan_in) {
+        const double a = std::sqrt(6.0 / (fan_in + fan_out));
+        for (int i = 0; i < fan_out * fan_in; ++i) p[off + i] = (float)((2.0 * rng.uniform() - 1.0) * a);
+    };
+    const int H = kMlpHidden, F = kMlpFeatures;
+    fill_w(0, H, F);
+    fill_w(H * F + H, H, H);
+    fill_w(H * F + H + H * H + H, 1, H);
+    TRY(M->params.ensure(kMlpParams * 4));
+    TRY(M->m.ensure(kMlpParams * 4));
+    TRY(M->v.ensure(kMlpParams * 4));
+    TRY(M->grad.ensure(kMlpParams * 4));
+    HIPTRY(hipMemcpyAsync(M->params.p, p.data(), kMlpParams * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPTRY(hipMemsetAsync(M->m.p, 0, kMlpParams * 4, ctx->stream));
+    HIPTRY(hipMemsetAsync(M->v.p, 0, kMlpParams * 4, ctx->stream));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    *out = M.release();
+    return FARCUR_OK;
+}
+
+extern "C" void farcur_model_destroy(farcur_model* M) {
+    if (!M) return;
+    hipSetDevice(M->ctx->device);
+    hipStreamSynchronize(M->ctx->stream);
+    DBuf* bufs[] = {&M->params, &M->m, &M->v, &M->grad, &M->loss, &M->X, &M->y, &M->idx, &M->partial, &M->ploss};
+    for (DBuf* b : bufs) b->release();
+    delete M;
+}
+
+extern "C" int farcur_model_get(farcur_model* M, float* params, float* norm) {
+    if (!M) return fail(FARCUR_EINVAL, "NULL model");
+    TRY(M->ctx->select());
+    if (params) {
+        HIPTRY(hipMemcpyAsync(params, M->params.p, kMlpParams * 4, hipMemcpyDeviceToHost, M->ctx->stream));
+        HIPTRY(hipStreamSynchronize(M->ctx->stream));
+    }
+    if (norm) {
+        for (int c = 0; c < kMlpFeatures; ++c) {
+            norm[c] = M->norm.xm[c];
+            norm[kMlpFeatures + c] = M->norm.xs[c];
+        }
+        norm[2 * kMlpFeatures] = M->norm.ym;
+        norm[2 * kMlpFeatures + 1] = M->norm.ys;
+    }
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_model_set(farcur_model* M, const float* params, const float* norm) {
+    if (!M) return fail(FARCUR_EINVAL, "NULL model");
+    TRY(M->ctx->select());
+    if (params) {
+        for (int i = 0; i < kMlpParams; ++i)
+            if (!std::isfinite(params[i])) return fail(FARCUR_EINVAL, "non-finite parameter %d", i);
+        HIPTRY(hipMemcpyAsync(M->params.p, params, kMlpParams * 4, hipMemcpyHostToDevice, M->ctx->stream));
+        HIPTRY(hipMemsetAsync(M->m.p, 0, kMlpParams * 4, M->ctx->stream));
+        HIPTRY(hipMemsetAsync(M->v.p, 0, kMlpParams * 4, M->ctx->stream));
+        HIPTRY(hipStreamSynchronize(M->ctx->stream));
+        M->step = 0;
+    }
+    if (norm) {
+        for (int c = 0; c < kMlpFeatures; ++c) {
+            if (!(norm[kMlpFeatures + c] > 0)) return fail(FARCUR_EINVAL, "feature scale %d must be > 0", c);
+            M->norm.xm[c] = norm[c];
+            M->norm.xs[c] = norm[kMlpFeatures + c];
+        }
+        if (!(norm[2 * kMlpFeatures + 1] > 0)) return fail(FARCUR_EINVAL, "target scale must be > 0");
+        M->norm.ym = norm[2 * kMlpFeatures];
+        M->norm.ys = norm[2 * kMlpFeatures + 1];
+        M->has_norm = true;
+    }
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_model_loss_grad(farcur_model* M, const float* X, const float* y, int n, float* loss,
+                                      float* grad) {
+    if (!M || !X || !y || n <= 0) return fail(FARCUR_EINVAL, "bad arguments");
+    TRY(check_data(X, y, n));
+    TRY(M->ctx->select());
+    hipStream_t s = M->ctx->stream;
+    if (!M->has_norm) {
+        norm_from_data(X, y, n, M->norm);
+        M->has_norm = true;
+    }
+    TRY(model_upload_data(M, X, y, n));
+    std::vector<int32_t> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    const int ntiles = (n + 63) / 64;
+    TRY(M->idx.ensure((size_t)n * 4));
+    TRY(M->partial.ensure((size_t)ntiles * kMlpParams * 4));
+    TRY(M->ploss.ensure((size_t)ntiles * 4));
+    TRY(M->loss.ensure(4));
+    HIPTRY(hipMemcpyAsync(M->idx.p, idx.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+    launch_mlp_train_tiles(M->params.as<float>(), M->norm, M->X.as<float>(), M->y.as<float>(), M->idx.as<int32_t>(),
+                           n, M->partial.as<float>(), M->ploss.as<float>(), s);
+    launch_mlp_reduce(M->partial.as<float>(), M->ploss.as<float>(), n, M->grad.as<float>(), M->loss.as<float>(), s);
+    HIPTRY(hipGetLastError());
+    if (grad) HIPTRY(hipMemcpyAsync(grad, M->grad.p, kMlpParams * 4, hipMemcpyDeviceToHost, s));
+    float l = 0;
+    HIPTRY(hipMemcpyAsync(&l, M->loss.p, 4, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    if (loss) *loss = l;
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_model_train(farcur_model* M, const float* X, const float* y, int n, int epochs, int batch,
+                                  float lr, uint64_t seed, float* epoch_loss) {
+    if (!M || !X || !y || n <= 0 || epochs <= 0 || batch <= 0 || !(lr > 0)) return fail(FARCUR_EINVAL, "bad arguments");
+    TRY(check_data(X, y, n));
+    TRY(M->ctx->select());
+    hipStream_t s = M->ctx->stream;
+    if (!M->has_norm) {
+        norm_from_data(X, y, n, M->norm);
+        M->has_norm = true;
+    }
+    TRY(model_upload_data(M, X, y, n));
+    batch = std::min(batch, n);
+    const int ntiles = (batch + 63) / 64;
+    const int steps_per_epoch = (n + batch - 1) / batch;
+    TRY(M->idx.ensure((size_t)n * 4));
+    TRY(M->partial.ensure((size_t)ntiles * kMlpParams * 4));
+    TRY(M->ploss.ensure((size_t)ntiles * 4));
+    TRY(M->loss.ensure((size_t)steps_per_epoch * 4));
+    std::vector<int32_t> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    SplitMixHost rng{seed};
+    std::vector<float> losses(steps_per_epoch);
+    for (int ep = 0; ep < epochs; ++ep) {
+        for (int i = n - 1; i > 0; --i) std::swap(idx[i], idx[(int)(rng.next() % (uint64_t)(i + 1))]);
+        HIPTRY(hipMemcpyAsync(M->idx.p, idx.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+        for (int st = 0; st < steps_per_epoch; ++st) {
+            const int off = st * batch, nb = std::min(batch, n - off);
+            launch_mlp_train_tiles(M->params.as<float>(), M->norm, M->X.as<float>(), M->y.as<float>(),
+                                   M->idx.as<int32_t>() + off, nb, M->partial.as<float>(), M->ploss.as<float>(), s);
+            launch_mlp_reduce(M->partial.as<float>(), M->ploss.as<float>(), nb, M->grad.as<float>(),
+                              M->loss.as<float>() + st, s);
+            launch_adam(M->params.as<float>(), M->grad.as<float>(), M->m.as<float>(), M->v.as<float>(), ++M->step, lr,
+                        s);
+        }
+        HIPTRY(hipGetLastError());
+        HIPTRY(hipMemcpyAsync(losses.data(), M->loss.p, steps_per_epoch * 4, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        if (epoch_loss) {
+            double a = 0;
+            for (int st = 0; st < steps_per_epoch; ++st) a += losses[st] * std::min(batch, n - st * batch);
+            epoch_loss[ep] = (float)(a / n);
+        }
+    }
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_model_predict(farcur_model* M, const float* X, int n, float* y) {
+    if (!M || !X || !y || n <= 0) return fail(FARCUR_EINVAL, "bad arguments");
+    if (!M->has_norm) return fail(FARCUR_ESTATE, "model has no normalisation");
+    TRY(check_data(X, nullptr, n));
+    TRY(M->ctx->select());
+    hipStream_t s = M->ctx->stream;
+    TRY(model_upload_data(M, X, nullptr, n));
+    TRY(M->y.ensure((size_t)n * 4));
+    launch_mlp_predict(M->params.as<float>(), M->norm, M->X.as<float>(), n, M->y.as<float>(), s);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(y, M->y.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_batch_features(farcur_batch* B, float* X) {
+    if (!B || !X) return fail(FARCUR_EINVAL, "NULL argument");
+    std::copy(B->feat.begin(), B->feat.end(), X);
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_batch_set_model(farcur_batch* B, farcur_model* M) {
+    if (!B) return fail(FARCUR_EINVAL, "NULL batch");
+    if (M && M->ctx != B->ctx) return fail(FARCUR_EINVAL, "model and batch belong to different contexts");
+    B->model = M;
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_batch_misses(farcur_batch* B) { return B ? B->first_round_misses : -1; }

@@ -36,10 +36,12 @@ __device__ __forceinline__ void circle_pair(int n, int step, int pi, int& p, int
 // global). On exit U (ld = ldu) holds W^+ with singular values below
 // tau = max(atol, rtol * sigma_max) dropped, sigma[j] the singular value carried
 // by column j (unsorted), *kept_out the number of sigma >= tau and *smax_out
-// sigma_max. s_flag: one int of LDS; s_w: r doubles of LDS or global.
+// sigma_max, *rank_eps_out the number of sigma >= label_eps * sigma_max.
+// s_flag: one int of LDS; s_w: r doubles of LDS or global.
 __device__ __forceinline__ void jacobi_pinv(c128* A, c128* V, int r, double atol, double rtol,
                                             c128* __restrict__ U, int ldu, double* __restrict__ sigma, int* s_flag,
-                                            double* s_w, int* kept_out, double* smax_out) {
+                                            double* s_w, int* kept_out, double* smax_out, double label_eps,
+                                            int* rank_eps_out) {
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int idx = tid; idx < r * r; idx += nt) {
         const int i = idx % r, j = idx / r;
@@ -116,8 +118,11 @@ __device__ __forceinline__ void jacobi_pinv(c128* A, c128* V, int r, double atol
     double smax = 0;
     for (int j = 0; j < r; ++j) smax = fmax(smax, sigma[j]);
     const double tau = fmax(atol, rtol * smax);
-    int kept = 0;
-    for (int j = 0; j < r; ++j) kept += (sigma[j] >= tau && s_w[j] > 0);
+    int kept = 0, reps = 0;
+    for (int j = 0; j < r; ++j) {
+        kept += (sigma[j] >= tau && s_w[j] > 0);
+        reps += (sigma[j] >= label_eps * smax && smax > 0);
+    }
     __syncthreads();
     for (int j = tid; j < r; j += nt) {
         const double nn = s_w[j];
@@ -141,6 +146,7 @@ __device__ __forceinline__ void jacobi_pinv(c128* A, c128* V, int r, double atol
     }
     *kept_out = kept;
     *smax_out = smax;
+    *rank_eps_out = reps;
     __syncthreads();
 }
 

@@ -18,6 +18,51 @@ namespace farcur {
 // block rows, by selection sampling; output is sorted by position in the block
 // and mapped to RWG ids (compres_single_block_spheres_for_sweep2.jl:492-497:
 // sample(1:NN, r, replace=false), sort, r2[samples]).
+// Words of stream (block, round, purpose) in groups of four.
+__device__ __forceinline__ U4 stream4(uint64_t seed, uint32_t block, uint32_t round, uint32_t purpose, uint32_t t4) {
+    U4 c;
+    c.x = t4 >> 2;
+    c.y = block;
+    c.z = (round << 8) | purpose;
+    c.w = 0x46435552u;
+    return philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// Held-out rows first (h of m, stream kStreamHeldout), then r sample rows among
+// the other m - h positions (stream kStreamRows, word index = candidate index).
+__device__ void sample_rows_heldout(const BlockJob& J, uint64_t seed) {
+    const uint32_t m = J.m, h = J.h, r = J.r;
+    uint32_t got = 0;
+    for (uint32_t t4 = 0; t4 < m && got < h; t4 += 4) {
+        const U4 o = stream4(seed, J.b, J.round, kStreamHeldout, t4);
+        const uint32_t w[4] = {o.x, o.y, o.z, o.w};
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            const uint32_t tt = t4 + l;
+            if (tt < m && got < h && select_step(w[l], m - tt, h - got)) {
+                J.hpos[got] = tt;
+                J.hs[got] = J.rows[tt];
+                ++got;
+            }
+        }
+    }
+    const uint32_t N = m - h;
+    uint32_t cand = 0, hk = 0;
+    got = 0;
+    U4 o{};
+    for (uint32_t t = 0; t < m && got < r; ++t) {
+        if (hk < h && (uint32_t)J.hpos[hk] == t) {
+            ++hk;
+            continue;
+        }
+        if ((cand & 3u) == 0) o = stream4(seed, J.b, J.round, kStreamRows, cand);
+        const uint32_t l = cand & 3u;
+        const uint32_t u = l == 0 ? o.x : (l == 1 ? o.y : (l == 2 ? o.z : o.w));
+        if (select_step(u, N - cand, r - got)) J.rs[got++] = J.rows[t];
+        ++cand;
+    }
+}
+
 __global__ void k_sample(const BlockJob* __restrict__ jobs, int njobs, uint64_t seed) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = t >> 1, kind = t & 1;
@@ -25,6 +70,10 @@ __global__ void k_sample(const BlockJob* __restrict__ jobs, int njobs, uint64_t 
     const BlockJob& J = jobs[j];
     if (J.sample_mode == 1) return;
     if (kind == 1 && J.sample_mode == 2) return;
+    if (kind == 1 && J.sample_mode == 3) {
+        sample_rows_heldout(J, seed);
+        return;
+    }
     const uint32_t N = kind ? J.m : J.n;
     const int32_t* ids = kind ? J.rows : J.cols;
     int32_t* out = kind ? J.rs : J.cs;
@@ -101,11 +150,12 @@ void launch_pivot_rows(const BlockJob* jobs, int njobs, int f32, hipStream_t s) 
 // ---------------------------------------------------------------- fill
 __host__ __device__ static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-int fill_tiles(int m, int n, int r, int panels) {
+int fill_tiles(int m, int n, int r, int h, int panels) {
     int t = 0;
     if (panels & kPanelC) t += cdiv(m, kTileRows) * cdiv(r, kTileCols);
     if (panels & kPanelW) t += cdiv(r, kTileRows) * cdiv(r, kTileCols);
     if (panels & kPanelR) t += cdiv(r, kTileRows) * cdiv(n, kTileCols);
+    if ((panels & kPanelH) && h > 0) t += cdiv(h, kTileRows) * cdiv(n, kTileCols);
     return t;
 }
 
@@ -164,7 +214,17 @@ __global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restric
             }
             local -= nW;
         }
-        fill_tile<T, PC>((PC*)J.R, r, J.rs, r, J.cols, n, (local % rt) * kTileRows, (local / rt) * kTileCols, rec,
+        if (panels & kPanelR) {
+            const int nR = rt * cdiv(n, kTileCols);
+            if (local < nR) {
+                fill_tile<T, PC>((PC*)J.R, r, J.rs, r, J.cols, n, (local % rt) * kTileRows, (local / rt) * kTileCols,
+                                 rec, cen, ne, fc, lds);
+                continue;
+            }
+            local -= nR;
+        }
+        const int ht = cdiv(J.h, kTileRows);
+        fill_tile<T, PC>((PC*)J.H, J.h, J.hs, J.h, J.cols, n, (local % ht) * kTileRows, (local / ht) * kTileCols, rec,
                          cen, ne, fc, lds);
     }
 }
@@ -210,7 +270,7 @@ void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32
 // LDS variant: A, V and the weights of one problem in dynamic LDS sized for
 // the largest r of the launch (r <= kPinvLdsMaxR); one workgroup per problem.
 __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist, int rmax,
-                           double atol, double rtol, JobOut* __restrict__ out) {
+                           double atol, double rtol, double label_eps, JobOut* __restrict__ out) {
     extern __shared__ __align__(16) char smem[];
     c128* s_a = (c128*)smem;
     c128* s_v = s_a + (size_t)rmax * rmax;
@@ -221,12 +281,13 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
         const BlockJob& J = jobs[jb];
         const int r = J.r;
         for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) s_a[idx] = J.W[idx];
-        int kept;
+        int kept, reps;
         double smax;
-        jacobi_pinv(s_a, s_v, r, atol, rtol, J.U, r, J.sigma, s_flag, s_w, &kept, &smax);
+        jacobi_pinv(s_a, s_v, r, atol, rtol, J.U, r, J.sigma, s_flag, s_w, &kept, &smax, label_eps, &reps);
         if (threadIdx.x == 0) {
             out[jb].kept = kept;
             out[jb].sigma_max = smax;
+            out[jb].rank_eps = reps;
         }
         __syncthreads();
     }
@@ -235,7 +296,7 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
 // Global-memory variant (any r): A = the W panel, V = the job's scratch; the
 // weights stay in LDS up to r = 1024, beyond that after V (pinv_v_elems).
 __global__ void k_pinv_global(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist,
-                              double atol, double rtol, JobOut* __restrict__ out) {
+                              double atol, double rtol, double label_eps, JobOut* __restrict__ out) {
     __shared__ double s_w[1024];
     __shared__ int s_flag;
     for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
@@ -243,12 +304,13 @@ __global__ void k_pinv_global(const BlockJob* __restrict__ jobs, const int32_t* 
         const BlockJob& J = jobs[jb];
         const int r = J.r;
         double* w = r <= 1024 ? s_w : (double*)(J.V + (size_t)r * r);
-        int kept;
+        int kept, reps;
         double smax;
-        jacobi_pinv(J.W, J.V, r, atol, rtol, J.U, r, J.sigma, &s_flag, w, &kept, &smax);
+        jacobi_pinv(J.W, J.V, r, atol, rtol, J.U, r, J.sigma, &s_flag, w, &kept, &smax, label_eps, &reps);
         if (threadIdx.x == 0) {
             out[jb].kept = kept;
             out[jb].sigma_max = smax;
+            out[jb].rank_eps = reps;
         }
         __syncthreads();
     }
@@ -256,8 +318,8 @@ __global__ void k_pinv_global(const BlockJob* __restrict__ jobs, const int32_t* 
 
 size_t pinv_lds_bytes(int rmax) { return (size_t)32 * rmax * rmax + (size_t)8 * rmax + 16; }
 
-void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, int rmax,
-                 int threads, JobOut* out, hipStream_t s) {
+void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, double label_eps,
+                 int rmax, int threads, JobOut* out, hipStream_t s) {
     if (nlist <= 0) return;
     const int grid = nlist < 16384 ? nlist : 16384;
     if (rmax > 0 && rmax <= kPinvLdsMaxR) {
@@ -268,14 +330,57 @@ void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double at
             attr_set = true;
         }
         hipLaunchKernelGGL(k_pinv_lds, dim3(grid), dim3(threads), pinv_lds_bytes(rmax), s, jobs, list, nlist, rmax,
-                           atol, rtol, out);
+                           atol, rtol, label_eps, out);
     } else {
-        hipLaunchKernelGGL(k_pinv_global, dim3(grid), dim3(threads), 0, s, jobs, list, nlist, atol, rtol, out);
+        hipLaunchKernelGGL(k_pinv_global, dim3(grid), dim3(threads), 0, s, jobs, list, nlist, atol, rtol, label_eps,
+                           out);
     }
 }
 
 // ---------------------------------------------------------------- residual
 constexpr int kResidMaxR = 1024;
+
+// Relative Frobenius error of the CUR on the held-out rows:
+// ||H - C[hpos, :] U R||_F / ||H||_F, one held-out row at a time.
+template <typename PC>
+__device__ double heldout_error(const BlockJob& J, c128* t, c128* u, double* s_red) {
+    const PC* C = (const PC*)J.C;
+    const PC* R = (const PC*)J.R;
+    const PC* H = (const PC*)J.H;
+    const int m = J.m, n = J.n, r = J.r, tid = threadIdx.x, nt = blockDim.x;
+    double num = 0, den = 0;
+    for (int i = 0; i < J.h; ++i) {
+        const int row = J.hpos[i];
+        for (int l = tid; l < r; l += nt) t[l] = to_c128(C[(size_t)l * m + row]);
+        __syncthreads();
+        for (int k = tid; k < r; k += nt) {  // u_k = sum_l t_l U[l, k]
+            const c128* uc = J.U + (size_t)k * r;
+            double ar = 0, ai = 0;
+            for (int l = 0; l < r; ++l) {
+                ar += t[l].re * uc[l].re - t[l].im * uc[l].im;
+                ai += t[l].re * uc[l].im + t[l].im * uc[l].re;
+            }
+            u[k] = c128{ar, ai};
+        }
+        __syncthreads();
+        for (int j = tid; j < n; j += nt) {  // z_j = sum_k u_k R[k, j]
+            double zr = 0, zi = 0;
+            for (int k = 0; k < r; ++k) {
+                const c128 b = to_c128(R[(size_t)j * r + k]);
+                zr += u[k].re * b.re - u[k].im * b.im;
+                zi += u[k].re * b.im + u[k].im * b.re;
+            }
+            const c128 hv = to_c128(H[(size_t)j * J.h + i]);
+            const double dr = hv.re - zr, di = hv.im - zi;
+            num += dr * dr + di * di;
+            den += hv.re * hv.re + hv.im * hv.im;
+        }
+        __syncthreads();
+    }
+    num = block_sum(num, s_red);
+    den = block_sum(den, s_red);
+    return den > 0 ? sqrt(num / den) : (num > 0 ? INFINITY : 0.0);
+}
 
 template <typename PC>
 __global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, JobOut* __restrict__ out) {
@@ -285,6 +390,12 @@ __global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, JobOut*
     __shared__ double s_red[16];
     for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
         const BlockJob& J = jobs[jb];
+        if (J.h > 0) {
+            const double e = J.r <= kResidMaxR ? heldout_error<PC>(J, s_t1, s_t2, s_red) : heldout_error<PC>(J, J.W, J.W + J.r, s_red);
+            if (threadIdx.x == 0) out[jb].err = e;
+            __syncthreads();
+            continue;
+        }
         // Beyond kResidMaxR the two r-vectors use the W panel (r*r >= 2r),
         // which the pinv has already consumed.
         const double err =

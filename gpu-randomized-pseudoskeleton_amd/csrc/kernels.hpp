@@ -16,8 +16,10 @@ namespace farcur {
 struct BlockJob {
     int32_t b;      // block id (selects the RNG streams)
     int32_t m, n, r, round;
-    int32_t sample_mode;  // 0: draw cols+rows, 1: samples given, 2: draw cols, rows by argmax |C|
-    int32_t pad0, pad1;
+    int32_t sample_mode;  // 0: draw cols+rows, 1: samples given, 2: draw cols, rows by argmax |C|,
+                          // 3: draw h held-out rows, then cols and rows among the rest
+    int32_t h;            // held-out rows (sample_mode 3), 0 otherwise
+    int32_t pad1;
     const int32_t* rows;  // m RWG ids
     const int32_t* cols;  // n RWG ids
     int32_t* cs;          // r sampled col ids (RWG ids)
@@ -31,14 +33,18 @@ struct BlockJob {
     const c128* v;        // n test vector
     const c128* y_old;    // m
     c128* y_new;          // m
+    int32_t* hs;          // h held-out row ids (RWG ids)
+    int32_t* hpos;        // h held-out row positions in the block
+    void* H;              // h x n exact rows Z[hs, cols] (ld h), panel precision
 };
 
 // Per-job results written by the kernels.
 struct JobOut {
-    double err;       // successive residual ||y_old - y|| / ||y||
+    double err;       // successive residual ||y_old - y|| / ||y||, or the held-out
+                      // rows' relative Frobenius error when h > 0
     double sigma_max; // largest singular value of W
     int32_t kept;     // singular values kept by the threshold
-    int32_t pad;
+    int32_t rank_eps; // singular values >= label_eps * sigma_max (rank label)
 };
 
 struct FillParams {
@@ -49,14 +55,14 @@ struct FillParams {
     double k, eta;
 };
 
-// Tiles of the fill for a job of size (m, n, r) restricted to `panels`.
-int fill_tiles(int m, int n, int r, int panels);
+// Tiles of the fill for a job of size (m, n, r, h held-out rows) restricted to `panels`.
+int fill_tiles(int m, int n, int r, int h, int panels);
 
 void launch_sample(const BlockJob* jobs, int njobs, uint64_t seed, hipStream_t s);
 void launch_pivot_rows(const BlockJob* jobs, int njobs, int f32, hipStream_t s);
 // panels: bit 0 = C, bit 1 = W, bit 2 = R. tile_prefix: njobs + 1 device ints
 // (tiles of job j are [tile_prefix[j], tile_prefix[j+1])).
-enum : int { kPanelC = 1, kPanelW = 2, kPanelR = 4 };
+enum : int { kPanelC = 1, kPanelW = 2, kPanelR = 4, kPanelH = 8 };
 void launch_fill(const BlockJob* jobs, int njobs, const int32_t* tile_prefix, int total_tiles, int panels,
                  const FillParams& fp, int grid, hipStream_t s);
 // Pinv of W for the listed jobs (indices into jobs). rmax in 1..kPinvLdsMaxR
@@ -72,8 +78,8 @@ size_t pinv_lds_bytes(int rmax);
 inline size_t pinv_v_elems(int r) { return (size_t)r * r + (size_t)(r + 1) / 2; }
 // Largest sample count any entry point accepts.
 constexpr int kMaxSamples = 8192;
-void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, int rmax,
-                 int threads, JobOut* out, hipStream_t s);
+void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, double label_eps,
+                 int rmax, int threads, JobOut* out, hipStream_t s);
 void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, JobOut* out, hipStream_t s);
 
 // Plain panel fill Z[i + j*ld] = Z(rows[i], cols[j]) (standalone, one panel).

@@ -58,6 +58,8 @@ class Opts(ctypes.Structure):
         ("max_rank", ci),
         ("block_id", ci),
         ("heldout", ci),
+        ("oversample", ci),
+        ("label_eps", cd),
     ]
 
 
@@ -71,10 +73,12 @@ class BlockInfo(ctypes.Structure):
         ("heldout_err", cd),
         ("sigma_max", cd),
         ("time_ms", cd),
+        ("rank_eps", ci),
+        ("pad", ci),
     ]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
 
 
 class Tuning(ctypes.Structure):
@@ -135,6 +139,17 @@ def _setup(L: ctypes.CDLL) -> None:
         L.farcur_batch_block.argtypes = [vp, ci, vp, vp, vp, vp, vp]
         L.farcur_batch_profile.argtypes = [vp, ci]
         L.farcur_batch_kernel_times.argtypes = [vp, vp, vp, vp, vp, ci]
+        L.farcur_batch_features.argtypes = [vp, vp]
+        L.farcur_batch_set_model.argtypes = [vp, vp]
+        L.farcur_batch_misses.argtypes = [vp]
+    L.farcur_model_create.argtypes = [vp, cu64, vp]
+    L.farcur_model_destroy.argtypes = [vp]
+    L.farcur_model_destroy.restype = None
+    L.farcur_model_get.argtypes = [vp, vp, vp]
+    L.farcur_model_set.argtypes = [vp, vp, vp]
+    L.farcur_model_loss_grad.argtypes = [vp, vp, vp, ci, vp, vp]
+    L.farcur_model_train.argtypes = [vp, vp, vp, ci, ci, ci, ctypes.c_float, cu64, vp]
+    L.farcur_model_predict.argtypes = [vp, vp, ci, vp]
 
 
 def check(rc: int) -> None:
@@ -448,6 +463,18 @@ class Batch:
         check(lib().farcur_batch_block(self._h, b, _p(C), _p(U), _p(R), _p(rs), _p(cs)))
         return {"C": C, "U": U, "R": R, "rs": rs, "cs": cs}
 
+    def features(self) -> np.ndarray:
+        X = np.empty((self.nblocks, 8), dtype=np.float32)
+        check(lib().farcur_batch_features(self._h, _p(X)))
+        return X
+
+    def set_model(self, model: "Model | None"):
+        self._model = model
+        check(lib().farcur_batch_set_model(self._h, model._h if model is not None else None))
+
+    def misses(self) -> int:
+        return lib().farcur_batch_misses(self._h)
+
     def profile(self, enable=True):
         check(lib().farcur_batch_profile(self._h, 1 if enable else 0))
 
@@ -460,6 +487,74 @@ class Batch:
         n = lib().farcur_batch_kernel_times(self._h, names, _p(ms), _p(launches), _p(evals), cap)
         return {names[i].decode(): {"ms": float(ms[i]), "launches": int(launches[i]), "work": float(evals[i])}
                 for i in range(n)}
+
+
+# ---------------------------------------------------------------- rank model
+N_PARAMS, N_FEATURES, N_NORM = 4801, 8, 18
+
+
+class Model:
+    """MLP 8 -> 64 -> 64 -> 1 (GELU) predicting log(eps-rank); runs on the GPU."""
+
+    def __init__(self, ctx: Context, seed=1):
+        self.ctx = ctx
+        self._h = vp()
+        check(lib().farcur_model_create(ctx._h, seed, ctypes.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().farcur_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def get(self):
+        p = np.empty(N_PARAMS, dtype=np.float32)
+        nrm = np.empty(N_NORM, dtype=np.float32)
+        check(lib().farcur_model_get(self._h, _p(p), _p(nrm)))
+        return p, nrm
+
+    def set(self, params=None, norm=None):
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        n = None if norm is None else np.ascontiguousarray(norm, dtype=np.float32)
+        check(lib().farcur_model_set(self._h, _p(p) if p is not None else None, _p(n) if n is not None else None))
+
+    def loss_grad(self, X, y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.empty(N_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float()
+        check(lib().farcur_model_loss_grad(self._h, _p(X), _p(y), y.size, ctypes.byref(loss), _p(g)))
+        return loss.value, g
+
+    def train(self, X, y, epochs=20, batch=4096, lr=2e-3, seed=1):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        losses = np.empty(epochs, dtype=np.float32)
+        check(lib().farcur_model_train(self._h, _p(X), _p(y), y.size, epochs, batch, lr, seed, _p(losses)))
+        return losses
+
+    def predict(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.empty(X.shape[0], dtype=np.float32)
+        check(lib().farcur_model_predict(self._h, _p(X), X.shape[0], _p(y)))
+        return y
+
+    def save(self, path):
+        import json
+        p, n = self.get()
+        pathlib.Path(path).write_text(json.dumps({"arch": "mlp-8-64-64-1-gelu", "params": p.tolist(),
+                                                  "norm": n.tolist()}))
+
+    def load(self, path):
+        import json
+        d = json.loads(pathlib.Path(path).read_text())
+        if d.get("arch") != "mlp-8-64-64-1-gelu":
+            raise FarcurError(f"{path}: unknown model architecture {d.get('arch')}")
+        self.set(np.array(d["params"], np.float32), np.array(d["norm"], np.float32))
+        return self
 
 
 # ---------------------------------------------------------------- reference driver, restated

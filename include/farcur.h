@@ -179,7 +179,9 @@ typedef struct {
     int r_start;     /* 0: reference rule r0 = D < 10 ? 1 : D / 10 (SW2:889-894) */
     int max_rank;    /* cap on the sample count (0: min(m, n), the reference cap) */
     int block_id;    /* stream id of a single-block call */
-    int heldout;     /* held-out rows checked after convergence (0: none) */
+    int heldout;     /* held-out rows per block in FARCUR_MODE_LEARNED (default 8) */
+    int oversample;  /* p added to the predicted rank in FARCUR_MODE_LEARNED (default 10) */
+    double label_eps;/* singular values >= label_eps * sigma_max are counted as rank_eps */
 } farcur_opts;
 
 void farcur_opts_default(farcur_opts* opts);
@@ -193,6 +195,8 @@ typedef struct {
     double heldout_err; /* relative Frobenius error on held-out rows (-1 if not checked) */
     double sigma_max;   /* largest singular value of the final W */
     double time_ms;     /* device time of all rounds */
+    int rank_eps;       /* singular values of the final W >= label_eps * sigma_max */
+    int pad;
 } farcur_block_info;
 
 /* The adaptive loop of cuda_cur_compress3 (SW2:878-973) on one block, rows x
@@ -212,7 +216,9 @@ typedef struct farcur_batch farcur_batch;
 
 enum {
     FARCUR_MODE_PLAIN = 0,   /* per-block host loop, one block at a time (drop-in semantics) */
-    FARCUR_MODE_BATCHED = 1  /* every active block advances one round per batched launch set */
+    FARCUR_MODE_BATCHED = 1, /* every active block advances one round per batched launch set */
+    FARCUR_MODE_LEARNED = 2  /* sample count from the rank model (or, with no model set, the reference's
+                                r0 rule), one round checked on held-out rows, doubling fallback */
 };
 
 int farcur_batch_create(farcur_ctx* ctx, int nblocks, const int* row_off, const int* row_ids, const int* col_off,
@@ -226,6 +232,39 @@ int farcur_batch_info(farcur_batch* batch, farcur_block_info* infos);
 /* Copies block b's C, U, R (any may be NULL). */
 int farcur_batch_block(farcur_batch* batch, int b, farcur_c128* C, farcur_c128* U, farcur_c128* R, int* row_samples,
                        int* col_samples);
+
+/* Eight features per block (nblocks x 8, row-major) for the rank model:
+ * diameter of the row group / lambda, diameter of the column group / lambda,
+ * centre distance / lambda, cosines between each group's mean normal and the
+ * direction to the other group, k x mean RWG edge length, log(rows), log(cols). */
+int farcur_batch_features(farcur_batch* batch, float* X);
+
+/* ------------------------------------------------------------------ rank model
+ * MLP 8 -> 64 -> 64 -> 1 (GELU) predicting log(eps-rank) of a block from its
+ * features, trained and run on the GPU by this library's kernels. No
+ * counterpart in the reference, whose sample count always starts at
+ * min(m, n) / 10 (SW2:889-894) and doubles. */
+typedef struct farcur_model farcur_model;
+#define FARCUR_MLP_FEATURES 8
+#define FARCUR_MLP_PARAMS 4801
+#define FARCUR_MLP_NORM 18 /* feature means[8], stds[8], target mean, std */
+
+int farcur_model_create(farcur_ctx* ctx, uint64_t seed, farcur_model** out);
+void farcur_model_destroy(farcur_model* model);
+int farcur_model_get(farcur_model* model, float* params, float* norm);
+int farcur_model_set(farcur_model* model, const float* params, const float* norm);
+/* Mean squared error (normalised target) and its gradient over n samples. */
+int farcur_model_loss_grad(farcur_model* model, const float* X, const float* y, int n, float* loss, float* grad);
+/* Adam on mini-batches; y = log(rank). The normalisation is taken from (X, y)
+ * unless set before. epoch_loss (epochs entries) may be NULL. */
+int farcur_model_train(farcur_model* model, const float* X, const float* y, int n, int epochs, int batch, float lr,
+                       uint64_t seed, float* epoch_loss);
+int farcur_model_predict(farcur_model* model, const float* X, int n, float* y);
+/* Model used by FARCUR_MODE_LEARNED on this batch. */
+int farcur_batch_set_model(farcur_batch* batch, farcur_model* model);
+/* Blocks whose predicted round failed the held-out check in the last learned
+ * compress (the miss count; not an error status). */
+int farcur_batch_misses(farcur_batch* batch);
 
 /* Device time (ms) and launch count of each kernel class over the last
  * batch_compress call with profiling on: names "sample", "fill", "pinv",

@@ -306,6 +306,22 @@ void orc_sample(uint64_t seed, int block, int round, int purpose, int N, int r, 
     }
 }
 
+/* Learned mode: h held-out rows of 0..m-1 (stream purpose 4), then r sample
+ * rows among the remaining m - h positions (purpose 2, word = candidate index). */
+void orc_sample_heldout(uint64_t seed, int block, int round, int m, int h, int r, int32_t* hpos, int32_t* rows) {
+    orc_sample(seed, block, round, 4, m, h, hpos);
+    int got = 0, cand = 0, hk = 0;
+    for (int t = 0; t < m && got < r; ++t) {
+        if (hk < h && hpos[hk] == t) {
+            ++hk;
+            continue;
+        }
+        const uint32_t u = stream_word(seed, (uint32_t)block, (uint32_t)round, 2u, (uint32_t)cand);
+        if ((uint32_t)(((uint64_t)u * (uint32_t)(m - h - cand)) >> 32) < (uint32_t)(r - got)) rows[got++] = t;
+        ++cand;
+    }
+}
+
 static double u01(uint32_t a, uint32_t b) {
     return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
 }

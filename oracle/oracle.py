@@ -95,6 +95,7 @@ def _setup(L: ctypes.CDLL) -> None:
     L.orc_philox.argtypes = [vp, vp, vp]
     L.orc_sample.argtypes = [ctypes.c_uint64, i, i, i, i, i, vp]
     L.orc_vtest.argtypes = [ctypes.c_uint64, i, i, vp]
+    L.orc_sample_heldout.argtypes = [ctypes.c_uint64, i, i, i, i, i, vp, vp]
     L.orc_pivot_rows.argtypes = [vp, i, i, vp]
     L.orc_compress_block.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.orc_compress_block.restype = i
@@ -181,6 +182,14 @@ def sample(seed: int, block: int, rnd: int, purpose: int, N: int, r: int) -> np.
     out = np.empty(r, dtype=np.int32)
     lib().orc_sample(seed, block, rnd, purpose, N, r, _ptr(out))
     return out
+
+
+def sample_heldout(seed: int, block: int, rnd: int, m: int, h: int, r: int):
+    """Returns (held-out positions, sample row positions)."""
+    hp = np.empty(max(h, 1), dtype=np.int32)
+    rows = np.empty(max(r, 1), dtype=np.int32)
+    lib().orc_sample_heldout(seed, block, rnd, m, h, r, _ptr(hp), _ptr(rows))
+    return hp[:h], rows[:r]
 
 
 def vtest(seed: int, block: int, n: int) -> np.ndarray:

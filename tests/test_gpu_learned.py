@@ -1,0 +1,141 @@
+"""Rank predictor on the GPU: gradients against the CPU oracle, training,
+held-out-row sampling parity, and the learned mode against the same
+held-out-checked method without the network."""
+import importlib.util
+import sys
+
+import numpy as np
+import pytest
+
+import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _rank_model(farcur):
+    if "rank_model" in sys.modules:
+        return sys.modules["rank_model"]
+    spec = importlib.util.spec_from_file_location("rank_model", farcur.HERE / "rank_model.py")
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["rank_model"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="module")
+def gctx(farcur):
+    c = farcur.Context(0)
+    yield c
+    c.close()
+
+
+def _data(seed=0, n=300):
+    rng = np.random.default_rng(seed)
+    X = (rng.standard_normal((n, 8)) * [0.3, 0.3, 2.0, 0.5, 0.5, 0.4, 1.0, 1.0]
+         + [0.8, 0.8, 3.0, 0, 0, 0.6, 4.5, 4.5]).astype(np.float32)
+    y = (2.5 + 0.3 * X[:, 0] + 0.3 * X[:, 1] - 0.2 * np.log(X[:, 2] ** 2 + 1) + 0.05 * rng.standard_normal(n))
+    return X, y.astype(np.float32)
+
+
+def test_gradient_matches_oracle(farcur, gctx):
+    X, y = _data()
+    model = farcur.Model(gctx, seed=3)
+    norm = np.concatenate([X.mean(0), X.std(0), [y.mean(), y.std()]]).astype(np.float32)
+    model.set(norm=norm)
+    p, nrm = model.get()
+    loss, g = model.loss_grad(X, y)
+    lo, go = mlp.loss_grad(p, nrm, X, y)
+    assert loss == pytest.approx(lo, rel=1e-4)
+    # FP32 kernels against the FP64 oracle
+    assert np.linalg.norm(g - go) <= 1e-4 * np.linalg.norm(go)
+    np.testing.assert_allclose(model.predict(X), mlp.forward(p, nrm, X), rtol=1e-5, atol=1e-5)
+
+
+def test_training_reduces_loss(farcur, gctx):
+    X, y = _data(1, 4000)
+    model = farcur.Model(gctx, seed=5)
+    losses = model.train(X, y, epochs=30, batch=512, lr=3e-3, seed=2)
+    assert losses[-1] < 0.25 * losses[0]
+    p, nrm = model.get()
+    # the trained parameters give the oracle the same loss
+    lo, _ = mlp.loss_grad(p, nrm, X, y)
+    l_gpu, _ = model.loss_grad(X, y)
+    assert l_gpu == pytest.approx(lo, rel=1e-3)
+
+
+@pytest.fixture(scope="module")
+def ellipsoid(farcur, oracle):
+    # training geometry and a different deployment geometry
+    mesh = farcur.Mesh.icosphere(1.6, 1.2, 1.0, 4, 0.01, 7).build_rwg()
+    part = farcur.Partition(mesh, 0.5, 2.0)
+    t = mesh.tables()
+    g = oracle.Geometry(t["vertex"], t["topol"], t["edges"], t["ln"])
+    return mesh, part, g
+
+
+def test_learned_mode_heldout_sampling_parity(farcur, gctx, ellipsoid, oracle):
+    mesh, part, g = ellipsoid
+    gctx.set_mesh(mesh)
+    ro, ri, co, cix = part.far_blocks()
+    bt = farcur.Batch(gctx, ro, ri, co, cix)
+    bt.compress(farcur.MODE_LEARNED, tol=1e-4, seed=11, heldout=8)  # no model: reference r0 rule
+    info = bt.info_arrays()
+    checked = 0
+    for b in range(0, bt.nblocks, max(1, bt.nblocks // 25)):
+        if info["rounds"][b] != 1:
+            continue
+        m, n, r = ro[b + 1] - ro[b], co[b + 1] - co[b], info["r"][b]
+        rows, cols = ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]]
+        h = min(8, m - r)
+        hp, rsp = oracle.sample_heldout(11, b, 0, m, h, r)
+        out = bt.block(b)
+        np.testing.assert_array_equal(out["rs"], rows[rsp])
+        np.testing.assert_array_equal(out["cs"], cols[oracle.sample(11, b, 0, 1, n, r)])
+        # held-out error as the oracle computes it from the same panels
+        if h > 0:
+            Zh = g.fill(rows[hp], cols)
+            A = out["C"][hp, :] @ out["U"] @ out["R"]
+            e = np.linalg.norm(Zh - A) / np.linalg.norm(Zh)
+            assert e == pytest.approx(info["heldout_err"][b], rel=1e-3, abs=1e-9)
+        checked += 1
+    assert checked >= 5
+
+
+def test_learned_gain_and_accuracy(farcur, gctx, ellipsoid):
+    # Train on blocks of two other ellipsoids, deploy on this one: the network
+    # must cut the rounds and filled entries of the held-out-checked method,
+    # keep the accuracy, and report its miss rate.
+    rm = _rank_model(farcur)
+    Xs, ys = [], []
+    for axes, seed in [((1.4, 1.4, 1.0), 21), ((2.0, 1.0, 1.0), 22), ((1.2, 1.8, 1.5), 23)]:
+        X, y = rm.ellipsoid_blocks(farcur, gctx, axes, 4, seed)
+        Xs.append(X)
+        ys.append(y)
+    X, y = np.concatenate(Xs), np.concatenate(ys)
+    model, losses = rm.train(farcur, gctx, X, y, epochs=60, batch=1024, lr=3e-3, log=lambda s: None)
+    assert losses[-1] < losses[0]
+
+    mesh, part, g = ellipsoid
+    gctx.set_mesh(mesh)
+    ro, ri, co, cix = part.far_blocks()
+    base = farcur.Batch(gctx, ro, ri, co, cix)
+    learned = farcur.Batch(gctx, ro, ri, co, cix)
+    learned.set_model(model)
+    opts = dict(tol=1e-4, seed=3, precision=farcur.F32, pinv_rtol=1e-6, heldout=8)
+    base.compress(farcur.MODE_LEARNED, **opts)
+    learned.compress(farcur.MODE_LEARNED, **opts)
+    ib, il = base.info_arrays(), learned.info_arrays()
+    m, n = np.diff(ro), np.diff(co)
+    filled = lambda info: np.sum((m + n) * info["r"] + info["r"] ** 2)
+    assert il["rounds"].mean() < ib["rounds"].mean()
+    assert learned.misses() / learned.nblocks < 0.5
+    assert np.all(il["converged"] == 1)
+    errs = []
+    for b in range(0, learned.nblocks, max(1, learned.nblocks // 20)):
+        out = learned.block(b)
+        Z = g.fill(ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]])
+        errs.append(np.linalg.norm(out["C"] @ out["U"] @ out["R"] - Z) / np.linalg.norm(Z))
+    assert np.max(errs) <= 3e-4
+    assert np.median(errs) <= 1e-4
+    print(f"rounds {ib['rounds'].mean():.2f} -> {il['rounds'].mean():.2f}, filled {filled(ib):.3e} -> {filled(il):.3e}, "
+          f"miss rate {learned.misses() / learned.nblocks:.3f}, max err {np.max(errs):.2e}")
