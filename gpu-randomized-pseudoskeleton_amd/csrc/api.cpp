@@ -500,6 +500,7 @@ int validate_opts(const farcur_opts* o) {
     if (o->heldout < 0 || o->heldout > 256) return fail(FARCUR_EINVAL, "heldout must be in [0, 256]");
     if (o->oversample < 0) return fail(FARCUR_EINVAL, "oversample must be >= 0");
     if (!(o->label_eps >= 0)) return fail(FARCUR_EINVAL, "label_eps must be >= 0");
+    if (!(o->heldout_safety > 0) || o->heldout_safety > 1) return fail(FARCUR_EINVAL, "heldout_safety must be in (0, 1]");
     return FARCUR_OK;
 }
 
@@ -536,6 +537,7 @@ void farcur_opts_default(farcur_opts* o) {
     o->heldout = 8;
     o->oversample = 10;
     o->label_eps = 1e-4;
+    o->heldout_safety = 0.5;
 }
 
 // ------------------------------------------------------------------ meshes
@@ -1684,7 +1686,7 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
             std::vector<int> next;
             for (size_t i = 0; i < active.size(); ++i) {
                 BState& s = B->st[active[i]];
-                advance_learned(s, outs[i], bufs[i], o.tol, o.heldout);
+                advance_learned(s, outs[i], bufs[i], o.tol * o.heldout_safety, o.heldout);
                 if (s.status == 0) next.push_back(active[i]);
             }
             if (first) B->first_round_misses = (int)next.size();

@@ -60,6 +60,7 @@ class Opts(ctypes.Structure):
         ("heldout", ci),
         ("oversample", ci),
         ("label_eps", cd),
+        ("heldout_safety", cd),
     ]
 
 

@@ -21,6 +21,9 @@ struct Opts
     max_rank::Cint
     block_id::Cint
     heldout::Cint
+    oversample::Cint
+    label_eps::Cdouble
+    heldout_safety::Cdouble
 end
 
 struct BlockInfo
@@ -32,6 +35,8 @@ struct BlockInfo
     heldout_err::Cdouble
     sigma_max::Cdouble
     time_ms::Cdouble
+    rank_eps::Cint
+    pad::Cint
 end
 
 function check(rc)

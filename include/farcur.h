@@ -182,6 +182,7 @@ typedef struct {
     int heldout;     /* held-out rows per block in FARCUR_MODE_LEARNED (default 8) */
     int oversample;  /* p added to the predicted rank in FARCUR_MODE_LEARNED (default 10) */
     double label_eps;/* singular values >= label_eps * sigma_max are counted as rank_eps */
+    double heldout_safety; /* learned mode accepts a round when held-out error <= heldout_safety * tol (0.5) */
 } farcur_opts;
 
 void farcur_opts_default(farcur_opts* opts);

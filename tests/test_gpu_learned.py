@@ -82,21 +82,21 @@ def test_learned_mode_heldout_sampling_parity(farcur, gctx, ellipsoid, oracle):
     info = bt.info_arrays()
     checked = 0
     for b in range(0, bt.nblocks, max(1, bt.nblocks // 25)):
-        if info["rounds"][b] != 1:
-            continue
         m, n, r = ro[b + 1] - ro[b], co[b + 1] - co[b], info["r"][b]
+        last = info["rounds"][b] - 1  # round index of the final samples
         rows, cols = ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]]
         h = min(8, m - r)
-        hp, rsp = oracle.sample_heldout(11, b, 0, m, h, r)
+        hp, rsp = oracle.sample_heldout(11, b, last, m, h, r)
         out = bt.block(b)
         np.testing.assert_array_equal(out["rs"], rows[rsp])
-        np.testing.assert_array_equal(out["cs"], cols[oracle.sample(11, b, 0, 1, n, r)])
+        np.testing.assert_array_equal(out["cs"], cols[oracle.sample(11, b, last, 1, n, r)])
         # held-out error as the oracle computes it from the same panels
         if h > 0:
             Zh = g.fill(rows[hp], cols)
             A = out["C"][hp, :] @ out["U"] @ out["R"]
             e = np.linalg.norm(Zh - A) / np.linalg.norm(Zh)
             assert e == pytest.approx(info["heldout_err"][b], rel=1e-3, abs=1e-9)
+            assert info["converged"][b] == (e <= 0.5e-4) or info["r"][b] == min(m, n)
         checked += 1
     assert checked >= 5
 
