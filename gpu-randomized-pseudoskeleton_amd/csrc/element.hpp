@@ -169,34 +169,43 @@ struct FillTileLds {
     double cen[kTileCols][3];
 };
 
-// Fills rows [row0, row0+64) x cols [col0, col0+16) of the panel
-// out[i + j*ld] = Z(rows[i], cols[j]). Must be called by all 256 threads.
-// F64: FillConsts (k, 4/k^2, k eta / (144 pi)); F32: F32Consts and centres.
-template <typename T, typename OutC, typename Consts>
+// One tile of a panel out[i + j*ld] = Z(rows[i], cols[j]). Lanes run over
+// 64 consecutive entries of one index set (the lane set), the 16 entries of
+// the other set (the LDS set) are staged in LDS and read as broadcasts.
+// kLanesAreRows: lanes over rows (tall panels, C and W); otherwise lanes over
+// columns (wide panels R and H), using reciprocity Z(m, n) = Z(n, m) of the
+// symmetric Galerkin element. Must be called by all 256 threads.
+template <typename T, typename OutC, typename Consts, bool kLanesAreRows>
 __device__ __forceinline__ void fill_tile(OutC* __restrict__ out, int ld, const int* __restrict__ rows, int nr,
                                           const int* __restrict__ cols, int nc, int row0, int col0,
                                           const T* __restrict__ rec, const double* __restrict__ cen, int ne,
                                           const Consts& fc, FillTileLds<T>& lds) {
+    const int* lane_ids = kLanesAreRows ? rows : cols;
+    const int* lds_ids = kLanesAreRows ? cols : rows;
+    const int nlane = kLanesAreRows ? nr : nc;
+    const int nlds = kLanesAreRows ? nc : nr;
+    const int lane0 = kLanesAreRows ? row0 : col0;
+    const int lds0 = kLanesAreRows ? col0 : row0;
     const int tid = threadIdx.x;
     __syncthreads();  // the LDS tile may still be read by a previous tile
     for (int idx = tid; idx < kTileCols * kRecFields; idx += kFillThreads) {
         const int c = idx / kRecFields, f = idx - c * kRecFields;
-        const int j = col0 + c;
-        lds.col[c][f] = (j < nc) ? rec[(size_t)f * ne + cols[j]] : T(0);
+        const int j = lds0 + c;
+        lds.col[c][f] = (j < nlds) ? rec[(size_t)f * ne + lds_ids[j]] : T(0);
     }
     if constexpr (sizeof(T) == 4) {
         if (tid < kTileCols * 3) {
             const int c = tid / 3, x = tid - 3 * c;
-            const int j = col0 + c;
-            lds.cen[c][x] = (j < nc) ? cen[(size_t)x * ne + cols[j]] : 0.0;
+            const int j = lds0 + c;
+            lds.cen[c][x] = (j < nlds) ? cen[(size_t)x * ne + lds_ids[j]] : 0.0;
         }
     }
     __syncthreads();
     const int lane = tid & (kWave - 1);
     const int w = tid / kWave;
-    const int i = row0 + lane;
-    if (i >= nr) return;
-    const int e = rows[i];
+    const int i = lane0 + lane;
+    if (i >= nlane) return;
+    const int e = lane_ids[i];
     RwgRec<T> a;
     load_rec(rec, ne, e, a);
     double ac3[3] = {0, 0, 0};
@@ -208,14 +217,17 @@ __device__ __forceinline__ void fill_tile(OutC* __restrict__ out, int ld, const 
 #pragma unroll 1
     for (int cc = 0; cc < kTileColsPerWave; ++cc) {
         const int c = w * kTileColsPerWave + cc;
-        const int j = col0 + c;
-        if (j >= nc) break;
+        const int j = lds0 + c;
+        if (j >= nlds) break;
         T zr, zi;
         if constexpr (sizeof(T) == 8)
             entry_f64(a, lds.col[c], fc.k, fc.c4k2, fc.scale, zr, zi);
         else
             entry_f32(a, ac3, lds.col[c], lds.cen[c], fc, zr, zi);
-        store_c(&out[(size_t)j * ld + i], zr, zi);
+        if constexpr (kLanesAreRows)
+            store_c(&out[(size_t)j * ld + i], zr, zi);
+        else
+            store_c(&out[(size_t)i * ld + j], zr, zi);
     }
 }
 

@@ -55,6 +55,10 @@ __device__ __forceinline__ void jacobi_pinv(c128* A, c128* V, int r, double atol
     const int team = tid / T, tl = tid - team * T;
     const double tol = fmax(1e-14, 8.8e-16 * r);
     __syncthreads();
+    // s_flag holds the largest relative coupling |a_p^H a_q| / (|a_p| |a_q|)
+    // rotated in the sweep (float bits; 0 = no rotation). Jacobi converges
+    // quadratically, so once every coupling of a sweep is below 1e-7 the next
+    // sweep would only see couplings near the rounding level and is skipped.
     for (int sweep = 0; sweep < kJacobiMaxSweeps && n > 1; ++sweep) {
         if (tid == 0) *s_flag = 0;
         __syncthreads();
@@ -78,7 +82,8 @@ __device__ __forceinline__ void jacobi_pinv(c128* A, c128* V, int r, double atol
                 gr = team_sum(gr, T);
                 gi = team_sum(gi, T);
                 const double g = sqrt(gr * gr + gi * gi);
-                if (!(g > tol * sqrt(al * be))) continue;
+                const double nrm = sqrt(al * be);
+                if (!(g > tol * nrm)) continue;
                 const double zeta = (be - al) / (2.0 * g);
                 const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                 const double c = 1.0 / sqrt(1.0 + t * t);
@@ -96,13 +101,13 @@ __device__ __forceinline__ void jacobi_pinv(c128* A, c128* V, int r, double atol
                     vp[i] = c128{c * x.re - s * ytr, c * x.im - s * yti};
                     vq[i] = c128{s * x.re + c * ytr, s * x.im + c * yti};
                 }
-                if (tl == 0) *s_flag = 1;
+                if (tl == 0) atomicMax(s_flag, __float_as_int(fmaxf((float)(g / nrm), 1e-30f)));
             }
             __syncthreads();
         }
-        const int rotated = *s_flag;
+        const int maxc = *s_flag;
         __syncthreads();
-        if (!rotated) break;
+        if (maxc == 0 || __int_as_float(maxc) < 1e-7f) break;
     }
     for (int j = team; j < r; j += nteams) {
         const c128* aj = A + (size_t)j * r;

@@ -150,12 +150,13 @@ void launch_pivot_rows(const BlockJob* jobs, int njobs, int f32, hipStream_t s) 
 // ---------------------------------------------------------------- fill
 __host__ __device__ static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
+// C and W: lanes over rows; R and H (few rows, many columns): lanes over columns.
 int fill_tiles(int m, int n, int r, int h, int panels) {
     int t = 0;
     if (panels & kPanelC) t += cdiv(m, kTileRows) * cdiv(r, kTileCols);
     if (panels & kPanelW) t += cdiv(r, kTileRows) * cdiv(r, kTileCols);
-    if (panels & kPanelR) t += cdiv(r, kTileRows) * cdiv(n, kTileCols);
-    if ((panels & kPanelH) && h > 0) t += cdiv(h, kTileRows) * cdiv(n, kTileCols);
+    if (panels & kPanelR) t += cdiv(n, kTileRows) * cdiv(r, kTileCols);
+    if ((panels & kPanelH) && h > 0) t += cdiv(n, kTileRows) * cdiv(h, kTileCols);
     return t;
 }
 
@@ -198,34 +199,34 @@ __global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restric
         if (panels & kPanelC) {
             const int mt = cdiv(m, kTileRows), nC = mt * cdiv(r, kTileCols);
             if (local < nC) {
-                fill_tile<T, PC>((PC*)J.C, m, J.rows, m, J.cs, r, (local % mt) * kTileRows,
-                                 (local / mt) * kTileCols, rec, cen, ne, fc, lds);
+                fill_tile<T, PC, decltype(fc), true>((PC*)J.C, m, J.rows, m, J.cs, r, (local % mt) * kTileRows,
+                                                     (local / mt) * kTileCols, rec, cen, ne, fc, lds);
                 continue;
             }
             local -= nC;
         }
-        const int rt = cdiv(r, kTileRows);
+        const int nt = cdiv(n, kTileRows);
         if (panels & kPanelW) {
+            const int rt = cdiv(r, kTileRows);
             const int nW = rt * cdiv(r, kTileCols);
             if (local < nW) {
-                fill_tile<T, c128>(J.W, r, J.rs, r, J.cs, r, (local % rt) * kTileRows, (local / rt) * kTileCols,
-                                   rec, cen, ne, fc, lds);
+                fill_tile<T, c128, decltype(fc), true>(J.W, r, J.rs, r, J.cs, r, (local % rt) * kTileRows,
+                                                       (local / rt) * kTileCols, rec, cen, ne, fc, lds);
                 continue;
             }
             local -= nW;
         }
         if (panels & kPanelR) {
-            const int nR = rt * cdiv(n, kTileCols);
+            const int nR = nt * cdiv(r, kTileCols);
             if (local < nR) {
-                fill_tile<T, PC>((PC*)J.R, r, J.rs, r, J.cols, n, (local % rt) * kTileRows, (local / rt) * kTileCols,
-                                 rec, cen, ne, fc, lds);
+                fill_tile<T, PC, decltype(fc), false>((PC*)J.R, r, J.rs, r, J.cols, n, (local / nt) * kTileCols,
+                                                      (local % nt) * kTileRows, rec, cen, ne, fc, lds);
                 continue;
             }
             local -= nR;
         }
-        const int ht = cdiv(J.h, kTileRows);
-        fill_tile<T, PC>((PC*)J.H, J.h, J.hs, J.h, J.cols, n, (local % ht) * kTileRows, (local / ht) * kTileCols, rec,
-                         cen, ne, fc, lds);
+        fill_tile<T, PC, decltype(fc), false>((PC*)J.H, J.h, J.hs, J.h, J.cols, n, (local / nt) * kTileCols,
+                                              (local % nt) * kTileRows, rec, cen, ne, fc, lds);
     }
 }
 
@@ -250,8 +251,8 @@ __global__ __launch_bounds__(kFillThreads) void k_fill_panel(PC* __restrict__ Z,
                                                              double eta) {
     __shared__ FillTileLds<T> lds;
     const auto fc = ConstsOf<T>::make(k, eta);
-    fill_tile<T, PC>(Z, ld, rows, nr, cols, nc, blockIdx.x * kTileRows, blockIdx.y * kTileCols, rec, cen, ne, fc,
-                     lds);
+    fill_tile<T, PC, decltype(fc), true>(Z, ld, rows, nr, cols, nc, blockIdx.x * kTileRows, blockIdx.y * kTileCols,
+                                         rec, cen, ne, fc, lds);
 }
 
 void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32_t* cols, int nc,
