@@ -501,6 +501,7 @@ int validate_opts(const farcur_opts* o) {
     if (o->oversample < 0) return fail(FARCUR_EINVAL, "oversample must be >= 0");
     if (!(o->label_eps >= 0)) return fail(FARCUR_EINVAL, "label_eps must be >= 0");
     if (!(o->heldout_safety > 0) || o->heldout_safety > 1) return fail(FARCUR_EINVAL, "heldout_safety must be in (0, 1]");
+    if (!(o->growth >= 1.1) || o->growth > 16) return fail(FARCUR_EINVAL, "growth must be in [1.1, 16]");
     return FARCUR_OK;
 }
 
@@ -538,6 +539,7 @@ void farcur_opts_default(farcur_opts* o) {
     o->oversample = 10;
     o->label_eps = 1e-4;
     o->heldout_safety = 0.5;
+    o->growth = 1.5;
 }
 
 // ------------------------------------------------------------------ meshes
@@ -1436,7 +1438,8 @@ BlockJob make_job(farcur_batch* B, int b, const RoundBufs& rb, const farcur_opts
 
 // Learned mode: the predicted round is accepted when its held-out rows meet
 // tol; otherwise the sample count doubles (capped) and the block runs again.
-void advance_learned(BState& s, const JobOut& out, const RoundBufs& rb, double tol, int heldout) {
+void advance_learned(BState& s, const JobOut& out, const RoundBufs& rb, double tol, int heldout, double growth,
+                     int oversample) {
     s.fin = rb;
     s.fin_r = s.r;
     s.kept = out.kept;
@@ -1457,7 +1460,8 @@ void advance_learned(BState& s, const JobOut& out, const RoundBufs& rb, double t
         s.status = 2;
         return;
     }
-    s.r = std::min(2 * s.r, s.cap);
+    // grow by the factor, at least by the oversampling
+    s.r = std::min(std::max((int)std::ceil(growth * s.r), s.r + std::max(oversample, 1)), s.cap);
     s.h = std::min(heldout, s.m - s.r);
 }
 
@@ -1686,7 +1690,7 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
             std::vector<int> next;
             for (size_t i = 0; i < active.size(); ++i) {
                 BState& s = B->st[active[i]];
-                advance_learned(s, outs[i], bufs[i], o.tol * o.heldout_safety, o.heldout);
+                advance_learned(s, outs[i], bufs[i], o.tol * o.heldout_safety, o.heldout, o.growth, o.oversample);
                 if (s.status == 0) next.push_back(active[i]);
             }
             if (first) B->first_round_misses = (int)next.size();
@@ -1744,7 +1748,7 @@ extern "C" int farcur_batch_info(farcur_batch* B, farcur_block_info* infos) {
         I.sigma_max = s.smax;
         I.time_ms = 0;
         I.rank_eps = s.rank_eps;
-        I.pad = 0;
+        I.status = s.status;
     }
     return FARCUR_OK;
 }

@@ -61,6 +61,7 @@ class Opts(ctypes.Structure):
         ("oversample", ci),
         ("label_eps", cd),
         ("heldout_safety", cd),
+        ("growth", cd),
     ]
 
 
@@ -75,11 +76,11 @@ class BlockInfo(ctypes.Structure):
         ("sigma_max", cd),
         ("time_ms", cd),
         ("rank_eps", ci),
-        ("pad", ci),
+        ("status", ci),
     ]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class Tuning(ctypes.Structure):

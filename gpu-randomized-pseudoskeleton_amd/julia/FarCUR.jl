@@ -24,6 +24,7 @@ struct Opts
     oversample::Cint
     label_eps::Cdouble
     heldout_safety::Cdouble
+    growth::Cdouble
 end
 
 struct BlockInfo
@@ -36,7 +37,7 @@ struct BlockInfo
     sigma_max::Cdouble
     time_ms::Cdouble
     rank_eps::Cint
-    pad::Cint
+    status::Cint
 end
 
 function check(rc)

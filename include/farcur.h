@@ -183,6 +183,7 @@ typedef struct {
     int oversample;  /* p added to the predicted rank in FARCUR_MODE_LEARNED (default 10) */
     double label_eps;/* singular values >= label_eps * sigma_max are counted as rank_eps */
     double heldout_safety; /* learned mode accepts a round when held-out error <= heldout_safety * tol (0.5) */
+    double growth;   /* learned mode: sample count factor after a failed check (1.5; the plain loop doubles) */
 } farcur_opts;
 
 void farcur_opts_default(farcur_opts* opts);
@@ -197,7 +198,7 @@ typedef struct {
     double sigma_max;   /* largest singular value of the final W */
     double time_ms;     /* device time of all rounds */
     int rank_eps;       /* singular values of the final W >= label_eps * sigma_max */
-    int pad;
+    int status;         /* 1 converged, 2 stopped at the sample cap, 3 NaN residual */
 } farcur_block_info;
 
 /* The adaptive loop of cuda_cur_compress3 (SW2:878-973) on one block, rows x
