@@ -1438,8 +1438,8 @@ BlockJob make_job(farcur_batch* B, int b, const RoundBufs& rb, const farcur_opts
 
 // Learned mode: the predicted round is accepted when its held-out rows meet
 // tol; otherwise the sample count doubles (capped) and the block runs again.
-void advance_learned(BState& s, const JobOut& out, const RoundBufs& rb, double tol, int heldout, double growth,
-                     int oversample) {
+void advance_learned(BState& s, const JobOut& out, const RoundBufs& rb, double accept, double tol, int heldout,
+                     double growth, int oversample) {
     s.fin = rb;
     s.fin_r = s.r;
     s.kept = out.kept;
@@ -1448,7 +1448,7 @@ void advance_learned(BState& s, const JobOut& out, const RoundBufs& rb, double t
     s.err = s.h > 0 ? out.err : 0.0;
     s.heldout_err = s.h > 0 ? out.err : -1.0;
     s.round += 1;
-    if (s.h == 0 || s.err <= tol) {
+    if (s.h == 0 || s.err <= accept) {
         s.status = 1;
         return;
     }
@@ -1457,7 +1457,9 @@ void advance_learned(BState& s, const JobOut& out, const RoundBufs& rb, double t
         return;
     }
     if (s.r >= s.cap) {
-        s.status = 2;
+        // No larger sample exists: the tolerance itself (without the safety
+        // margin of the acceptance test) decides.
+        s.status = s.err <= tol ? 1 : 2;
         return;
     }
     // grow by the factor, at least by the oversampling
@@ -1690,7 +1692,8 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
             std::vector<int> next;
             for (size_t i = 0; i < active.size(); ++i) {
                 BState& s = B->st[active[i]];
-                advance_learned(s, outs[i], bufs[i], o.tol * o.heldout_safety, o.heldout, o.growth, o.oversample);
+                advance_learned(s, outs[i], bufs[i], o.tol * o.heldout_safety, o.tol, o.heldout, o.growth,
+                                o.oversample);
                 if (s.status == 0) next.push_back(active[i]);
             }
             if (first) B->first_round_misses = (int)next.size();

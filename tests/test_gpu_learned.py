@@ -96,7 +96,7 @@ def test_learned_mode_heldout_sampling_parity(farcur, gctx, ellipsoid, oracle):
             A = out["C"][hp, :] @ out["U"] @ out["R"]
             e = np.linalg.norm(Zh - A) / np.linalg.norm(Zh)
             assert e == pytest.approx(info["heldout_err"][b], rel=1e-3, abs=1e-9)
-            assert info["converged"][b] == (e <= 0.5e-4) or info["r"][b] == min(m, n)
+            assert info["converged"][b] == (e <= 0.5e-4) or info["r"][b] == min(m, n) or abs(e - 0.5e-4) < 1e-7
         checked += 1
     assert checked >= 5
 
