@@ -1321,7 +1321,7 @@ struct BState {
     int m = 0, n = 0, cap = 0, r = 0, round = 0;
     int status = 0;  // 0 active, 1 converged, 2 stopped at the cap, 3 NaN residual
     int ycur = 0;
-    int kept = 0, rank_eps = 0;
+    int kept = 0, rank_eps = 0, sweeps = 0;
     int h = 0;       // held-out rows of the current round (learned mode)
     double err = INFINITY, smax = 0, heldout_err = -1;
     RoundBufs fin{};
@@ -1444,6 +1444,7 @@ void advance_learned(BState& s, const JobOut& out, const RoundBufs& rb, double a
     s.fin_r = s.r;
     s.kept = out.kept;
     s.rank_eps = out.rank_eps;
+    s.sweeps = out.sweeps;
     s.smax = out.sigma_max;
     s.err = s.h > 0 ? out.err : 0.0;
     s.heldout_err = s.h > 0 ? out.err : -1.0;
@@ -1474,6 +1475,7 @@ void advance(BState& s, const JobOut& out, const RoundBufs& rb, double tol) {
     s.fin_r = s.r;
     s.kept = out.kept;
     s.rank_eps = out.rank_eps;
+    s.sweeps = out.sweeps;
     s.smax = out.sigma_max;
     if (s.round == 0) {
         int rn;
@@ -1752,6 +1754,8 @@ extern "C" int farcur_batch_info(farcur_batch* B, farcur_block_info* infos) {
         I.time_ms = 0;
         I.rank_eps = s.rank_eps;
         I.status = s.status;
+        I.sweeps = s.sweeps;
+        I.pad = 0;
     }
     return FARCUR_OK;
 }
@@ -1796,7 +1800,7 @@ namespace {
 
 // Times the pinv of nprob random r x r problems with the given launch shape
 // (min over reps, ms).
-int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* best_ms) {
+int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* best_ms, double* sweeps = nullptr) {
     hipStream_t s = ctx->stream;
     const size_t rr = (size_t)r * r;
     DBuf W0, W, V, U, sig, plan;
@@ -1844,18 +1848,26 @@ int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* 
         HIPTRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         *best_ms = std::min(*best_ms, (double)ms);
     }
+    if (sweeps) {
+        std::vector<JobOut> o(nprob);
+        HIPTRY(hipMemcpy(o.data(), d + bj + bl + 256, nprob * sizeof(JobOut), hipMemcpyDeviceToHost));
+        double a = 0;
+        for (auto& x : o) a += x.sweeps;
+        *sweeps = a / nprob;
+    }
     return FARCUR_OK;
 }
 
 }  // namespace
 
-extern "C" int farcur_pinv_bench(farcur_ctx* ctx, int r, int nprob, int threads, int use_lds, double* ms) {
+extern "C" int farcur_pinv_bench(farcur_ctx* ctx, int r, int nprob, int threads, int use_lds, double* ms,
+                                 double* sweeps) {
     if (!ctx || !ms) return fail(FARCUR_EINVAL, "NULL argument");
     if (r <= 0 || r > 1024 || nprob <= 0 || nprob > 65536) return fail(FARCUR_EINVAL, "bad sizes");
     if (threads <= 0 || threads > 1024 || threads % 64) return fail(FARCUR_EINVAL, "threads must be a multiple of 64");
     if (use_lds && r > kPinvLdsMaxR) return fail(FARCUR_EINVAL, "LDS variant needs r <= %d", kPinvLdsMaxR);
     TRY(ctx->select());
-    return time_pinv(ctx, r, nprob, threads, use_lds ? r : 0, ms);
+    return time_pinv(ctx, r, nprob, threads, use_lds ? r : 0, ms, sweeps);
 }
 
 extern "C" int farcur_autotune(farcur_ctx* ctx, farcur_tuning* out) {

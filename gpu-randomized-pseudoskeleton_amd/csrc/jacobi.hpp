@@ -41,7 +41,7 @@ __device__ __forceinline__ void circle_pair(int n, int step, int pi, int& p, int
 __device__ __forceinline__ void jacobi_pinv(c128* A, c128* V, int r, double atol, double rtol,
                                             c128* __restrict__ U, int ldu, double* __restrict__ sigma, int* s_flag,
                                             double* s_w, int* kept_out, double* smax_out, double label_eps,
-                                            int* rank_eps_out) {
+                                            int* rank_eps_out, int* sweeps_out) {
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int idx = tid; idx < r * r; idx += nt) {
         const int i = idx % r, j = idx / r;
@@ -59,7 +59,9 @@ __device__ __forceinline__ void jacobi_pinv(c128* A, c128* V, int r, double atol
     // rotated in the sweep (float bits; 0 = no rotation). Jacobi converges
     // quadratically, so once every coupling of a sweep is below 1e-7 the next
     // sweep would only see couplings near the rounding level and is skipped.
+    int sweeps = 0;
     for (int sweep = 0; sweep < kJacobiMaxSweeps && n > 1; ++sweep) {
+        ++sweeps;
         if (tid == 0) *s_flag = 0;
         __syncthreads();
         for (int step = 0; step < n - 1; ++step) {
@@ -152,6 +154,7 @@ __device__ __forceinline__ void jacobi_pinv(c128* A, c128* V, int r, double atol
     *kept_out = kept;
     *smax_out = smax;
     *rank_eps_out = reps;
+    *sweeps_out = sweeps;
     __syncthreads();
 }
 

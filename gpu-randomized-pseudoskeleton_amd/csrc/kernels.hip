@@ -282,13 +282,14 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
         const BlockJob& J = jobs[jb];
         const int r = J.r;
         for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) s_a[idx] = J.W[idx];
-        int kept, reps;
+        int kept, reps, sweeps;
         double smax;
-        jacobi_pinv(s_a, s_v, r, atol, rtol, J.U, r, J.sigma, s_flag, s_w, &kept, &smax, label_eps, &reps);
+        jacobi_pinv(s_a, s_v, r, atol, rtol, J.U, r, J.sigma, s_flag, s_w, &kept, &smax, label_eps, &reps, &sweeps);
         if (threadIdx.x == 0) {
             out[jb].kept = kept;
             out[jb].sigma_max = smax;
             out[jb].rank_eps = reps;
+            out[jb].sweeps = sweeps;
         }
         __syncthreads();
     }
@@ -305,13 +306,14 @@ __global__ void k_pinv_global(const BlockJob* __restrict__ jobs, const int32_t* 
         const BlockJob& J = jobs[jb];
         const int r = J.r;
         double* w = r <= 1024 ? s_w : (double*)(J.V + (size_t)r * r);
-        int kept, reps;
+        int kept, reps, sweeps;
         double smax;
-        jacobi_pinv(J.W, J.V, r, atol, rtol, J.U, r, J.sigma, &s_flag, w, &kept, &smax, label_eps, &reps);
+        jacobi_pinv(J.W, J.V, r, atol, rtol, J.U, r, J.sigma, &s_flag, w, &kept, &smax, label_eps, &reps, &sweeps);
         if (threadIdx.x == 0) {
             out[jb].kept = kept;
             out[jb].sigma_max = smax;
             out[jb].rank_eps = reps;
+            out[jb].sweeps = sweeps;
         }
         __syncthreads();
     }

@@ -45,6 +45,8 @@ struct JobOut {
     double sigma_max; // largest singular value of W
     int32_t kept;     // singular values kept by the threshold
     int32_t rank_eps; // singular values >= label_eps * sigma_max (rank label)
+    int32_t sweeps;   // Jacobi sweeps run
+    int32_t pad;
 };
 
 struct FillParams {

@@ -77,10 +77,12 @@ class BlockInfo(ctypes.Structure):
         ("time_ms", cd),
         ("rank_eps", ci),
         ("status", ci),
+        ("sweeps", ci),
+        ("pad", ci),
     ]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
 
 
 class Tuning(ctypes.Structure):
@@ -131,7 +133,7 @@ def _setup(L: ctypes.CDLL) -> None:
     L.farcur_autotune.argtypes = [vp, vp]
     L.farcur_get_tuning.argtypes = [vp, vp]
     L.farcur_set_tuning.argtypes = [vp, vp]
-    L.farcur_pinv_bench.argtypes = [vp, ci, ci, ci, ci, vp]
+    L.farcur_pinv_bench.argtypes = [vp, ci, ci, ci, ci, vp, vp]
     if hasattr(L, "farcur_batch_create"):
         L.farcur_batch_create.argtypes = [vp, ci, vp, vp, vp, vp, vp]
         L.farcur_batch_destroy.argtypes = [vp]
@@ -390,10 +392,12 @@ class Context:
             t.resid_threads = resid_threads
         check(lib().farcur_set_tuning(self._h, ctypes.byref(t)))
 
-    def pinv_bench(self, r, nprob, threads, use_lds) -> float:
-        ms = cd()
-        check(lib().farcur_pinv_bench(self._h, r, nprob, threads, 1 if use_lds else 0, ctypes.byref(ms)))
-        return ms.value
+    def pinv_bench(self, r, nprob, threads, use_lds):
+        """(ms, mean sweeps) for nprob random r x r problems."""
+        ms, sw = cd(), cd()
+        check(lib().farcur_pinv_bench(self._h, r, nprob, threads, 1 if use_lds else 0, ctypes.byref(ms),
+                                      ctypes.byref(sw)))
+        return ms.value, sw.value
 
     def compress_block(self, rows, cols, **opts):
         rows, cols = _ivec(rows), _ivec(cols)

@@ -38,6 +38,8 @@ struct BlockInfo
     time_ms::Cdouble
     rank_eps::Cint
     status::Cint
+    sweeps::Cint
+    pad::Cint
 end
 
 function check(rc)

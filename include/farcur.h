@@ -199,6 +199,8 @@ typedef struct {
     double time_ms;     /* device time of all rounds */
     int rank_eps;       /* singular values of the final W >= label_eps * sigma_max */
     int status;         /* 1 converged, 2 stopped at the sample cap, 3 NaN residual */
+    int sweeps;         /* Jacobi sweeps of the final core pinv */
+    int pad;
 } farcur_block_info;
 
 /* The adaptive loop of cuda_cur_compress3 (SW2:878-973) on one block, rows x
@@ -292,8 +294,9 @@ int farcur_autotune(farcur_ctx* ctx, farcur_tuning* out);
 int farcur_get_tuning(farcur_ctx* ctx, farcur_tuning* out);
 int farcur_set_tuning(farcur_ctx* ctx, const farcur_tuning* in);
 /* Device time (ms, best of 3) of the pinv of nprob random r x r problems with
- * `threads` per workgroup, LDS (use_lds = 1, r <= 64) or global variant. */
-int farcur_pinv_bench(farcur_ctx* ctx, int r, int nprob, int threads, int use_lds, double* ms);
+ * `threads` per workgroup, LDS (use_lds = 1, r <= 64) or global variant;
+ * *sweeps (may be NULL) receives the mean Jacobi sweep count. */
+int farcur_pinv_bench(farcur_ctx* ctx, int r, int nprob, int threads, int use_lds, double* ms, double* sweeps);
 
 #ifdef __cplusplus
 }
