@@ -12,6 +12,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <unordered_set>
 #include <vector>
 
@@ -20,6 +21,7 @@
 #include "geometry.hpp"
 #include "kernels.hpp"
 #include "mlp.hpp"
+#include "mrhs.hpp"
 #include "philox.hpp"
 
 using namespace farcur;
@@ -2169,3 +2171,160 @@ extern "C" int farcur_batch_set_model(farcur_batch* B, farcur_model* M) {
 }
 
 extern "C" int farcur_batch_misses(farcur_batch* B) { return B ? B->first_round_misses : -1; }
+
+// ------------------------------------------------------------------ multi-RHS product
+extern "C" int farcur_batch_matvec(farcur_batch* B, const farcur_c128* V, int s, farcur_c128* Y, int mode,
+                                   int symmetric, double* ms) {
+    if (!B || !V || !Y) return fail(FARCUR_EINVAL, "NULL argument");
+    if (!B->has_result) return fail(FARCUR_ESTATE, "no batch result (compress first)");
+    if (s <= 0) return fail(FARCUR_EINVAL, "nrhs must be > 0");
+    if (mode != kMrhsValu && mode != kMrhsMfmaF32 && mode != kMrhsMfmaBf16x3) return fail(FARCUR_EINVAL, "bad mode");
+    farcur_ctx* ctx = B->ctx;
+    TRY(ctx->select());
+    hipStream_t st = ctx->stream;
+    const int N = ctx->ne;
+    // Contributions and target groups (identical id lists form one group).
+    std::vector<MrhsContrib> cs;
+    std::vector<std::vector<int>> group_contribs;
+    std::vector<const int32_t*> group_ids_dev;
+    std::vector<int> group_m;
+    std::unordered_map<uint64_t, std::vector<int>> by_hash;
+    std::vector<int> owner(N, -1);
+    auto group_of = [&](const int32_t* host_ids, const int32_t* dev_ids, int cnt, int* out) -> int {
+        uint64_t h = fnv1a(0xcbf29ce484222325ull, host_ids, (size_t)cnt * 4) ^ (uint64_t)cnt;
+        for (int g : by_hash[h]) {
+            if (group_m[g] != cnt) continue;
+            bool same = true;
+            for (int i = 0; i < cnt && same; ++i) same = owner[host_ids[i]] == g;
+            if (same) {
+                *out = g;
+                return FARCUR_OK;
+            }
+        }
+        const int g = (int)group_m.size();
+        for (int i = 0; i < cnt; ++i) {
+            if (owner[host_ids[i]] != -1)
+                return fail(FARCUR_EINVAL, "row/column sets of the blocks overlap without being identical (RWG %d)",
+                            host_ids[i]);
+            owner[host_ids[i]] = g;
+        }
+        by_hash[h].push_back(g);
+        group_m.push_back(cnt);
+        group_ids_dev.push_back(dev_ids);
+        group_contribs.emplace_back();
+        *out = g;
+        return FARCUR_OK;
+    };
+    size_t t2_total = 0;
+    int rmax = 0;
+    for (int b = 0; b < B->nb; ++b) {
+        const BState& S = B->st[b];
+        const int r = S.fin_r;
+        if (r > mrhs_max_r()) return fail(FARCUR_EINVAL, "block %d: r = %d exceeds %d", b, r, mrhs_max_r());
+        rmax = std::max(rmax, r);
+        const int32_t* hrows = &B->row_ids[B->row_off[b]];
+        const int32_t* hcols = &B->col_ids[B->col_off[b]];
+        const int32_t* drows = B->d_rows.as<int32_t>() + B->row_off[b];
+        const int32_t* dcols = B->d_cols.as<int32_t>() + B->col_off[b];
+        for (int o = 0; o < (symmetric ? 2 : 1); ++o) {
+            MrhsContrib c{};
+            c.r = r;
+            c.u = S.fin.U;
+            c.t2_off = t2_total;
+            if (o == 0) {
+                c.a1 = S.fin.R;
+                c.a1_ld = r;
+                c.a1_trans = 0;
+                c.k1 = S.n;
+                c.x_ids = dcols;
+                c.u_trans = 0;
+                c.l = S.fin.C;
+                c.l_ld = S.m;
+                c.l_trans = 0;
+            } else {
+                c.a1 = S.fin.C;
+                c.a1_ld = S.m;
+                c.a1_trans = 1;
+                c.k1 = S.m;
+                c.x_ids = drows;
+                c.u_trans = 1;
+                c.l = S.fin.R;
+                c.l_ld = r;
+                c.l_trans = 1;
+            }
+            int g;
+            if (o == 0)
+                TRY(group_of(hrows, drows, S.m, &g));
+            else
+                TRY(group_of(hcols, dcols, S.n, &g));
+            group_contribs[g].push_back((int)cs.size());
+            cs.push_back(c);
+            t2_total += (size_t)r * s;
+        }
+    }
+    const int ng = (int)group_m.size();
+    std::vector<MrhsGroup> groups(ng);
+    std::vector<int32_t> cog, tile_group, tile_row0;
+    for (int g = 0; g < ng; ++g) {
+        groups[g].ids = group_ids_dev[g];
+        groups[g].m = group_m[g];
+        groups[g].c_begin = (int)cog.size();
+        cog.insert(cog.end(), group_contribs[g].begin(), group_contribs[g].end());
+        groups[g].c_end = (int)cog.size();
+        for (int r0 = 0; r0 < group_m[g]; r0 += 64) {
+            tile_group.push_back(g);
+            tile_row0.push_back(r0);
+        }
+    }
+    // Device buffers: V and Y row-major complex64, T2, descriptors.
+    DBuf dV, dY, dT2, dcs, dg, dcog, dtg, dtr;
+    TRY(dV.ensure((size_t)N * s * 8));
+    TRY(dY.ensure((size_t)N * s * 8));
+    TRY(dT2.ensure(std::max<size_t>(t2_total, 1) * 8));
+    TRY(dcs.ensure(cs.size() * sizeof(MrhsContrib)));
+    TRY(dg.ensure(groups.size() * sizeof(MrhsGroup)));
+    TRY(dcog.ensure(cog.size() * 4));
+    TRY(dtg.ensure(tile_group.size() * 4));
+    TRY(dtr.ensure(tile_row0.size() * 4));
+    std::vector<float2> hv((size_t)N * s);
+    for (int j = 0; j < s; ++j)
+        for (int i = 0; i < N; ++i)
+            hv[(size_t)i * s + j] = make_float2((float)V[(size_t)j * N + i].re, (float)V[(size_t)j * N + i].im);
+    HIPTRY(hipMemcpyAsync(dV.p, hv.data(), hv.size() * 8, hipMemcpyHostToDevice, st));
+    HIPTRY(hipMemsetAsync(dY.p, 0, (size_t)N * s * 8, st));
+    HIPTRY(hipMemcpyAsync(dcs.p, cs.data(), cs.size() * sizeof(MrhsContrib), hipMemcpyHostToDevice, st));
+    HIPTRY(hipMemcpyAsync(dg.p, groups.data(), groups.size() * sizeof(MrhsGroup), hipMemcpyHostToDevice, st));
+    HIPTRY(hipMemcpyAsync(dcog.p, cog.data(), cog.size() * 4, hipMemcpyHostToDevice, st));
+    HIPTRY(hipMemcpyAsync(dtg.p, tile_group.data(), tile_group.size() * 4, hipMemcpyHostToDevice, st));
+    HIPTRY(hipMemcpyAsync(dtr.p, tile_row0.data(), tile_row0.size() * 4, hipMemcpyHostToDevice, st));
+    hipEvent_t e0, e1, e2;
+    HIPTRY(hipEventCreate(&e0));
+    HIPTRY(hipEventCreate(&e1));
+    HIPTRY(hipEventCreate(&e2));
+    HIPTRY(hipEventRecord(e0, st));
+    launch_mrhs_pass_a(dcs.as<MrhsContrib>(), (int)cs.size(), rmax, B->f32, dV.as<float2>(), s, dT2.as<float2>(), st);
+    HIPTRY(hipEventRecord(e1, st));
+    launch_mrhs_pass_b(dg.as<MrhsGroup>(), (int)tile_group.size(), dtg.as<int32_t>(), dtr.as<int32_t>(),
+                       dcs.as<MrhsContrib>(), dcog.as<int32_t>(), B->f32, mode, dT2.as<float2>(), s,
+                       dY.as<float2>(), st);
+    HIPTRY(hipEventRecord(e2, st));
+    HIPTRY(hipGetLastError());
+    std::vector<float2> hy((size_t)N * s);
+    HIPTRY(hipMemcpyAsync(hy.data(), dY.p, hy.size() * 8, hipMemcpyDeviceToHost, st));
+    HIPTRY(hipStreamSynchronize(st));
+    float ta = 0, tb = 0;
+    hipEventElapsedTime(&ta, e0, e1);
+    hipEventElapsedTime(&tb, e1, e2);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipEventDestroy(e2);
+    if (ms) {
+        ms[0] = ta;
+        ms[1] = tb;
+    }
+    for (int j = 0; j < s; ++j)
+        for (int i = 0; i < N; ++i) Y[(size_t)j * N + i] = farcur_c128{hy[(size_t)i * s + j].x, hy[(size_t)i * s + j].y};
+    DBuf* bufs[] = {&dV, &dY, &dT2, &dcs, &dg, &dcog, &dtg, &dtr};
+    for (DBuf* b : bufs) b->release();
+    return FARCUR_OK;
+}

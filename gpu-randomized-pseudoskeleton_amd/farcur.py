@@ -25,6 +25,7 @@ LIB_PATH = HERE / "libfarcur.so"
 F64, F32 = 0, 1
 ROWS_UNIFORM, ROWS_PIVOT = 0, 1
 MODE_PLAIN, MODE_BATCHED, MODE_LEARNED = 0, 1, 2
+MV_VALU, MV_MFMA_F32, MV_MFMA_BF16X3 = 0, 1, 2
 
 _lib = None
 
@@ -146,6 +147,7 @@ def _setup(L: ctypes.CDLL) -> None:
         L.farcur_batch_features.argtypes = [vp, vp]
         L.farcur_batch_set_model.argtypes = [vp, vp]
         L.farcur_batch_misses.argtypes = [vp]
+        L.farcur_batch_matvec.argtypes = [vp, vp, ci, vp, ci, ci, vp]
     L.farcur_model_create.argtypes = [vp, cu64, vp]
     L.farcur_model_destroy.argtypes = [vp]
     L.farcur_model_destroy.restype = None
@@ -477,6 +479,16 @@ class Batch:
     def set_model(self, model: "Model | None"):
         self._model = model
         check(lib().farcur_batch_set_model(self._h, model._h if model is not None else None))
+
+    def matvec(self, V, mode=1, symmetric=True):
+        """Y (N x s) = compressed far field times V (N x s); returns (Y, [ms pass A, ms pass B])."""
+        V = np.asfortranarray(V, dtype=np.complex128)
+        if V.ndim == 1:
+            V = V.reshape(-1, 1, order="F")
+        Y = np.empty_like(V, order="F")
+        ms = np.zeros(2)
+        check(lib().farcur_batch_matvec(self._h, _p(V), V.shape[1], _p(Y), mode, 1 if symmetric else 0, _p(ms)))
+        return Y, ms
 
     def misses(self) -> int:
         return lib().farcur_batch_misses(self._h)

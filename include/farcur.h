@@ -270,6 +270,20 @@ int farcur_batch_set_model(farcur_batch* batch, farcur_model* model);
  * compress (the miss count; not an error status). */
 int farcur_batch_misses(farcur_batch* batch);
 
+/* Multi-RHS product of the compressed far field: Y = sum_b C_b U_b R_b V[cols_b]
+ * added to rows_b, and with symmetric = 1 also (C_b U_b R_b)^T V[rows_b] added
+ * to cols_b (the transposed block of a symmetric operator, which the
+ * reference builds with transpose(Cs*Us*Rs) for self == 0,
+ * codes/GPU_implementation/seed/compres_single_block_spheres.jl:2608-2612).
+ * V and Y are N x nrhs column-major (N = RWGs of the geometry); V is cast to
+ * complex64 and Y accumulated in FP32. Rows of Y outside every target set are
+ * zero. The blocks' row (and column) sets must be disjoint groups or identical.
+ * mode: FARCUR_MV_VALU, FARCUR_MV_MFMA_F32, FARCUR_MV_MFMA_BF16X3.
+ * ms (may be NULL) receives the device ms of pass A and pass B. */
+enum { FARCUR_MV_VALU = 0, FARCUR_MV_MFMA_F32 = 1, FARCUR_MV_MFMA_BF16X3 = 2 };
+int farcur_batch_matvec(farcur_batch* batch, const farcur_c128* V, int nrhs, farcur_c128* Y, int mode, int symmetric,
+                        double* ms);
+
 /* Device time (ms) and launch count of each kernel class over the last
  * batch_compress call with profiling on: names "sample", "fill", "pinv",
  * "residual". Returns the number of classes written (<= cap). */
