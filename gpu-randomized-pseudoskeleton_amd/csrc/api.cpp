@@ -2328,3 +2328,41 @@ extern "C" int farcur_batch_matvec(farcur_batch* B, const farcur_c128* V, int s,
     for (DBuf* b : bufs) b->release();
     return FARCUR_OK;
 }
+
+// ------------------------------------------------------------------ excitations
+// RWG-tested plane waves with the centroid rule of user_compute_Ei.m
+// (codes/GPU_implementation/seed/user_compute_Ei.m:40-57): propagation
+// k_i = -(sin th cos ph, sin th sin ph, cos th), polarisation
+// e_i = cos(rot) (-sin ph, cos ph, 0) + sin(rot) (cos th cos ph, cos th sin ph, -sin th),
+// V_m = l_m / 2 [E(c+) . (c+ - v+) + E(c-) . (v- - c-)], E(r) = e_i exp(-j k k_i . r).
+extern "C" int farcur_mesh_plane_waves(const farcur_mesh* mesh, double k, const double* theta, const double* phi,
+                                       int nrhs, double rot_eh, farcur_c128* V) {
+    if (!mesh || !theta || !phi || !V) return fail(FARCUR_EINVAL, "NULL argument");
+    if (!mesh->rwg) return fail(FARCUR_ESTATE, "RWG tables not built");
+    if (nrhs <= 0 || !(k > 0)) return fail(FARCUR_EINVAL, "bad nrhs or k");
+    const Mesh& M = mesh->m;
+    const int ne = M.ne();
+    for (int s = 0; s < nrhs; ++s) {
+        const double th = theta[s], ph = phi[s];
+        const double ki[3] = {-std::sin(th) * std::cos(ph), -std::sin(th) * std::sin(ph), -std::cos(th)};
+        const double ei[3] = {std::cos(rot_eh) * -std::sin(ph) + std::sin(rot_eh) * std::cos(th) * std::cos(ph),
+                              std::cos(rot_eh) * std::cos(ph) + std::sin(rot_eh) * std::cos(th) * std::sin(ph),
+                              std::sin(rot_eh) * -std::sin(th)};
+        for (int e = 0; e < ne; ++e) {
+            double re = 0, im = 0;
+            for (int h = 0; h < 2; ++h) {
+                const int t = M.edges[4 * e + h];
+                const double* c = &M.cent[3 * t];
+                const double* v = &M.vertex[3 * M.edges[4 * e + 2 + h]];
+                double rho[3];
+                for (int x = 0; x < 3; ++x) rho[x] = h == 0 ? c[x] - v[x] : v[x] - c[x];
+                const double ph0 = -k * (ki[0] * c[0] + ki[1] * c[1] + ki[2] * c[2]);
+                const double dp = ei[0] * rho[0] + ei[1] * rho[1] + ei[2] * rho[2];
+                re += dp * std::cos(ph0);
+                im += dp * std::sin(ph0);
+            }
+            V[(size_t)s * ne + e] = farcur_c128{0.5 * M.ln[e] * re, 0.5 * M.ln[e] * im};
+        }
+    }
+    return FARCUR_OK;
+}

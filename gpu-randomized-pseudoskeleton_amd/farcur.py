@@ -107,6 +107,7 @@ def _setup(L: ctypes.CDLL) -> None:
     L.farcur_mesh_tables.argtypes = [vp] * 9
     L.farcur_mesh_destroy.argtypes = [vp]
     L.farcur_mesh_destroy.restype = None
+    L.farcur_mesh_plane_waves.argtypes = [vp, cd, vp, vp, ci, cd, vp]
     L.farcur_partition_build.argtypes = [vp, cd, cd, vp]
     L.farcur_partition_sizes.argtypes = [vp, vp, vp, vp, vp]
     L.farcur_partition_tables.argtypes = [vp] * 7
@@ -233,6 +234,15 @@ class Mesh:
         for k in ("vertex", "topol", "trian", "edges", "un", "cent"):
             out[k] = np.asfortranarray(out[k].T)
         return out
+
+    def plane_waves(self, theta, phi, k=2 * math.pi, rot_eh=math.pi / 2) -> np.ndarray:
+        """RWG-tested plane waves, ne x len(theta) (radians; rot_eh = pi/2: theta-polarised)."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        ph = np.ascontiguousarray(phi, dtype=np.float64)
+        ne = self.sizes()[2]
+        V = np.empty((ne, th.size), dtype=np.complex128, order="F")
+        check(lib().farcur_mesh_plane_waves(self._h, k, _p(th), _p(ph), th.size, rot_eh, _p(V)))
+        return V
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

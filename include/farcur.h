@@ -99,6 +99,12 @@ int farcur_mesh_sizes(const farcur_mesh* mesh, int* nv, int* nt, int* ne);
 int farcur_mesh_tables(const farcur_mesh* mesh, double* vertex, double* topol, double* trian, double* edges,
                        double* un, double* ds, double* ln, double* cent);
 void farcur_mesh_destroy(farcur_mesh* mesh);
+/* RWG-tested plane waves (centroid rule of user_compute_Ei.m,
+ * codes/GPU_implementation/seed/user_compute_Ei.m:40-57): V (ne x nrhs,
+ * column-major) for incidence angles theta[s], phi[s] (radians) and the
+ * polarisation angle rot_eh (radians; pi/2 = theta-hat). Host computation. */
+int farcur_mesh_plane_waves(const farcur_mesh* mesh, double k, const double* theta, const double* phi, int nrhs,
+                            double rot_eh, farcur_c128* V);
 
 /* Octree leaf groups of side leaf_size and the admissible (far) pairs
  * |c_a - c_b| >= eta_adm * max(diam_a, diam_b); the Matlab Z_comp partition

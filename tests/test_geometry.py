@@ -112,3 +112,16 @@ def test_bad_arguments(farcur):
     m = farcur.Mesh.plate(1.0, 1.0, 2, 2)
     with pytest.raises(farcur.FarcurError):
         m.tables()  # RWG tables not built
+
+
+def test_plane_waves_match_user_compute_Ei(farcur):
+    import excitation
+
+    m = farcur.Mesh.icosphere(2.0, 2.0, 2.0, 2, 0.01, 1).build_rwg()
+    t = m.tables()
+    th = np.linspace(0, np.pi, 5)
+    ph = np.array([0.0, 0.3, 1.0, 2.0, 5.0])
+    V = m.plane_waves(th, ph, k=2 * np.pi)
+    for j in range(5):
+        ref = excitation.plane_wave(t["vertex"], t["edges"], t["ln"], t["cent"], 2 * np.pi, th[j], ph[j])
+        np.testing.assert_allclose(V[:, j], ref, rtol=1e-12, atol=1e-14)
