@@ -1363,6 +1363,9 @@ struct farcur_batch {
     farcur_model* model = nullptr;
     bool learned = false;
     int first_round_misses = 0;
+    // Interpolation matrices P_b = C_b U_b for the product (arena memory).
+    std::vector<void*> interp;
+    bool interp_ready = false;
 };
 
 namespace {
@@ -1644,6 +1647,7 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
     B->learned = mode == FARCUR_MODE_LEARNED;
     B->first_round_misses = 0;
     B->has_result = false;
+    B->interp_ready = false;
     B->rounds.clear();
     B->kt.clear();
     B->kt.on = B->profile;
@@ -2173,6 +2177,55 @@ extern "C" int farcur_batch_set_model(farcur_batch* B, farcur_model* M) {
 extern "C" int farcur_batch_misses(farcur_batch* B) { return B ? B->first_round_misses : -1; }
 
 // ------------------------------------------------------------------ multi-RHS product
+namespace {
+
+// Interpolation matrices P_b = C_b U_b of the last compress (device, panel
+// precision), formed on the first product call.
+int batch_interp(farcur_batch* B, double* ms) {
+    if (B->interp_ready) {
+        *ms = 0;
+        return FARCUR_OK;
+    }
+    hipStream_t st = B->ctx->stream;
+    std::vector<MrhsInterp> jobs(B->nb);
+    B->interp.assign(B->nb, nullptr);
+    for (int b = 0; b < B->nb; ++b) {
+        const BState& S = B->st[b];
+        void* p;
+        TRY(B->arena.alloc((size_t)S.m * S.fin_r * psize(B->f32), &p));
+        B->interp[b] = p;
+        jobs[b] = MrhsInterp{S.fin.C, S.fin.U, p, S.m, S.fin_r};
+    }
+    DBuf dj;
+    TRY(dj.ensure(jobs.size() * sizeof(MrhsInterp)));
+    HIPTRY(hipMemcpyAsync(dj.p, jobs.data(), jobs.size() * sizeof(MrhsInterp), hipMemcpyHostToDevice, st));
+    hipEvent_t e0, e1;
+    HIPTRY(hipEventCreate(&e0));
+    HIPTRY(hipEventCreate(&e1));
+    HIPTRY(hipEventRecord(e0, st));
+    launch_mrhs_interp(dj.as<MrhsInterp>(), B->nb, B->f32, st);
+    HIPTRY(hipEventRecord(e1, st));
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipStreamSynchronize(st));
+    float t = 0;
+    hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms = t;
+    B->interp_ready = true;
+    return FARCUR_OK;
+}
+
+// 64-row tiles of a list of problems.
+void add_tiles(int prob, int m, std::vector<int32_t>& tp, std::vector<int32_t>& tr) {
+    for (int r0 = 0; r0 < m; r0 += 64) {
+        tp.push_back(prob);
+        tr.push_back(r0);
+    }
+}
+
+}  // namespace
+
 extern "C" int farcur_batch_matvec(farcur_batch* B, const farcur_c128* V, int s, farcur_c128* Y, int mode,
                                    int symmetric, double* ms) {
     if (!B || !V || !Y) return fail(FARCUR_EINVAL, "NULL argument");
@@ -2183,20 +2236,21 @@ extern "C" int farcur_batch_matvec(farcur_batch* B, const farcur_c128* V, int s,
     TRY(ctx->select());
     hipStream_t st = ctx->stream;
     const int N = ctx->ne;
-    // Contributions and target groups (identical id lists form one group).
-    std::vector<MrhsContrib> cs;
-    std::vector<std::vector<int>> group_contribs;
-    std::vector<const int32_t*> group_ids_dev;
+    double ms_interp = 0;
+    TRY(batch_interp(B, &ms_interp));
+    // Target groups: identical id lists form one group; distinct groups must
+    // not share an RWG.
+    std::vector<std::vector<int>> group_contribs;  // contribution indices per group
+    std::vector<const int32_t*> group_ids_dev, group_ids_host;
     std::vector<int> group_m;
     std::unordered_map<uint64_t, std::vector<int>> by_hash;
     std::vector<int> owner(N, -1);
+    // A group is an id list in one order: its contributions' rows follow it.
     auto group_of = [&](const int32_t* host_ids, const int32_t* dev_ids, int cnt, int* out) -> int {
         uint64_t h = fnv1a(0xcbf29ce484222325ull, host_ids, (size_t)cnt * 4) ^ (uint64_t)cnt;
         for (int g : by_hash[h]) {
             if (group_m[g] != cnt) continue;
-            bool same = true;
-            for (int i = 0; i < cnt && same; ++i) same = owner[host_ids[i]] == g;
-            if (same) {
+            if (std::memcmp(group_ids_host[g], host_ids, (size_t)cnt * 4) == 0) {
                 *out = g;
                 return FARCUR_OK;
             }
@@ -2204,109 +2258,100 @@ extern "C" int farcur_batch_matvec(farcur_batch* B, const farcur_c128* V, int s,
         const int g = (int)group_m.size();
         for (int i = 0; i < cnt; ++i) {
             if (owner[host_ids[i]] != -1)
-                return fail(FARCUR_EINVAL, "row/column sets of the blocks overlap without being identical (RWG %d)",
+                return fail(FARCUR_EINVAL,
+                            "row/column sets of the blocks overlap without being identical lists (RWG %d)",
                             host_ids[i]);
             owner[host_ids[i]] = g;
         }
         by_hash[h].push_back(g);
         group_m.push_back(cnt);
         group_ids_dev.push_back(dev_ids);
+        group_ids_host.push_back(host_ids);
         group_contribs.emplace_back();
         *out = g;
         return FARCUR_OK;
     };
+    // Pass A: one problem per contribution (block, orientation), T = A1 X.
+    // Pass B segments: one per contribution, L T into the contribution's group.
+    std::vector<MrhsProb> pa;
+    std::vector<MrhsSeg> sa, sb;
+    std::vector<int32_t> ta_p, ta_r;
     size_t t2_total = 0;
-    int rmax = 0;
     for (int b = 0; b < B->nb; ++b) {
         const BState& S = B->st[b];
         const int r = S.fin_r;
-        if (r > mrhs_max_r()) return fail(FARCUR_EINVAL, "block %d: r = %d exceeds %d", b, r, mrhs_max_r());
-        rmax = std::max(rmax, r);
         const int32_t* hrows = &B->row_ids[B->row_off[b]];
         const int32_t* hcols = &B->col_ids[B->col_off[b]];
         const int32_t* drows = B->d_rows.as<int32_t>() + B->row_off[b];
         const int32_t* dcols = B->d_cols.as<int32_t>() + B->col_off[b];
         for (int o = 0; o < (symmetric ? 2 : 1); ++o) {
-            MrhsContrib c{};
-            c.r = r;
-            c.u = S.fin.U;
-            c.t2_off = t2_total;
-            if (o == 0) {
-                c.a1 = S.fin.R;
-                c.a1_ld = r;
-                c.a1_trans = 0;
-                c.k1 = S.n;
-                c.x_ids = dcols;
-                c.u_trans = 0;
-                c.l = S.fin.C;
-                c.l_ld = S.m;
-                c.l_trans = 0;
-            } else {
-                c.a1 = S.fin.C;
-                c.a1_ld = S.m;
-                c.a1_trans = 1;
-                c.k1 = S.m;
-                c.x_ids = drows;
-                c.u_trans = 1;
-                c.l = S.fin.R;
-                c.l_ld = r;
-                c.l_trans = 1;
+            MrhsSeg a{}, l{};
+            if (o == 0) {  // Y[rows] += P (R V[cols])
+                a = MrhsSeg{S.fin.R, r, 0, S.n, 0, dcols, 0};
+                l = MrhsSeg{B->interp[b], S.m, 0, r, 0, nullptr, t2_total};
+            } else {       // Y[cols] += R^T (P^T V[rows])
+                a = MrhsSeg{B->interp[b], S.m, 1, S.m, 0, drows, 0};
+                l = MrhsSeg{S.fin.R, r, 1, r, 0, nullptr, t2_total};
             }
             int g;
             if (o == 0)
                 TRY(group_of(hrows, drows, S.m, &g));
             else
                 TRY(group_of(hcols, dcols, S.n, &g));
-            group_contribs[g].push_back((int)cs.size());
-            cs.push_back(c);
+            const int prob = (int)pa.size();
+            pa.push_back(MrhsProb{nullptr, t2_total, r, (int)sa.size(), (int)sa.size() + 1, 0});
+            sa.push_back(a);
+            add_tiles(prob, r, ta_p, ta_r);
+            group_contribs[g].push_back((int)sb.size());
+            sb.push_back(l);
             t2_total += (size_t)r * s;
         }
     }
     const int ng = (int)group_m.size();
-    std::vector<MrhsGroup> groups(ng);
-    std::vector<int32_t> cog, tile_group, tile_row0;
+    std::vector<MrhsProb> pb(ng);
+    std::vector<MrhsSeg> sb_sorted;
+    std::vector<int32_t> tb_p, tb_r;
+    sb_sorted.reserve(sb.size());
     for (int g = 0; g < ng; ++g) {
-        groups[g].ids = group_ids_dev[g];
-        groups[g].m = group_m[g];
-        groups[g].c_begin = (int)cog.size();
-        cog.insert(cog.end(), group_contribs[g].begin(), group_contribs[g].end());
-        groups[g].c_end = (int)cog.size();
-        for (int r0 = 0; r0 < group_m[g]; r0 += 64) {
-            tile_group.push_back(g);
-            tile_row0.push_back(r0);
-        }
+        pb[g] = MrhsProb{group_ids_dev[g], 0, group_m[g], (int)sb_sorted.size(), 0, 0};
+        for (int c : group_contribs[g]) sb_sorted.push_back(sb[c]);
+        pb[g].seg_end = (int)sb_sorted.size();
+        add_tiles(g, group_m[g], tb_p, tb_r);
     }
-    // Device buffers: V and Y row-major complex64, T2, descriptors.
-    DBuf dV, dY, dT2, dcs, dg, dcog, dtg, dtr;
+    // Device buffers: V and Y row-major complex64, T, descriptors.
+    DBuf dV, dY, dT, dpa, dsa, dtap, dtar, dpb, dsb, dtbp, dtbr;
     TRY(dV.ensure((size_t)N * s * 8));
     TRY(dY.ensure((size_t)N * s * 8));
-    TRY(dT2.ensure(std::max<size_t>(t2_total, 1) * 8));
-    TRY(dcs.ensure(cs.size() * sizeof(MrhsContrib)));
-    TRY(dg.ensure(groups.size() * sizeof(MrhsGroup)));
-    TRY(dcog.ensure(cog.size() * 4));
-    TRY(dtg.ensure(tile_group.size() * 4));
-    TRY(dtr.ensure(tile_row0.size() * 4));
+    TRY(dT.ensure(std::max<size_t>(t2_total, 1) * 8));
+    auto up = [&](DBuf& d, const void* h, size_t bytes) -> int {
+        TRY(d.ensure(std::max<size_t>(bytes, 4)));
+        if (bytes) HIPTRY(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, st));
+        return FARCUR_OK;
+    };
+    TRY(up(dpa, pa.data(), pa.size() * sizeof(MrhsProb)));
+    TRY(up(dsa, sa.data(), sa.size() * sizeof(MrhsSeg)));
+    TRY(up(dtap, ta_p.data(), ta_p.size() * 4));
+    TRY(up(dtar, ta_r.data(), ta_r.size() * 4));
+    TRY(up(dpb, pb.data(), pb.size() * sizeof(MrhsProb)));
+    TRY(up(dsb, sb_sorted.data(), sb_sorted.size() * sizeof(MrhsSeg)));
+    TRY(up(dtbp, tb_p.data(), tb_p.size() * 4));
+    TRY(up(dtbr, tb_r.data(), tb_r.size() * 4));
     std::vector<float2> hv((size_t)N * s);
     for (int j = 0; j < s; ++j)
         for (int i = 0; i < N; ++i)
             hv[(size_t)i * s + j] = make_float2((float)V[(size_t)j * N + i].re, (float)V[(size_t)j * N + i].im);
     HIPTRY(hipMemcpyAsync(dV.p, hv.data(), hv.size() * 8, hipMemcpyHostToDevice, st));
     HIPTRY(hipMemsetAsync(dY.p, 0, (size_t)N * s * 8, st));
-    HIPTRY(hipMemcpyAsync(dcs.p, cs.data(), cs.size() * sizeof(MrhsContrib), hipMemcpyHostToDevice, st));
-    HIPTRY(hipMemcpyAsync(dg.p, groups.data(), groups.size() * sizeof(MrhsGroup), hipMemcpyHostToDevice, st));
-    HIPTRY(hipMemcpyAsync(dcog.p, cog.data(), cog.size() * 4, hipMemcpyHostToDevice, st));
-    HIPTRY(hipMemcpyAsync(dtg.p, tile_group.data(), tile_group.size() * 4, hipMemcpyHostToDevice, st));
-    HIPTRY(hipMemcpyAsync(dtr.p, tile_row0.data(), tile_row0.size() * 4, hipMemcpyHostToDevice, st));
     hipEvent_t e0, e1, e2;
     HIPTRY(hipEventCreate(&e0));
     HIPTRY(hipEventCreate(&e1));
     HIPTRY(hipEventCreate(&e2));
     HIPTRY(hipEventRecord(e0, st));
-    launch_mrhs_pass_a(dcs.as<MrhsContrib>(), (int)cs.size(), rmax, B->f32, dV.as<float2>(), s, dT2.as<float2>(), st);
+    launch_mrhs_gemm(dpa.as<MrhsProb>(), (int)ta_p.size(), dtap.as<int32_t>(), dtar.as<int32_t>(), dsa.as<MrhsSeg>(),
+                     B->f32, mode, dV.as<float2>(), s, dT.as<float2>(), st);
     HIPTRY(hipEventRecord(e1, st));
-    launch_mrhs_pass_b(dg.as<MrhsGroup>(), (int)tile_group.size(), dtg.as<int32_t>(), dtr.as<int32_t>(),
-                       dcs.as<MrhsContrib>(), dcog.as<int32_t>(), B->f32, mode, dT2.as<float2>(), s,
-                       dY.as<float2>(), st);
+    launch_mrhs_gemm(dpb.as<MrhsProb>(), (int)tb_p.size(), dtbp.as<int32_t>(), dtbr.as<int32_t>(), dsb.as<MrhsSeg>(),
+                     B->f32, mode, dT.as<float2>(), s, dY.as<float2>(), st);
     HIPTRY(hipEventRecord(e2, st));
     HIPTRY(hipGetLastError());
     std::vector<float2> hy((size_t)N * s);
@@ -2321,11 +2366,10 @@ extern "C" int farcur_batch_matvec(farcur_batch* B, const farcur_c128* V, int s,
     if (ms) {
         ms[0] = ta;
         ms[1] = tb;
+        ms[2] = ms_interp;
     }
     for (int j = 0; j < s; ++j)
         for (int i = 0; i < N; ++i) Y[(size_t)j * N + i] = farcur_c128{hy[(size_t)i * s + j].x, hy[(size_t)i * s + j].y};
-    DBuf* bufs[] = {&dV, &dY, &dT2, &dcs, &dg, &dcog, &dtg, &dtr};
-    for (DBuf* b : bufs) b->release();
     return FARCUR_OK;
 }
 

@@ -9,36 +9,41 @@ namespace farcur {
 
 enum : int { kMrhsValu = 0, kMrhsMfmaF32 = 1, kMrhsMfmaBf16x3 = 2 };
 
-// One contribution: a block in one orientation. Pass A forms
-// T2 = U' (A1 X) with X = V[x_ids]; pass B adds L T2 to the target group.
-struct MrhsContrib {
-    const void* a1;        // R (r x n, ld r) or C (m x r, ld m) read transposed
-    int32_t a1_ld, a1_trans;
-    int32_t k1;            // columns of A1 (n, or m when transposed)
-    int32_t r;
-    const int32_t* x_ids;  // k1 RWG ids selecting rows of V
-    const c128* u;         // r x r
-    int32_t u_trans, pad0;
-    const void* l;         // C (m x r, ld m) or R (r x n) read transposed
-    int32_t l_ld, l_trans;
-    size_t t2_off;         // offset (complex elements) of T2 (r x s, row-major)
+// Interpolation matrix of one block: P = C U (m x r, column-major), formed in
+// FP64 and stored in the panel precision.
+struct MrhsInterp {
+    const void* c;     // C, m x r column-major (panel precision)
+    const c128* u;     // U, r x r column-major
+    void* p;           // P, m x r column-major (panel precision)
+    int32_t m, r;
 };
 
-// A target group: rows ids[0..m) of Y, contributions [c_begin, c_end) of the
-// contribution-of-group list.
-struct MrhsGroup {
-    const int32_t* ids;
-    int32_t m;
-    int32_t c_begin, c_end, pad;
+// One K segment of a grouped GEMM problem: A (M x K) times B (K x s).
+//   A[i][k] = a_trans ? a[i * lda + k] : a[k * lda + i]   (panel precision)
+//   B[k][j] = b_ids ? B[b_ids[k] * s + j] : B[b_off + k * s + j]
+struct MrhsSeg {
+    const void* a;
+    int32_t lda, a_trans;
+    int32_t k, pad;
+    const int32_t* b_ids;
+    size_t b_off;
 };
 
-int mrhs_max_r();
+// One problem: rows [0, m) of the output, the sum over segments
+// [seg_begin, seg_end). Row i goes to Out[out_ids[i] * s + j] when out_ids is
+// set, else to Out[out_off + i * s + j]. Every output row belongs to exactly one
+// problem, so the kernel stores without atomics.
+struct MrhsProb {
+    const int32_t* out_ids;
+    size_t out_off;
+    int32_t m, seg_begin, seg_end, pad;
+};
+
 int mrhs_chunk();
-// rmax: largest r of the listed contributions (<= mrhs_max_r()).
-void launch_mrhs_pass_a(const MrhsContrib* cs, int nc, int rmax, int f32, const float2* V, int s, float2* T2,
-                        hipStream_t st);
-void launch_mrhs_pass_b(const MrhsGroup* groups, int ntiles, const int32_t* tile_group, const int32_t* tile_row0,
-                        const MrhsContrib* cs, const int32_t* contrib_of_group, int f32, int mode, const float2* T2,
-                        int s, float2* Y, hipStream_t st);
+void launch_mrhs_interp(const MrhsInterp* jobs, int nb, int f32, hipStream_t st);
+// ntiles 64-row tiles (problem tile_prob[t], first row tile_row0[t]) times
+// ceil(s / 64) right-hand-side chunks.
+void launch_mrhs_gemm(const MrhsProb* probs, int ntiles, const int32_t* tile_prob, const int32_t* tile_row0,
+                      const MrhsSeg* segs, int f32, int mode, const float2* B, int s, float2* Out, hipStream_t st);
 
 }  // namespace farcur

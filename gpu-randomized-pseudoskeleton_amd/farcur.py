@@ -491,12 +491,13 @@ class Batch:
         check(lib().farcur_batch_set_model(self._h, model._h if model is not None else None))
 
     def matvec(self, V, mode=1, symmetric=True):
-        """Y (N x s) = compressed far field times V (N x s); returns (Y, [ms pass A, ms pass B])."""
+        """Y (N x s) = compressed far field times V (N x s); returns
+        (Y, [ms pass A, ms pass B, ms forming P = C U (0 once formed)])."""
         V = np.asfortranarray(V, dtype=np.complex128)
         if V.ndim == 1:
             V = V.reshape(-1, 1, order="F")
         Y = np.empty_like(V, order="F")
-        ms = np.zeros(2)
+        ms = np.zeros(3)
         check(lib().farcur_batch_matvec(self._h, _p(V), V.shape[1], _p(Y), mode, 1 if symmetric else 0, _p(ms)))
         return Y, ms
 

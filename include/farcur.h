@@ -283,9 +283,15 @@ int farcur_batch_misses(farcur_batch* batch);
  * codes/GPU_implementation/seed/compres_single_block_spheres.jl:2608-2612).
  * V and Y are N x nrhs column-major (N = RWGs of the geometry); V is cast to
  * complex64 and Y accumulated in FP32. Rows of Y outside every target set are
- * zero. The blocks' row (and column) sets must be disjoint groups or identical.
+ * zero. The blocks' row (and column) id lists must be disjoint or identical
+ * (same ids in the same order).
  * mode: FARCUR_MV_VALU, FARCUR_MV_MFMA_F32, FARCUR_MV_MFMA_BF16X3.
- * ms (may be NULL) receives the device ms of pass A and pass B. */
+ * The product runs in interpolative form, A_b = P_b R_b with P_b = C_b U_b
+ * formed in FP64 on the first call after a compress (both grouped GEMMs then
+ * have intermediates the size of the result; the core U alone is as
+ * ill-conditioned as the kept W).
+ * ms (may be NULL) receives 3 values: device ms of pass A, of pass B, and of
+ * forming P (0 when already formed). */
 enum { FARCUR_MV_VALU = 0, FARCUR_MV_MFMA_F32 = 1, FARCUR_MV_MFMA_BF16X3 = 2 };
 int farcur_batch_matvec(farcur_batch* batch, const farcur_c128* V, int nrhs, farcur_c128* Y, int mode, int symmetric,
                         double* ms);

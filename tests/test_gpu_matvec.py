@@ -1,6 +1,8 @@
 """Multi-RHS product of the compressed far field (VALU, FP32 MFMA, split
 BF16x3 MFMA) against numpy on the downloaded C, U, R panels and against the
-dense FP64 far field."""
+dense FP64 far field. The product runs in interpolative form P (R X) with
+P = C U formed in FP64, so FP32 arithmetic keeps the tolerances below even
+though the core U is ill-conditioned."""
 import numpy as np
 import pytest
 
@@ -46,6 +48,26 @@ def test_valu_matches_numpy(farcur, compressed, symmetric):
     Yr = _reference(bt, blocks, V.astype(np.complex64).astype(np.complex128), symmetric)
     assert np.linalg.norm(Y - Yr) <= 2e-5 * np.linalg.norm(Yr)
     assert ms[0] > 0 and ms[1] > 0
+    # P = C U is formed once per compress
+    Y2, ms2 = bt.matvec(V, farcur.MV_VALU, symmetric)
+    assert ms2[2] == 0.0
+    np.testing.assert_array_equal(Y2, Y)
+
+
+def test_permuted_group_rejected(farcur, compressed):
+    # two blocks over the same RWG set in different orders cannot share a target group
+    ctx, bt, g, blocks, N = compressed
+    ro, ri, co, cix = blocks
+    rows = ri[ro[0]:ro[1]]
+    cols = cix[co[0]:co[1]]
+    r2 = np.concatenate([rows, rows[::-1]]).astype(np.int32)
+    c2 = np.concatenate([cols, cols]).astype(np.int32)
+    m, n = rows.size, cols.size
+    b2 = farcur.Batch(ctx, np.array([0, m, 2 * m], np.int32), r2, np.array([0, n, 2 * n], np.int32), c2)
+    b2.compress(farcur.MODE_BATCHED, tol=1e-4, seed=4, precision=farcur.F32)
+    with pytest.raises(farcur.FarcurError):
+        b2.matvec(np.ones((N, 2)), farcur.MV_VALU, False)
+    b2.close()
 
 
 @pytest.mark.parametrize("mode,tol", [(1, 2e-5), (2, 1e-4)])
