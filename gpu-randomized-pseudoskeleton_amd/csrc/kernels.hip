@@ -270,6 +270,9 @@ void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32
 // ---------------------------------------------------------------- pinv
 // LDS variant: A, V and the weights of one problem in dynamic LDS sized for
 // the largest r of the launch (r <= kPinvLdsMaxR); one workgroup per problem.
+// Team width kT and cached elements kE follow from that r and the launch's
+// thread count (pinv_lds_shape).
+template <int kT, int kE>
 __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist, int rmax,
                            double atol, double rtol, double label_eps, JobOut* __restrict__ out) {
     extern __shared__ __align__(16) char smem[];
@@ -284,7 +287,8 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
         for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) s_a[idx] = J.W[idx];
         int kept, reps, sweeps;
         double smax;
-        jacobi_pinv(s_a, s_v, r, atol, rtol, J.U, r, J.sigma, s_flag, s_w, &kept, &smax, label_eps, &reps, &sweeps);
+        jacobi_pinv_t<kT, kE>(s_a, s_v, r, atol, rtol, J.U, r, J.sigma, s_flag, s_w, &kept, &smax, label_eps, &reps,
+                              &sweeps);
         if (threadIdx.x == 0) {
             out[jb].kept = kept;
             out[jb].sigma_max = smax;
@@ -297,6 +301,23 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
 
 // Global-memory variant (any r): A = the W panel, V = the job's scratch; the
 // weights stay in LDS up to r = 1024, beyond that after V (pinv_v_elems).
+template <int kT>
+__device__ void pinv_global_one(const BlockJob& J, int jb, double atol, double rtol, double label_eps, double* s_w,
+                                int* s_flag, JobOut* __restrict__ out) {
+    const int r = J.r;
+    double* w = r <= 1024 ? s_w : (double*)(J.V + (size_t)r * r);
+    int kept, reps, sweeps;
+    double smax;
+    jacobi_pinv_t<kT, 0>(J.W, J.V, r, atol, rtol, J.U, r, J.sigma, s_flag, w, &kept, &smax, label_eps, &reps,
+                         &sweeps);
+    if (threadIdx.x == 0) {
+        out[jb].kept = kept;
+        out[jb].sigma_max = smax;
+        out[jb].rank_eps = reps;
+        out[jb].sweeps = sweeps;
+    }
+}
+
 __global__ void k_pinv_global(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist,
                               double atol, double rtol, double label_eps, JobOut* __restrict__ out) {
     __shared__ double s_w[1024];
@@ -304,16 +325,14 @@ __global__ void k_pinv_global(const BlockJob* __restrict__ jobs, const int32_t* 
     for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
         const int jb = list[li];
         const BlockJob& J = jobs[jb];
-        const int r = J.r;
-        double* w = r <= 1024 ? s_w : (double*)(J.V + (size_t)r * r);
-        int kept, reps, sweeps;
-        double smax;
-        jacobi_pinv(J.W, J.V, r, atol, rtol, J.U, r, J.sigma, &s_flag, w, &kept, &smax, label_eps, &reps, &sweeps);
-        if (threadIdx.x == 0) {
-            out[jb].kept = kept;
-            out[jb].sigma_max = smax;
-            out[jb].rank_eps = reps;
-            out[jb].sweeps = sweeps;
+        switch (jacobi_team(J.r, blockDim.x)) {
+            case 1: pinv_global_one<1>(J, jb, atol, rtol, label_eps, s_w, &s_flag, out); break;
+            case 2: pinv_global_one<2>(J, jb, atol, rtol, label_eps, s_w, &s_flag, out); break;
+            case 4: pinv_global_one<4>(J, jb, atol, rtol, label_eps, s_w, &s_flag, out); break;
+            case 8: pinv_global_one<8>(J, jb, atol, rtol, label_eps, s_w, &s_flag, out); break;
+            case 16: pinv_global_one<16>(J, jb, atol, rtol, label_eps, s_w, &s_flag, out); break;
+            case 32: pinv_global_one<32>(J, jb, atol, rtol, label_eps, s_w, &s_flag, out); break;
+            default: pinv_global_one<64>(J, jb, atol, rtol, label_eps, s_w, &s_flag, out); break;
         }
         __syncthreads();
     }
@@ -321,19 +340,59 @@ __global__ void k_pinv_global(const BlockJob* __restrict__ jobs, const int32_t* 
 
 size_t pinv_lds_bytes(int rmax) { return (size_t)32 * rmax * rmax + (size_t)8 * rmax + 16; }
 
+// (T, E) of the LDS variant for class size rmax on `threads` threads: T the
+// team width (at most 32), E the elements per lane.
+void pinv_lds_shape(int rmax, int threads, int* T, int* E) {
+    *T = std::min(jacobi_team(rmax, threads), 32);
+    *E = (rmax + *T - 1) / *T;
+}
+
+namespace {
+
+template <int kT, int kE>
+void launch_lds(int grid, int threads, size_t lds, hipStream_t s, const BlockJob* jobs, const int32_t* list,
+                int nlist, int rmax, double atol, double rtol, double label_eps, JobOut* out) {
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)k_pinv_lds<kT, kE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)pinv_lds_bytes(kPinvLdsMaxR));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_pinv_lds<kT, kE>), dim3(grid), dim3(threads), lds, s, jobs, list, nlist, rmax, atol, rtol,
+                       label_eps, out);
+}
+
+// E rounded up to a compiled register count; past 16 elements the team
+// streams its columns from LDS (kE = 0).
+template <int kT>
+void launch_lds_t(int E, int grid, int threads, size_t lds, hipStream_t s, const BlockJob* jobs,
+                  const int32_t* list, int nlist, int rmax, double atol, double rtol, double label_eps, JobOut* out) {
+    if (E <= 1) launch_lds<kT, 1>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out);
+    else if (E <= 2) launch_lds<kT, 2>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out);
+    else if (E <= 4) launch_lds<kT, 4>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out);
+    else if (E <= 8) launch_lds<kT, 8>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out);
+    else if (E <= 16) launch_lds<kT, 16>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out);
+    else launch_lds<kT, 0>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out);
+}
+
+}  // namespace
+
 void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, double label_eps,
                  int rmax, int threads, JobOut* out, hipStream_t s) {
     if (nlist <= 0) return;
     const int grid = nlist < 16384 ? nlist : 16384;
     if (rmax > 0 && rmax <= kPinvLdsMaxR) {
-        static bool attr_set = false;
-        if (!attr_set) {
-            (void)hipFuncSetAttribute((const void*)k_pinv_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)pinv_lds_bytes(kPinvLdsMaxR));
-            attr_set = true;
+        int T, E;
+        pinv_lds_shape(rmax, threads, &T, &E);
+        const size_t lds = pinv_lds_bytes(rmax);
+        switch (T) {
+            case 1:
+            case 2: launch_lds_t<2>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out); break;
+            case 4: launch_lds_t<4>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out); break;
+            case 8: launch_lds_t<8>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out); break;
+            case 16: launch_lds_t<16>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out); break;
+            default: launch_lds_t<32>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out); break;
         }
-        hipLaunchKernelGGL(k_pinv_lds, dim3(grid), dim3(threads), pinv_lds_bytes(rmax), s, jobs, list, nlist, rmax,
-                           atol, rtol, label_eps, out);
     } else {
         hipLaunchKernelGGL(k_pinv_global, dim3(grid), dim3(threads), 0, s, jobs, list, nlist, atol, rtol, label_eps,
                            out);
