@@ -156,7 +156,7 @@ def large_run(fc, args, log):
     model, model_src = rank_model(fc, ctx, log)
     ctx.set_mesh(mesh)
     log(f"# large workload: {N} RWGs, {part.ngroups} groups, {nb} far blocks (sizes {m.min()}-{max(m.max(), n.max())})")
-    opts = dict(tol=1e-4, seed=12345, precision=fc.F32, pinv_rtol=1e-6, heldout=8, oversample=10)
+    opts = dict(tol=1e-4, seed=12345, precision=fc.F32, pinv_rtol=1e-6, heldout=8, oversample=14)
     bt = fc.Batch(ctx, ro, ri, co, cix)
     bt.set_model(model)
     bt.compress(fc.MODE_LEARNED, **opts)
@@ -237,7 +237,7 @@ def main():
     ctx.set_mesh(mesh)
     prec = fc.F32 if args.precision == "f32" else fc.F64
     opts = dict(tol=1e-4, seed=12345, precision=prec, pinv_rtol=1e-6 if prec == fc.F32 else 0.0, heldout=8,
-                oversample=10)
+                oversample=14)
     log(f"# workload: {mesh.sizes()[2]} RWGs, {part.ngroups} groups, {nb} far blocks "
         f"(sizes {m.min()}-{max(m.max(), n.max())}), represented {represented:.3e} entries")
 

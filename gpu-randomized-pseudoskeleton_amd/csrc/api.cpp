@@ -538,7 +538,7 @@ void farcur_opts_default(farcur_opts* o) {
     o->precision = FARCUR_F64;
     o->row_mode = FARCUR_ROWS_UNIFORM;
     o->heldout = 8;
-    o->oversample = 10;
+    o->oversample = 14;
     o->label_eps = 1e-4;
     o->heldout_safety = 0.5;
     o->growth = 1.5;

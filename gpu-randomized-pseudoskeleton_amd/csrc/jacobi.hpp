@@ -107,24 +107,14 @@ __device__ __forceinline__ void rotate_pair(const Rot& R, c128& a, c128& b) {
     b = c128{R.s * x.re + R.c * btr, R.s * x.im + R.c * bti};
 }
 
-// All threads of the workgroup must call. A, V: r x r col-major scratch (LDS or
-// global). Teams of kT lanes; with kE > 0 every r <= kE * kT (elements cached
-// in registers), with kE = 0 any r (elements streamed). On exit U (ld = ldu)
-// holds W^+ with singular values below tau = max(atol, rtol * sigma_max)
-// dropped, sigma[j] the singular value carried by column j (unsorted),
-// *kept_out the number of sigma >= tau and *smax_out sigma_max, *rank_eps_out
-// the number of sigma >= label_eps * sigma_max, *sweeps_out the sweeps run.
-// s_flag: one int of LDS; s_w: r doubles of LDS or global.
+// Sweeps of the circle schedule over the columns of A (r x r, col-major, LDS
+// or global), rotations accumulated into V. All threads of the workgroup must
+// call; teams of kT lanes; with kE > 0 every r <= kE * kT (elements cached in
+// registers), with kE = 0 any r (elements streamed). s_flag: one int of LDS.
+// Returns the number of sweeps.
 template <int kT, int kE>
-__device__ __forceinline__ void jacobi_pinv_t(c128* A, c128* V, int r, double atol, double rtol,
-                                              c128* __restrict__ U, int ldu, double* __restrict__ sigma,
-                                              int* s_flag, double* s_w, int* kept_out, double* smax_out,
-                                              double label_eps, int* rank_eps_out, int* sweeps_out) {
+__device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int* s_flag) {
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int idx = tid; idx < r * r; idx += nt) {
-        const int i = idx % r, j = idx / r;
-        V[idx] = c128{i == j ? 1.0 : 0.0, 0.0};
-    }
     const int n = r + (r & 1);
     const int npairs = n / 2;
     const int nteams = nt / kT;
@@ -134,9 +124,11 @@ __device__ __forceinline__ void jacobi_pinv_t(c128* A, c128* V, int r, double at
     __syncthreads();
     // s_flag holds the largest squared relative coupling |a_p^H a_q|^2 /
     // (|a_p|^2 |a_q|^2) rotated in the sweep (float bits; 0 = no rotation).
-    // Jacobi converges quadratically, so once every coupling of a sweep is
-    // below 1e-7 the next sweep would only see couplings near the rounding
-    // level and is skipped.
+    // Jacobi converges quadratically: after a sweep whose couplings were all
+    // below c the remaining ones are of order c^2. The loop stops after a
+    // sweep with every coupling below 1e-5, leaving the columns orthogonal to
+    // about 1e-10 relative: the weights 1 / |a_j|^2 and the products below then
+    // differ from those of exactly orthogonal columns at that level only.
     int sweeps = 0;
     for (int sweep = 0; sweep < kJacobiMaxSweeps && n > 1; ++sweep) {
         ++sweeps;
@@ -220,8 +212,21 @@ __device__ __forceinline__ void jacobi_pinv_t(c128* A, c128* V, int r, double at
         }
         const int maxc = *s_flag;
         __syncthreads();
-        if (maxc == 0 || __int_as_float(maxc) < 1e-14f) break;
+        if (maxc == 0 || __int_as_float(maxc) < 1e-10f) break;
     }
+    return sweeps;
+}
+
+// Singular values sigma[j] = |a_j| of the converged columns, the pinv weights
+// s_w[j] = 1 / sigma_j^2 for sigma_j >= tau = max(atol, rtol * sigma_max)
+// (else 0), *kept the number of weights set, *smax sigma_max and *reps the
+// number of sigma_j >= label_eps * sigma_max. All threads must call.
+template <int kT>
+__device__ __forceinline__ void jacobi_weights(const c128* A, int r, double atol, double rtol, double label_eps,
+                                               double* __restrict__ sigma, double* s_w, int* kept_out,
+                                               double* smax_out, int* reps_out) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nteams = nt / kT, team = tid / kT, tl = tid - team * kT;
     for (int j = team; j < r; j += nteams) {
         const c128* aj = A + (size_t)j * r;
         double nn = 0;
@@ -247,25 +252,222 @@ __device__ __forceinline__ void jacobi_pinv_t(c128* A, c128* V, int r, double at
         s_w[j] = (sigma[j] >= tau && nn > 0) ? 1.0 / nn : 0.0;
     }
     __syncthreads();
-    // U[i, k] = sum_j V[i, j] w_j conj(A[k, j])
-    for (int idx = tid; idx < r * r; idx += nt) {
+    *kept_out = kept;
+    *smax_out = smax;
+    *reps_out = reps;
+}
+
+// U[k, map(i)] = sum_j L[i, j] w_j conj(M[k, j]) (L, M r x r col-major; map the
+// identity when rowmap is NULL). All threads must call.
+__device__ __forceinline__ void pinv_form(const c128* L, const c128* M, const double* s_w, int r,
+                                          const int* rowmap, c128* __restrict__ U, int ldu) {
+    for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
         const int i = idx % r, k = idx / r;
         double ur = 0, ui = 0;
         for (int j = 0; j < r; ++j) {
             const double w = s_w[j];
             if (w == 0.0) continue;
-            const c128 v = V[(size_t)j * r + i];
-            const c128 a = A[(size_t)j * r + k];
+            const c128 v = L[(size_t)j * r + i];
+            const c128 a = M[(size_t)j * r + k];
             // v * conj(a) * w
             ur += w * (v.re * a.re + v.im * a.im);
             ui += w * (v.im * a.re - v.re * a.im);
         }
-        U[(size_t)k * ldu + i] = c128{ur, ui};
+        U[(size_t)k * ldu + (rowmap ? rowmap[i] : i)] = c128{ur, ui};
     }
-    *kept_out = kept;
-    *smax_out = smax;
-    *rank_eps_out = reps;
-    *sweeps_out = sweeps;
+}
+
+// One-sided Jacobi pinv of W = A (r x r, col-major, LDS or global; destroyed),
+// V: r x r scratch. On exit U (ld = ldu) holds W^+ = sum_j [s_j >= tau]
+// v_j a_j^H / s_j^2 with A_final = W V, sigma[j] the singular value carried by
+// column j (unsorted), plus the counts of jacobi_weights and the sweeps.
+// s_flag: one int of LDS; s_w: r doubles of LDS or global.
+template <int kT, int kE>
+__device__ __forceinline__ void jacobi_pinv_t(c128* A, c128* V, int r, double atol, double rtol,
+                                              c128* __restrict__ U, int ldu, double* __restrict__ sigma,
+                                              int* s_flag, double* s_w, int* kept_out, double* smax_out,
+                                              double label_eps, int* rank_eps_out, int* sweeps_out) {
+    for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
+        const int i = idx % r, j = idx / r;
+        V[idx] = c128{i == j ? 1.0 : 0.0, 0.0};
+    }
+    *sweeps_out = jacobi_sweeps<kT, kE>(A, V, r, s_flag);
+    jacobi_weights<kT>(A, r, atol, rtol, label_eps, sigma, s_w, kept_out, smax_out, rank_eps_out);
+    pinv_form(V, A, s_w, r, nullptr, U, ldu);
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- QR-preconditioned
+// Householder QR with column pivoting, A P = Q R (A r x r col-major in LDS).
+// Step k takes the trailing column of largest norm (first on ties) as logical
+// column k and reflects it onto alpha_k e_k with
+//   H_k = I - tau_k v_k v_k^H,  v_k = x - alpha_k e_k,  alpha_k = -e^{i arg x_k} |x|,
+//   tau_k = 2 / |v_k|^2 = 1 / (|x|^2 + |x| |x_k|)   (real; H_k Hermitian and unitary).
+// The map from logical to physical columns and the trailing norms are double
+// buffered, so each step needs one barrier. On exit:
+//   R[i][j] = A[phys[j] * r + i] for i < j, R[j][j] = s_alpha[j],
+//   v_k (rows k..r-1) in Wg[k * r + i] (Wg global, r x r), tau_k in s_tau[k],
+//   phys = s_phys (the final map).
+// Returns the buffer (0 or 1) of s_phys / s_norm that holds the final map.
+template <int kT>
+__device__ __forceinline__ int qrcp_lds(c128* A, int r, int* s_phys, double* s_norm, double* s_tau, c128* s_alpha,
+                                        c128* __restrict__ Wg) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nteams = nt / kT, team = tid / kT, tl = tid - team * kT;
+    __syncthreads();  // A was just written by every thread of the workgroup
+    for (int j = tid; j < r; j += nt) s_phys[j] = j;
+    for (int j = team; j < r; j += nteams) {
+        const c128* aj = A + (size_t)j * r;
+        double nn = 0;
+        for (int i = tl; i < r; i += kT) nn += aj[i].re * aj[i].re + aj[i].im * aj[i].im;
+        nn = team_sum_t<kT>(nn);
+        if (tl == 0) s_norm[j] = nn;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int k = 0; k < r; ++k) {
+        const int* phys = s_phys + cur * r;
+        int* phys_next = s_phys + (cur ^ 1) * r;
+        const double* norm = s_norm + cur * r;
+        double* norm_next = s_norm + (cur ^ 1) * r;
+        int p = k;
+        double best = norm[phys[k]];
+        for (int j = k + 1; j < r; ++j) {
+            const double v = norm[phys[j]];
+            if (v > best) {
+                best = v;
+                p = j;
+            }
+        }
+        const int ck = phys[p], co = phys[k];
+        const c128 x0 = A[(size_t)ck * r + k];
+        const double xn = sqrt(best), ax0 = sqrt(x0.re * x0.re + x0.im * x0.im);
+        c128 alpha;
+        if (ax0 > 0) {
+            const double f = -xn / ax0;
+            alpha = c128{f * x0.re, f * x0.im};
+        } else {
+            alpha = c128{-xn, 0.0};
+        }
+        const c128 v0{x0.re - alpha.re, x0.im - alpha.im};
+        const double den = best + xn * ax0;
+        const double tau = den > 0 ? 1.0 / den : 0.0;
+        for (int j = tid; j < r; j += nt) phys_next[j] = j == k ? ck : (j == p ? co : phys[j]);
+        if (tid == 0) {
+            s_alpha[k] = alpha;
+            s_tau[k] = tau;
+            Wg[(size_t)k * r + k] = v0;
+        }
+        for (int j = k + 1 + team; j < r; j += nteams) {
+            int c = co;
+            if (j != p) c = phys[j];
+            c128* a = A + (size_t)c * r;
+            const c128* v = A + (size_t)ck * r;
+            // w = v^H a (row k uses v0)
+            double wr = 0, wi = 0;
+            for (int i = k + tl; i < r; i += kT) {
+                const c128 vi = i == k ? v0 : v[i];
+                const c128 ai = a[i];
+                wr += vi.re * ai.re + vi.im * ai.im;
+                wi += vi.re * ai.im - vi.im * ai.re;
+            }
+            wr = team_sum_t<kT>(wr) * tau;
+            wi = team_sum_t<kT>(wi) * tau;
+            double nn = 0;
+            for (int i = k + tl; i < r; i += kT) {
+                const c128 vi = i == k ? v0 : v[i];
+                c128 ai = a[i];
+                ai.re -= vi.re * wr - vi.im * wi;
+                ai.im -= vi.re * wi + vi.im * wr;
+                a[i] = ai;
+                if (i > k) nn += ai.re * ai.re + ai.im * ai.im;
+            }
+            nn = team_sum_t<kT>(nn);
+            if (tl == 0) norm_next[c] = nn;
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    // reflector bodies (rows below k) to global
+    const int* phys = s_phys + cur * r;
+    for (int idx = tid; idx < r * r; idx += nt) {
+        const int i = idx % r, k = idx / r;
+        if (i > k) Wg[idx] = A[(size_t)phys[k] * r + i];
+    }
+    return cur;
+}
+
+// Z <- Q Z = H_0 H_1 ... H_{r-1} Z for the reflectors of qrcp_lds (v_k in Wg,
+// tau_k in s_tau). Columns of Z are independent: a team applies every
+// reflector to its columns with team reductions only.
+template <int kT>
+__device__ __forceinline__ void apply_q_left(c128* Z, int r, const c128* __restrict__ Wg, const double* s_tau) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nteams = nt / kT, team = tid / kT, tl = tid - team * kT;
+    for (int j = team; j < r; j += nteams) {
+        c128* z = Z + (size_t)j * r;
+        for (int k = r - 1; k >= 0; --k) {
+            const c128* v = Wg + (size_t)k * r;
+            const double tau = s_tau[k];
+            double sr = 0, si = 0;
+            for (int i = k + tl; i < r; i += kT) {  // s = v^H z
+                const c128 vi = v[i], zi = z[i];
+                sr += vi.re * zi.re + vi.im * zi.im;
+                si += vi.re * zi.im - vi.im * zi.re;
+            }
+            sr = team_sum_t<kT>(sr) * tau;
+            si = team_sum_t<kT>(si) * tau;
+            for (int i = k + tl; i < r; i += kT) {
+                const c128 vi = v[i];
+                c128 zi = z[i];
+                zi.re -= vi.re * sr - vi.im * si;
+                zi.im -= vi.re * si + vi.im * sr;
+                z[i] = zi;
+            }
+        }
+    }
+}
+
+// Pinv of W (in A, LDS; destroyed) preconditioned by QR with column pivoting:
+// W P = Q R, one-sided Jacobi on X = R^H (X V = X_f, orthogonal columns
+// x_j = sigma_j u_j), and
+//   W^+ = P X_f diag(w) (Q V)^H,  w_j = [sigma_j >= tau] / sigma_j^2,
+// since R = V Sigma U^H gives W = (Q V) Sigma (P U)^H. X = R^H of a
+// column-pivoted R has nearly orthogonal columns, so the sweeps drop to
+// about 5 from about 13 for the graded core matrices of far-field blocks.
+// B: second r x r LDS buffer; Wg: the global W panel (receives the
+// reflectors). Outputs as jacobi_pinv_t.
+template <int kT, int kE>
+__device__ __forceinline__ void pinv_qr_lds(c128* A, c128* B, int r, double atol, double rtol, c128* __restrict__ U,
+                                            int ldu, double* __restrict__ sigma, c128* __restrict__ Wg, int* s_flag,
+                                            double* s_w, int* s_phys, double* s_norm, double* s_tau,
+                                            c128* s_alpha, int* kept_out, double* smax_out, double label_eps,
+                                            int* rank_eps_out, int* sweeps_out) {
+    const int cur = qrcp_lds<kT>(A, r, s_phys, s_norm, s_tau, s_alpha, Wg);
+    const int* phys = s_phys + cur * r;
+    __syncthreads();
+    // X = R^H into B (X[i][j] = conj(R[j][i]) for i >= j)
+    for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
+        const int i = idx % r, j = idx / r;
+        c128 x{0.0, 0.0};
+        if (i == j) {
+            x = c128{s_alpha[j].re, -s_alpha[j].im};
+        } else if (i > j) {
+            const c128 rji = A[(size_t)phys[i] * r + j];
+            x = c128{rji.re, -rji.im};
+        }
+        B[idx] = x;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
+        const int i = idx % r, j = idx / r;
+        A[idx] = c128{i == j ? 1.0 : 0.0, 0.0};
+    }
+    *sweeps_out = jacobi_sweeps<kT, kE>(B, A, r, s_flag);
+    jacobi_weights<kT>(B, r, atol, rtol, label_eps, sigma, s_w, kept_out, smax_out, rank_eps_out);
+    apply_q_left<kT>(A, r, Wg, s_tau);
+    __syncthreads();
+    pinv_form(B, A, s_w, r, phys, U, ldu);
     __syncthreads();
 }
 

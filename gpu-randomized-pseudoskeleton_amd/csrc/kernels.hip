@@ -268,27 +268,46 @@ void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32
 }
 
 // ---------------------------------------------------------------- pinv
-// LDS variant: A, V and the weights of one problem in dynamic LDS sized for
-// the largest r of the launch (r <= kPinvLdsMaxR); one workgroup per problem.
-// Team width kT and cached elements kE follow from that r and the launch's
-// thread count (pinv_lds_shape).
+// LDS variant: W (then X = R^H), the accumulated rotations and the small
+// per-column arrays of one problem in dynamic LDS sized for the largest r of
+// the launch (r <= kPinvLdsMaxR); one workgroup per problem. The pinv is
+// preconditioned by QR with column pivoting (pinv_qr_lds); the reflectors go to
+// the problem's W panel in HBM. Team width kT and cached elements kE follow
+// from that r and the launch's thread count (pinv_lds_shape).
+struct PinvLds {
+    c128 *a, *b, *alpha;
+    double *w, *norm, *tau;
+    int *phys, *flag;
+    __device__ PinvLds(char* smem, int rmax) {
+        a = (c128*)smem;
+        b = a + (size_t)rmax * rmax;
+        alpha = b + (size_t)rmax * rmax;
+        w = (double*)(alpha + rmax);
+        norm = w + rmax;     // 2 x rmax
+        tau = norm + 2 * rmax;
+        phys = (int*)(tau + rmax);  // 2 x rmax
+        flag = phys + 2 * rmax;
+    }
+};
+
+size_t pinv_lds_bytes(int rmax) {
+    return (size_t)32 * rmax * rmax + (size_t)16 * rmax + (size_t)8 * 4 * rmax + (size_t)4 * 2 * rmax + 16;
+}
+
 template <int kT, int kE>
 __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist, int rmax,
                            double atol, double rtol, double label_eps, JobOut* __restrict__ out) {
     extern __shared__ __align__(16) char smem[];
-    c128* s_a = (c128*)smem;
-    c128* s_v = s_a + (size_t)rmax * rmax;
-    double* s_w = (double*)(s_v + (size_t)rmax * rmax);
-    int* s_flag = (int*)(s_w + rmax);
+    const PinvLds L(smem, rmax);
     for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
         const int jb = list[li];
         const BlockJob& J = jobs[jb];
         const int r = J.r;
-        for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) s_a[idx] = J.W[idx];
+        for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) L.a[idx] = J.W[idx];
         int kept, reps, sweeps;
         double smax;
-        jacobi_pinv_t<kT, kE>(s_a, s_v, r, atol, rtol, J.U, r, J.sigma, s_flag, s_w, &kept, &smax, label_eps, &reps,
-                              &sweeps);
+        pinv_qr_lds<kT, kE>(L.a, L.b, r, atol, rtol, J.U, r, J.sigma, J.W, L.flag, L.w, L.phys, L.norm, L.tau,
+                            L.alpha, &kept, &smax, label_eps, &reps, &sweeps);
         if (threadIdx.x == 0) {
             out[jb].kept = kept;
             out[jb].sigma_max = smax;
@@ -337,8 +356,6 @@ __global__ void k_pinv_global(const BlockJob* __restrict__ jobs, const int32_t* 
         __syncthreads();
     }
 }
-
-size_t pinv_lds_bytes(int rmax) { return (size_t)32 * rmax * rmax + (size_t)8 * rmax + 16; }
 
 // (T, E) of the LDS variant for class size rmax on `threads` threads: T the
 // team width (at most 32), E the elements per lane.

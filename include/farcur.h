@@ -186,7 +186,7 @@ typedef struct {
     int max_rank;    /* cap on the sample count (0: min(m, n), the reference cap) */
     int block_id;    /* stream id of a single-block call */
     int heldout;     /* held-out rows per block in FARCUR_MODE_LEARNED (default 8) */
-    int oversample;  /* p added to the predicted rank in FARCUR_MODE_LEARNED (default 10) */
+    int oversample;  /* p added to the predicted rank in FARCUR_MODE_LEARNED (default 14) */
     double label_eps;/* singular values >= label_eps * sigma_max are counted as rank_eps */
     double heldout_safety; /* learned mode accepts a round when held-out error <= heldout_safety * tol (0.5) */
     double growth;   /* learned mode: sample count factor after a failed check (1.5; the plain loop doubles) */

@@ -95,3 +95,25 @@ def test_batch_profile_accounting(farcur, sphere):
     assert {"sample", "fill", "pinv", "residual", "pinv_r16", "pinv_global"} <= set(kt)
     assert kt["pinv"]["ms"] == pytest.approx(sum(v["ms"] for k, v in kt.items() if k.startswith("pinv_")))
     assert kt["fill"]["ms"] > 0 and kt["fill"]["work"] > 0 and kt["fill"]["launches"] >= 2
+
+
+@pytest.mark.parametrize("mode,precision", [(1, 1), (1, 0), (2, 1)])
+def test_batch_every_block_accurate(farcur, sphere, mode, precision):
+    # Every block of the batch (not a sample): a core pinv gone wrong anywhere
+    # in the launch shows up as a block whose CUR misses the dense block.
+    ctx, g, part, (ro, ri, co, cix) = sphere
+    bt = farcur.Batch(ctx, ro, ri, co, cix)
+    opts = dict(tol=1e-4, seed=17, precision=precision, heldout=8)
+    if precision == farcur.F32:
+        opts["pinv_rtol"] = 1e-6
+    bt.compress(mode, **opts)
+    info = bt.info_arrays()
+    if mode == farcur.MODE_LEARNED:
+        assert np.all(info["converged"] == 1)
+    # (the reference loop ends blocks that reach r = min(m, n) as "stopped at
+    # the cap"; their CUR is then exact up to the pinv threshold)
+    for b in range(bt.nblocks):
+        rows, cols = ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]]
+        out = bt.block(b)
+        e = _relfro(out["C"] @ out["U"] @ out["R"], g.fill(rows, cols))
+        assert e <= 1e-3, (b, e, info["r"][b], info["kept"][b])

@@ -95,6 +95,28 @@ def test_pinv_matches_lapack(plates, r):
     assert _relfro(U, Uref) <= 1e-10
 
 
+@pytest.mark.parametrize("r", [5, 16, 31, 48, 64])
+def test_pinv_graded_relative_threshold(plates, r):
+    # Singular values spread over 12 decades (the shape of far-field cores),
+    # none within a factor 3 of the relative threshold 1e-6: the kept set and
+    # the truncated pinv must match LAPACK's. r <= 64 runs the QR-preconditioned
+    # LDS path.
+    ctx, _, _ = plates
+    rng = np.random.default_rng(100 + r)
+    qa, _ = np.linalg.qr(rng.standard_normal((r, r)) + 1j * rng.standard_normal((r, r)))
+    qb, _ = np.linalg.qr(rng.standard_normal((r, r)) + 1j * rng.standard_normal((r, r)))
+    sv = 10.0 ** np.linspace(0, -12, r)
+    sv = sv[np.abs(np.log10(sv) + 6) > np.log10(3)]
+    sv = np.concatenate([sv, np.full(r - sv.size, 1e-13)])
+    W = 3.0 * (qa * sv) @ qb.conj().T
+    U, s, kept = ctx.pinv(W, 0.0, 1e-6)
+    Uref, sref, kref = _lapack_pinv(W, 0.0, 1e-6)
+    assert kept == kref
+    np.testing.assert_allclose(s[:kept], sref[:kept], rtol=1e-9)
+    assert _relfro(U, Uref) <= 1e-7
+    assert _relfro(W @ U @ W, W @ Uref @ W) <= 1e-8
+
+
 def test_pinv_ill_conditioned_core(plates, oracle):
     # A real intersection matrix of the config-1 block: rank-deficient far
     # field, absolute threshold 1e-10. The truncated pinv is checked where it
