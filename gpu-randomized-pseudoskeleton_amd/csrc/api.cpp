@@ -257,9 +257,12 @@ void build_records(const std::vector<double>& vertex, const std::vector<int32_t>
                    const std::vector<int32_t>& edges, const std::vector<double>& ln, std::vector<double>& rec,
                    std::vector<float>& rec32, std::vector<double>& cen) {
     const int ne = (int)ln.size();
-    rec.assign((size_t)kRecFields * ne, 0.0);
+    rec.assign((size_t)kRecStride * ne, 0.0);
+    rec32.assign((size_t)kRecStride * ne, 0.0f);
+    cen.assign((size_t)kCenStride * ne, 0.0);
     const double b[3][3] = {{2.0 / 3, 1.0 / 6, 1.0 / 6}, {1.0 / 6, 2.0 / 3, 1.0 / 6}, {1.0 / 6, 1.0 / 6, 2.0 / 3}};
     for (int e = 0; e < ne; ++e) {
+        double* R = &rec[(size_t)e * kRecStride];
         for (int h = 0; h < 2; ++h) {
             const int t = edges[4 * e + h];
             const int vf = edges[4 * e + 2 + h];
@@ -270,27 +273,24 @@ void build_records(const std::vector<double>& vertex, const std::vector<int32_t>
             for (int i = 0; i < 3; ++i) {
                 for (int c = 0; c < 3; ++c) {
                     const double q = b[i][0] * V0[c] + b[i][1] * V1[c] + b[i][2] * V2[c];
-                    rec[(size_t)(h * 9 + i * 3 + c) * ne + e] = q;
-                    rec[(size_t)(18 + h * 9 + i * 3 + c) * ne + e] = s * (q - vertex[3 * vf + c]);
+                    R[h * 9 + i * 3 + c] = q;
+                    R[18 + h * 9 + i * 3 + c] = s * (q - vertex[3 * vf + c]);
                 }
             }
         }
-        rec[(size_t)36 * ne + e] = ln[e];
-    }
-    // FP32 records: quadrature points as offsets from the RWG centre (mean of
-    // the six points), centre kept in FP64.
-    rec32.assign(rec.begin(), rec.end());
-    cen.assign(3 * (size_t)ne, 0.0);
-    for (int e = 0; e < ne; ++e) {
+        R[36] = ln[e];
+        // FP32 record: quadrature points as offsets from the RWG centre (mean of
+        // the six points), centre kept in FP64.
+        float* R32 = &rec32[(size_t)e * kRecStride];
+        for (int f = 18; f < kRecFields; ++f) R32[f] = (float)R[f];
         for (int c = 0; c < 3; ++c) {
             double m = 0;
             for (int h = 0; h < 2; ++h)
-                for (int i = 0; i < 3; ++i) m += rec[(size_t)(h * 9 + i * 3 + c) * ne + e];
+                for (int i = 0; i < 3; ++i) m += R[h * 9 + i * 3 + c];
             m /= 6.0;
-            cen[(size_t)c * ne + e] = m;
+            cen[(size_t)e * kCenStride + c] = m;
             for (int h = 0; h < 2; ++h)
-                for (int i = 0; i < 3; ++i)
-                    rec32[(size_t)(h * 9 + i * 3 + c) * ne + e] = (float)(rec[(size_t)(h * 9 + i * 3 + c) * ne + e] - m);
+                for (int i = 0; i < 3; ++i) R32[h * 9 + i * 3 + c] = (float)(R[h * 9 + i * 3 + c] - m);
         }
     }
 }

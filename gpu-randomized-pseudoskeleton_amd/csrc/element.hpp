@@ -17,15 +17,21 @@
 
 namespace farcur {
 
-// Per-RWG record, structure of arrays: field f of RWG e at rec[f * ne + e].
+// Per-RWG record, array of structures: field f of RWG e at
+// rec[e * kRecStride + f] (panel rows and columns are scattered RWG ids, so a
+// lane's whole record is a few contiguous cache lines read by vector loads).
 // FP64 records:
 //   f = h*9 + i*3 + c        quadrature point q_hi (xyz c) of triangle T+ (h=0) / T- (h=1)
 //   f = 18 + h*9 + i*3 + c   rho_hi = s_h (q_hi - v_h)
 //   f = 36                   edge length l
+//   f = 37..39               padding (zero)
 // FP32 records hold u_hi = q_hi - c_e (offsets from the RWG centre c_e, the
 // mean of its six quadrature points) in place of q_hi; c_e itself is kept in
-// FP64 (cen[c * ne + e]) so that centre-to-centre distances keep full accuracy.
+// FP64 (cen[e * kCenStride + c]) so that centre-to-centre distances keep full
+// accuracy.
 constexpr int kRecFields = 37;
+constexpr int kRecStride = 40;
+constexpr int kCenStride = 4;
 
 template <typename T>
 struct RwgRec {
@@ -35,17 +41,18 @@ struct RwgRec {
 };
 
 template <typename T>
-__device__ __forceinline__ void load_rec(const T* __restrict__ rec, int ne, int e, RwgRec<T>& r) {
+__device__ __forceinline__ void load_rec(const T* __restrict__ rec, int e, RwgRec<T>& r) {
+    const T* src = (const T*)__builtin_assume_aligned(rec + (size_t)e * kRecStride, 16);
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                r.q[h][i][c] = rec[(size_t)(h * 9 + i * 3 + c) * ne + e];
-                r.p[h][i][c] = rec[(size_t)(18 + h * 9 + i * 3 + c) * ne + e];
+                r.q[h][i][c] = src[h * 9 + i * 3 + c];
+                r.p[h][i][c] = src[18 + h * 9 + i * 3 + c];
             }
-    r.l = rec[(size_t)36 * ne + e];
+    r.l = src[36];
 }
 
 // ---------------------------------------------------------------- FP64
@@ -191,13 +198,13 @@ __device__ __forceinline__ void fill_tile(OutC* __restrict__ out, int ld, const 
     for (int idx = tid; idx < kTileCols * kRecFields; idx += kFillThreads) {
         const int c = idx / kRecFields, f = idx - c * kRecFields;
         const int j = lds0 + c;
-        lds.col[c][f] = (j < nlds) ? rec[(size_t)f * ne + lds_ids[j]] : T(0);
+        lds.col[c][f] = (j < nlds) ? rec[(size_t)lds_ids[j] * kRecStride + f] : T(0);
     }
     if constexpr (sizeof(T) == 4) {
         if (tid < kTileCols * 3) {
             const int c = tid / 3, x = tid - 3 * c;
             const int j = lds0 + c;
-            lds.cen[c][x] = (j < nlds) ? cen[(size_t)x * ne + lds_ids[j]] : 0.0;
+            lds.cen[c][x] = (j < nlds) ? cen[(size_t)lds_ids[j] * kCenStride + x] : 0.0;
         }
     }
     __syncthreads();
@@ -207,12 +214,12 @@ __device__ __forceinline__ void fill_tile(OutC* __restrict__ out, int ld, const 
     if (i >= nlane) return;
     const int e = lane_ids[i];
     RwgRec<T> a;
-    load_rec(rec, ne, e, a);
+    load_rec(rec, e, a);
     double ac3[3] = {0, 0, 0};
     if constexpr (sizeof(T) == 4) {
-        ac3[0] = cen[e];
-        ac3[1] = cen[(size_t)ne + e];
-        ac3[2] = cen[2 * (size_t)ne + e];
+        ac3[0] = cen[(size_t)e * kCenStride];
+        ac3[1] = cen[(size_t)e * kCenStride + 1];
+        ac3[2] = cen[(size_t)e * kCenStride + 2];
     }
 #pragma unroll 1
     for (int cc = 0; cc < kTileColsPerWave; ++cc) {

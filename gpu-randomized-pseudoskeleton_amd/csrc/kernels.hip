@@ -420,39 +420,71 @@ void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double at
 constexpr int kResidMaxR = 1024;
 
 // Relative Frobenius error of the CUR on the held-out rows:
-// ||H - C[hpos, :] U R||_F / ||H||_F, one held-out row at a time.
+// ||H - C[hpos, :] U R||_F / ||H||_F. The rows go in groups of up to
+// kHeldGroup (as many as sT / sY, `cap` complex values each, hold), so that U
+// and R are read once per group rather than once per row:
+//   T = C[hpos, :] (g x r), Y = T U, z_j = Y R[:, j] against H[j, :].
+constexpr int kHeldGroup = 8;
 template <typename PC>
-__device__ double heldout_error(const BlockJob& J, c128* t, c128* u, double* s_red) {
+__device__ double heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, double* s_red) {
     const PC* C = (const PC*)J.C;
     const PC* R = (const PC*)J.R;
     const PC* H = (const PC*)J.H;
-    const int m = J.m, n = J.n, r = J.r, tid = threadIdx.x, nt = blockDim.x;
+    const int m = J.m, n = J.n, r = J.r, h = J.h, tid = threadIdx.x, nt = blockDim.x;
+    const int G = max(1, min(kHeldGroup, cap / r));
     double num = 0, den = 0;
-    for (int i = 0; i < J.h; ++i) {
-        const int row = J.hpos[i];
-        for (int l = tid; l < r; l += nt) t[l] = to_c128(C[(size_t)l * m + row]);
-        __syncthreads();
-        for (int k = tid; k < r; k += nt) {  // u_k = sum_l t_l U[l, k]
-            const c128* uc = J.U + (size_t)k * r;
-            double ar = 0, ai = 0;
-            for (int l = 0; l < r; ++l) {
-                ar += t[l].re * uc[l].re - t[l].im * uc[l].im;
-                ai += t[l].re * uc[l].im + t[l].im * uc[l].re;
-            }
-            u[k] = c128{ar, ai};
+    for (int i0 = 0; i0 < h; i0 += G) {
+        const int g = min(G, h - i0);
+        for (int idx = tid; idx < g * r; idx += nt) {
+            const int i = idx / r, l = idx - i * r;
+            sT[idx] = to_c128(C[(size_t)l * m + J.hpos[i0 + i]]);
         }
         __syncthreads();
-        for (int j = tid; j < n; j += nt) {  // z_j = sum_k u_k R[k, j]
-            double zr = 0, zi = 0;
+        for (int k = tid; k < r; k += nt) {  // Y[i][k] = sum_l T[i][l] U[l][k]
+            const c128* uc = J.U + (size_t)k * r;
+            double ar[kHeldGroup], ai[kHeldGroup];
+#pragma unroll
+            for (int i = 0; i < kHeldGroup; ++i) ar[i] = ai[i] = 0.0;
+            for (int l = 0; l < r; ++l) {
+                const c128 u = uc[l];
+#pragma unroll
+                for (int i = 0; i < kHeldGroup; ++i) {
+                    if (i < g) {
+                        const c128 t = sT[i * r + l];
+                        ar[i] += t.re * u.re - t.im * u.im;
+                        ai[i] += t.re * u.im + t.im * u.re;
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kHeldGroup; ++i)
+                if (i < g) sY[i * r + k] = c128{ar[i], ai[i]};
+        }
+        __syncthreads();
+        for (int j = tid; j < n; j += nt) {  // z_i = sum_k Y[i][k] R[k, j]
+            double zr[kHeldGroup], zi[kHeldGroup];
+#pragma unroll
+            for (int i = 0; i < kHeldGroup; ++i) zr[i] = zi[i] = 0.0;
             for (int k = 0; k < r; ++k) {
                 const c128 b = to_c128(R[(size_t)j * r + k]);
-                zr += u[k].re * b.re - u[k].im * b.im;
-                zi += u[k].re * b.im + u[k].im * b.re;
+#pragma unroll
+                for (int i = 0; i < kHeldGroup; ++i) {
+                    if (i < g) {
+                        const c128 y = sY[i * r + k];
+                        zr[i] += y.re * b.re - y.im * b.im;
+                        zi[i] += y.re * b.im + y.im * b.re;
+                    }
+                }
             }
-            const c128 hv = to_c128(H[(size_t)j * J.h + i]);
-            const double dr = hv.re - zr, di = hv.im - zi;
-            num += dr * dr + di * di;
-            den += hv.re * hv.re + hv.im * hv.im;
+#pragma unroll
+            for (int i = 0; i < kHeldGroup; ++i) {
+                if (i < g) {
+                    const c128 hv = to_c128(H[(size_t)j * h + i0 + i]);
+                    const double dr = hv.re - zr[i], di = hv.im - zi[i];
+                    num += dr * dr + di * di;
+                    den += hv.re * hv.re + hv.im * hv.im;
+                }
+            }
         }
         __syncthreads();
     }
@@ -470,7 +502,10 @@ __global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, JobOut*
     for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
         const BlockJob& J = jobs[jb];
         if (J.h > 0) {
-            const double e = J.r <= kResidMaxR ? heldout_error<PC>(J, s_t1, s_t2, s_red) : heldout_error<PC>(J, J.W, J.W + J.r, s_red);
+            // Beyond kResidMaxR the group buffers use the W panel (r*r >= 2r),
+            // which the pinv has already consumed.
+            const double e = J.r <= kResidMaxR ? heldout_error<PC>(J, s_t1, s_t2, kResidMaxR, s_red)
+                                               : heldout_error<PC>(J, J.W, J.W + J.r, J.r, s_red);
             if (threadIdx.x == 0) out[jb].err = e;
             __syncthreads();
             continue;
