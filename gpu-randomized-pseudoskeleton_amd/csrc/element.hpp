@@ -162,13 +162,16 @@ __device__ __forceinline__ void entry_f32(const RwgRec<float>& a, const double* 
     zi = -kap * pr;
 }
 
-// Tile geometry of the panel fill: a workgroup of 4 waves covers 64 rows x
-// (4 * kTileColsPerWave) columns; lane = row, wave = group of columns. The
+// Tile geometry of the panel fill: a workgroup of 4 waves covers 64 rows x up
+// to 64 columns; lane = row, and the waves take the tile's columns round-robin
+// (wave w: columns w, w + 4, ...). Sampled panels have r of a few tens of
+// columns, so one tile spans all of them and the waves stay balanced (a 16-
+// column tile left whole waves idle on the last partial column group). The
 // column (source) records of the tile are staged in LDS and read as broadcasts.
 constexpr int kTileRows = 64;
-constexpr int kTileColsPerWave = 4;
-constexpr int kTileCols = 4 * kTileColsPerWave;
-constexpr int kFillThreads = 256;
+constexpr int kTileCols = 64;
+constexpr int kFillWaves = 4;
+constexpr int kFillThreads = kFillWaves * 64;
 
 template <typename T>
 struct FillTileLds {
@@ -177,7 +180,7 @@ struct FillTileLds {
 };
 
 // One tile of a panel out[i + j*ld] = Z(rows[i], cols[j]). Lanes run over
-// 64 consecutive entries of one index set (the lane set), the 16 entries of
+// 64 consecutive entries of one index set (the lane set), up to 64 entries of
 // the other set (the LDS set) are staged in LDS and read as broadcasts.
 // kLanesAreRows: lanes over rows (tall panels, C and W); otherwise lanes over
 // columns (wide panels R and H), using reciprocity Z(m, n) = Z(n, m) of the
@@ -194,17 +197,16 @@ __device__ __forceinline__ void fill_tile(OutC* __restrict__ out, int ld, const 
     const int lane0 = kLanesAreRows ? row0 : col0;
     const int lds0 = kLanesAreRows ? col0 : row0;
     const int tid = threadIdx.x;
+    const int ncol = min(kTileCols, nlds - lds0);
     __syncthreads();  // the LDS tile may still be read by a previous tile
-    for (int idx = tid; idx < kTileCols * kRecFields; idx += kFillThreads) {
+    for (int idx = tid; idx < ncol * kRecFields; idx += kFillThreads) {
         const int c = idx / kRecFields, f = idx - c * kRecFields;
-        const int j = lds0 + c;
-        lds.col[c][f] = (j < nlds) ? rec[(size_t)lds_ids[j] * kRecStride + f] : T(0);
+        lds.col[c][f] = rec[(size_t)lds_ids[lds0 + c] * kRecStride + f];
     }
     if constexpr (sizeof(T) == 4) {
-        if (tid < kTileCols * 3) {
-            const int c = tid / 3, x = tid - 3 * c;
-            const int j = lds0 + c;
-            lds.cen[c][x] = (j < nlds) ? cen[(size_t)lds_ids[j] * kCenStride + x] : 0.0;
+        for (int idx = tid; idx < ncol * 3; idx += kFillThreads) {
+            const int c = idx / 3, x = idx - 3 * c;
+            lds.cen[c][x] = cen[(size_t)lds_ids[lds0 + c] * kCenStride + x];
         }
     }
     __syncthreads();
@@ -222,10 +224,8 @@ __device__ __forceinline__ void fill_tile(OutC* __restrict__ out, int ld, const 
         ac3[2] = cen[(size_t)e * kCenStride + 2];
     }
 #pragma unroll 1
-    for (int cc = 0; cc < kTileColsPerWave; ++cc) {
-        const int c = w * kTileColsPerWave + cc;
+    for (int c = w; c < ncol; c += kFillWaves) {
         const int j = lds0 + c;
-        if (j >= nlds) break;
         T zr, zi;
         if constexpr (sizeof(T) == 8)
             entry_f64(a, lds.col[c], fc.k, fc.c4k2, fc.scale, zr, zi);
