@@ -232,6 +232,10 @@ struct farcur_ctx {
     int last_m = 0, last_n = 0, last_r = 0, last_f32 = 0, last_y = 0;
     bool has_result = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Second stream of the batched rounds' two-stream pipeline (default
+    // priority, like the first) and the events joining the two.
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     Tuning tune;
 
     int select() {
@@ -391,8 +395,7 @@ struct RoundPlan {
     }
 };
 
-void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt) {
-    hipStream_t s = ctx->stream;
+void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt, hipStream_t s) {
     const FillParams fp = ctx->fill_params(P.f32);
     hipEvent_t a;
     kt.begin(0, s, &a);
@@ -653,8 +656,11 @@ int farcur_create(int device, farcur_ctx** out) {
     ctx->device = device;
     HIPTRY(hipSetDevice(device));
     HIPTRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIPTRY(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
     HIPTRY(hipEventCreate(&ctx->ev0));
     HIPTRY(hipEventCreate(&ctx->ev1));
+    HIPTRY(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    HIPTRY(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
     *out = ctx.release();
     return FARCUR_OK;
 }
@@ -667,8 +673,12 @@ void farcur_destroy(farcur_ctx* ctx) {
                     &ctx->d_C,   &ctx->d_R,   &ctx->d_W,    &ctx->d_V,    &ctx->d_U,     &ctx->d_sigma,
                     &ctx->d_v,   &ctx->d_y0,  &ctx->d_y1,   &ctx->d_plan, &ctx->d_out};
     for (DBuf* b : bufs) b->release();
+    if (ctx->stream2) hipStreamSynchronize(ctx->stream2);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
+    if (ctx->stream2) hipStreamDestroy(ctx->stream2);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -966,7 +976,7 @@ int block_loop(farcur_ctx* ctx, const BlockLoopIO& io, const farcur_opts& o, All
         if (r > kPinvLdsMaxR) P.pinv_threads[kPinvClasses - 1] = r > 128 ? 1024 : 256;
         if (big) P.resid_threads = 1024;
         HIPTRY(hipEventRecord(ctx->ev0, ctx->stream));
-        run_round(ctx, P, kt);
+        run_round(ctx, P, kt, ctx->stream);
         HIPTRY(hipEventRecord(ctx->ev1, ctx->stream));
         HIPTRY(hipGetLastError());
         HIPTRY(hipMemcpyAsync(&res, P.out, sizeof res, hipMemcpyDeviceToHost, ctx->stream));
@@ -1209,7 +1219,7 @@ int compat_impl(farcur_c128* Cc, farcur_c128* Uc, farcur_c128* Rc, int r, double
     if ((size_t)m * r > (size_t)256 * 1024 || r > 128) P.resid_threads = 1024;
     KTimer kt;
     HIPTRY(hipEventRecord(ctx->ev0, s));
-    run_round(ctx, P, kt);
+    run_round(ctx, P, kt, ctx->stream);
     HIPTRY(hipEventRecord(ctx->ev1, s));
     HIPTRY(hipGetLastError());
     JobOut res{};
@@ -1343,6 +1353,30 @@ int distinct_triangles(const std::vector<int32_t>& tri_of_rwg, const int32_t* id
 
 }  // namespace
 
+// A captured round: the plan upload (from pinned host memory), the round's
+// kernels and the JobOut download, replayed as one hipGraph launch. The entry
+// owns the device plan the kernels read and the pinned staging buffers the
+// memcpy nodes name, so nothing it references is freed or moved while cached.
+struct RoundGraph {
+    uint64_t key = 0;
+    hipGraphExec_t exec = nullptr;
+    DBuf d_plan;
+    char* h_plan = nullptr;
+    JobOut* h_out = nullptr;
+    size_t plan_bytes = 0;
+    int njobs = 0;
+    uint64_t last_use = 0;
+    void release() {
+        if (exec) hipGraphExecDestroy(exec);
+        exec = nullptr;
+        d_plan.release();
+        if (h_plan) hipHostFree(h_plan);
+        if (h_out) hipHostFree(h_out);
+        h_plan = nullptr;
+        h_out = nullptr;
+    }
+};
+
 struct farcur_batch {
     farcur_ctx* ctx = nullptr;
     int nb = 0;
@@ -1363,6 +1397,16 @@ struct farcur_batch {
     farcur_model* model = nullptr;
     bool learned = false;
     int first_round_misses = 0;
+    // Two-stream pipeline of the batched rounds: number of chunks (1 = off).
+    int pipeline_chunks = 1;
+    std::vector<char> h_plan;
+    // Rounds captured as hipGraphs, keyed by plan content and launch
+    // parameters (at most kMaxGraphs, least recently used evicted).
+    static constexpr int kMaxGraphs = 8;
+    bool use_graphs = true;
+    std::vector<RoundGraph> graphs;
+    uint64_t graph_clock = 0;
+    int graph_captures = 0, graph_replays = 0;
     // Interpolation matrices P_b = C_b U_b for the product (arena memory).
     std::vector<void*> interp;
     bool interp_ready = false;
@@ -1503,35 +1547,173 @@ void advance(BState& s, const JobOut& out, const RoundBufs& rb, double tol) {
     s.r = rn;
 }
 
+// Cost estimate of a job for balancing pipeline chunks: fill entries (36 Green
+// evaluations each) plus the pinv (~ r^3 x sweeps, in the same units).
+double job_cost(const BlockJob& J) {
+    const double entries = (double)(J.m + J.n + J.r) * J.r + (double)J.h * J.n;
+    return 36.0 * entries + 60.0 * (double)J.r * J.r * J.r;
+}
+
+// One round over the active blocks. With pipelining, the jobs are cut into
+// chunks of equal estimated cost that alternate between the context's two
+// streams, so that one chunk's fill (VALU-bound) runs beside another's pinv
+// (latency-bound); each chunk is its own launch sequence with its own plan.
+// Profiling keeps one chunk on one stream, so that every kernel class is timed
+// alone.
 int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_opts& o, std::vector<RoundBufs>& bufs,
                 std::vector<JobOut>& outs) {
     farcur_ctx* ctx = B->ctx;
     std::vector<BlockJob> jobs;
     jobs.reserve(active.size());
     bufs.resize(active.size());
+    double total_cost = 0;
     for (size_t i = 0; i < active.size(); ++i) {
         const int b = active[i];
         TRY(batch_alloc_round(B, B->st[b].r, B->st[b].m, B->st[b].n, bufs[i]));
         if (B->learned) TRY(batch_alloc_heldout(B, B->st[b].h, B->st[b].n, bufs[i]));
         jobs.push_back(make_job(B, b, bufs[i], o));
+        total_cost += job_cost(jobs.back());
     }
-    PackedRound pk;
-    pack_round(jobs, o.row_mode == FARCUR_ROWS_PIVOT, pk);
-    TRY(B->d_plan.ensure(pk.bytes.size()));
-    HIPTRY(hipMemcpyAsync(B->d_plan.p, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
-    RoundPlan P = plan_from(pk, B->d_plan.p, (int)jobs.size());
-    P.f32 = B->f32;
-    P.pivot = o.row_mode == FARCUR_ROWS_PIVOT;
-    P.atol = o.pinv_tol;
-    P.rtol = o.pinv_rtol;
-    P.seed = o.seed;
-    P.label_eps = o.label_eps;
-    P.tune(ctx->tune);
-    run_round(ctx, P, B->kt);
-    HIPTRY(hipGetLastError());
-    outs.resize(jobs.size());
-    HIPTRY(hipMemcpyAsync(outs.data(), P.out, jobs.size() * sizeof(JobOut), hipMemcpyDeviceToHost, ctx->stream));
-    HIPTRY(hipStreamSynchronize(ctx->stream));
+    const int nj = (int)jobs.size();
+    int nchunks = 1;
+    if (!B->profile && B->pipeline_chunks > 1 && nj >= 64 * B->pipeline_chunks) nchunks = B->pipeline_chunks;
+    // chunk boundaries at equal cost
+    std::vector<int> cut{0};
+    {
+        double acc = 0;
+        int c = 1;
+        for (int j = 0; j < nj && c < nchunks; ++j) {
+            acc += job_cost(jobs[j]);
+            if (acc >= total_cost * c / nchunks) {
+                cut.push_back(j + 1);
+                ++c;
+            }
+        }
+        if (cut.back() != nj) cut.push_back(nj);
+    }
+    nchunks = (int)cut.size() - 1;
+    std::vector<PackedRound> pk(nchunks);
+    std::vector<size_t> off(nchunks + 1, 0);
+    for (int c = 0; c < nchunks; ++c) {
+        std::vector<BlockJob> part(jobs.begin() + cut[c], jobs.begin() + cut[c + 1]);
+        pack_round(part, o.row_mode == FARCUR_ROWS_PIVOT, pk[c]);
+        off[c + 1] = off[c] + ((pk[c].bytes.size() + 255) & ~(size_t)255);
+    }
+    const size_t plan_bytes = off[nchunks];
+    B->h_plan.resize(plan_bytes);
+    for (int c = 0; c < nchunks; ++c) std::memcpy(B->h_plan.data() + off[c], pk[c].bytes.data(), pk[c].bytes.size());
+    hipStream_t s0 = ctx->stream, s1 = ctx->stream2;
+    // Launch parameters that the kernels take by value.
+    struct KeyParams {
+        int f32, pivot, nchunks, nj;
+        double atol, rtol, label_eps, k, eta;
+        uint64_t seed, geom;
+        Tuning tune;
+    } kp{};
+    kp.f32 = B->f32;
+    kp.pivot = o.row_mode == FARCUR_ROWS_PIVOT;
+    kp.nchunks = nchunks;
+    kp.nj = nj;
+    kp.atol = o.pinv_tol;
+    kp.rtol = o.pinv_rtol;
+    kp.label_eps = o.label_eps;
+    kp.k = ctx->k;
+    kp.eta = ctx->eta;
+    kp.seed = o.seed;
+    kp.geom = ctx->geom_hash;
+    kp.tune = ctx->tune;
+    // Enqueues the round: plan upload from `hplan` to `dplan`, kernels, JobOut
+    // download to `hout`.
+    std::vector<RoundPlan> plans(nchunks);
+    auto enqueue = [&](const void* hplan, void* dplan, JobOut* hout) -> int {
+        HIPTRY(hipMemcpyAsync(dplan, hplan, plan_bytes, hipMemcpyHostToDevice, s0));
+        if (nchunks > 1) {
+            HIPTRY(hipEventRecord(ctx->ev_fork, s0));
+            HIPTRY(hipStreamWaitEvent(s1, ctx->ev_fork, 0));
+        }
+        for (int c = 0; c < nchunks; ++c) {
+            RoundPlan& P = plans[c];
+            P = plan_from(pk[c], (char*)dplan + off[c], cut[c + 1] - cut[c]);
+            P.f32 = B->f32;
+            P.pivot = o.row_mode == FARCUR_ROWS_PIVOT;
+            P.atol = o.pinv_tol;
+            P.rtol = o.pinv_rtol;
+            P.seed = o.seed;
+            P.label_eps = o.label_eps;
+            P.tune(ctx->tune);
+            run_round(ctx, P, B->kt, (c & 1) ? s1 : s0);
+        }
+        HIPTRY(hipGetLastError());
+        if (nchunks > 1) {
+            HIPTRY(hipEventRecord(ctx->ev_join, s1));
+            HIPTRY(hipStreamWaitEvent(s0, ctx->ev_join, 0));
+        }
+        for (int c = 0; c < nchunks; ++c)
+            HIPTRY(hipMemcpyAsync(hout + cut[c], plans[c].out, (size_t)(cut[c + 1] - cut[c]) * sizeof(JobOut),
+                                  hipMemcpyDeviceToHost, s0));
+        return FARCUR_OK;
+    };
+    outs.resize(nj);
+    if (!B->use_graphs || B->profile) {
+        TRY(B->d_plan.ensure(plan_bytes));
+        TRY(enqueue(B->h_plan.data(), B->d_plan.p, outs.data()));
+        HIPTRY(hipStreamSynchronize(s0));
+    } else {
+        // The key covers the launch parameters and the plan size; the plan
+        // itself (megabytes for a full round) is compared exactly below.
+        const uint64_t key = fnv1a(0xcbf29ce484222325ull, &kp, sizeof kp) ^ (uint64_t)plan_bytes;
+        RoundGraph* g = nullptr;
+        for (RoundGraph& e : B->graphs)
+            if (e.exec && e.key == key && e.plan_bytes == plan_bytes && e.njobs == nj &&
+                std::memcmp(e.h_plan, B->h_plan.data(), plan_bytes) == 0)
+                g = &e;
+        if (g) {
+            ++B->graph_replays;
+        } else {
+            if ((int)B->graphs.size() < farcur_batch::kMaxGraphs) {
+                B->graphs.emplace_back();
+                g = &B->graphs.back();
+            } else {
+                g = &B->graphs[0];
+                for (RoundGraph& e : B->graphs)
+                    if (e.last_use < g->last_use) g = &e;
+                HIPTRY(hipStreamSynchronize(s0));
+                g->release();
+            }
+            g->key = key;
+            g->plan_bytes = plan_bytes;
+            g->njobs = nj;
+            TRY(g->d_plan.ensure(plan_bytes));
+            HIPTRY(hipHostMalloc((void**)&g->h_plan, plan_bytes, hipHostMallocDefault));
+            HIPTRY(hipHostMalloc((void**)&g->h_out, std::max<size_t>(nj, 1) * sizeof(JobOut), hipHostMallocDefault));
+            std::memcpy(g->h_plan, B->h_plan.data(), plan_bytes);
+            hipGraph_t graph = nullptr;
+            HIPTRY(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue(g->h_plan, g->d_plan.p, g->h_out);
+            const hipError_t ec = hipStreamEndCapture(s0, &graph);
+            if (rc != FARCUR_OK) {
+                if (graph) hipGraphDestroy(graph);
+                g->release();
+                return rc;
+            }
+            if (ec != hipSuccess) {
+                g->release();
+                return fail(FARCUR_EHIP, "graph capture of a round failed: %s", hipGetErrorString(ec));
+            }
+            const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (ei != hipSuccess) {
+                g->exec = nullptr;
+                g->release();
+                return fail(FARCUR_EHIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+            }
+            ++B->graph_captures;
+        }
+        g->last_use = ++B->graph_clock;
+        HIPTRY(hipGraphLaunch(g->exec, s0));
+        HIPTRY(hipStreamSynchronize(s0));
+        std::memcpy(outs.data(), g->h_out, (size_t)nj * sizeof(JobOut));
+    }
     B->kt.collect();
     if (B->profile) B->rounds.push_back(std::move(jobs));
     return FARCUR_OK;
@@ -1623,6 +1805,7 @@ extern "C" void farcur_batch_destroy(farcur_batch* B) {
     DBuf* bufs[] = {&B->d_rows, &B->d_cols, &B->d_v,    &B->d_ya,  &B->d_yb,
                     &B->d_vo,   &B->d_plan, &B->d_feat, &B->d_cap, &B->d_rpred};
     for (DBuf* b : bufs) b->release();
+    for (RoundGraph& g : B->graphs) g.release();
     B->arena.release();
     delete B;
 }
@@ -2175,6 +2358,31 @@ extern "C" int farcur_batch_set_model(farcur_batch* B, farcur_model* M) {
 }
 
 extern "C" int farcur_batch_misses(farcur_batch* B) { return B ? B->first_round_misses : -1; }
+
+extern "C" int farcur_batch_set_graphs(farcur_batch* B, int enable) {
+    if (!B) return fail(FARCUR_EINVAL, "NULL batch");
+    TRY(B->ctx->select());
+    HIPTRY(hipStreamSynchronize(B->ctx->stream));
+    if (!enable)
+        for (RoundGraph& g : B->graphs) g.release();
+    if (!enable) B->graphs.clear();
+    B->use_graphs = enable != 0;
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_batch_graph_stats(farcur_batch* B, int* captures, int* replays) {
+    if (!B) return fail(FARCUR_EINVAL, "NULL batch");
+    if (captures) *captures = B->graph_captures;
+    if (replays) *replays = B->graph_replays;
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_batch_set_pipeline(farcur_batch* B, int chunks) {
+    if (!B) return fail(FARCUR_EINVAL, "NULL batch");
+    if (chunks < 0 || chunks > 64) return fail(FARCUR_EINVAL, "chunks must be in [0, 64]");
+    B->pipeline_chunks = chunks < 1 ? 1 : chunks;
+    return FARCUR_OK;
+}
 
 // ------------------------------------------------------------------ multi-RHS product
 namespace {

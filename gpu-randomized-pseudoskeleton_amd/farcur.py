@@ -39,8 +39,12 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise FarcurError(f"{LIB_PATH} is not built (run __graft_entry__.build() or make -C {HERE})")
-        _lib = ctypes.CDLL(str(LIB_PATH))
-        _setup(_lib)
+        L = ctypes.CDLL(str(LIB_PATH))
+        try:
+            _setup(L)
+        except AttributeError as e:  # a stale build missing an entry point
+            raise FarcurError(f"{LIB_PATH} does not export {e}; rebuild it (make -C {HERE})") from None
+        _lib = L  # only a fully configured library is kept
     return _lib
 
 
@@ -148,6 +152,9 @@ def _setup(L: ctypes.CDLL) -> None:
         L.farcur_batch_features.argtypes = [vp, vp]
         L.farcur_batch_set_model.argtypes = [vp, vp]
         L.farcur_batch_misses.argtypes = [vp]
+        L.farcur_batch_set_pipeline.argtypes = [vp, ci]
+        L.farcur_batch_set_graphs.argtypes = [vp, ci]
+        L.farcur_batch_graph_stats.argtypes = [vp, vp, vp]
         L.farcur_batch_matvec.argtypes = [vp, vp, ci, vp, ci, ci, vp]
     L.farcur_model_create.argtypes = [vp, cu64, vp]
     L.farcur_model_destroy.argtypes = [vp]
@@ -503,6 +510,20 @@ class Batch:
 
     def misses(self) -> int:
         return lib().farcur_batch_misses(self._h)
+
+    def set_pipeline(self, chunks: int):
+        """Two-stream pipeline of the batched rounds (chunks <= 1: off)."""
+        check(lib().farcur_batch_set_pipeline(self._h, chunks))
+
+    def set_graphs(self, enable: bool):
+        """hipGraph capture / replay of the batched rounds (on by default)."""
+        check(lib().farcur_batch_set_graphs(self._h, 1 if enable else 0))
+
+    def graph_stats(self):
+        """(captures, replays) of round graphs since the batch was created."""
+        c, r = ci(), ci()
+        check(lib().farcur_batch_graph_stats(self._h, ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
 
     def profile(self, enable=True):
         check(lib().farcur_batch_profile(self._h, 1 if enable else 0))

@@ -276,6 +276,26 @@ int farcur_batch_set_model(farcur_batch* batch, farcur_model* model);
  * compress (the miss count; not an error status). */
 int farcur_batch_misses(farcur_batch* batch);
 
+/* Two-stream pipeline of the batched rounds (no counterpart in the reference,
+ * whose loop is per block): a round's jobs are cut into `chunks` parts of equal
+ * estimated cost that alternate between two HIP streams, so that one part's
+ * fill overlaps another's core pinv. 0 or 1 turns it off (default). Results
+ * do not depend on it. Profiling (farcur_batch_profile) runs one part.
+ * Measured on the 30,720-RWG sphere it does not pay (each phase already
+ * fills the GPU), so it is off by default. */
+int farcur_batch_set_pipeline(farcur_batch* batch, int chunks);
+
+/* hipGraph capture of the batched rounds (no counterpart in the reference):
+ * each round (plan upload, the persistent sample / fill / pinv / residual
+ * launches, result download) is captured once and replayed as one graph
+ * launch whenever a later round has the same plan content and launch
+ * parameters (for instance every round of a repeated compress of the same
+ * batch). Up to 8 graphs are kept per batch. On by default; profiling runs
+ * eagerly. set_graphs(0) drops the cached graphs. graph_stats reports the
+ * captures and replays since the batch was created. */
+int farcur_batch_set_graphs(farcur_batch* batch, int enable);
+int farcur_batch_graph_stats(farcur_batch* batch, int* captures, int* replays);
+
 /* Multi-RHS product of the compressed far field: Y = sum_b C_b U_b R_b V[cols_b]
  * added to rows_b, and with symmetric = 1 also (C_b U_b R_b)^T V[rows_b] added
  * to cols_b (the transposed block of a symmetric operator, which the

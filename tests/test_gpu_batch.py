@@ -117,3 +117,53 @@ def test_batch_every_block_accurate(farcur, sphere, mode, precision):
         out = bt.block(b)
         e = _relfro(out["C"] @ out["U"] @ out["R"], g.fill(rows, cols))
         assert e <= 1e-3, (b, e, info["r"][b], info["kept"][b])
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_pipeline_chunks_do_not_change_results(farcur, sphere, mode):
+    # The two-stream chunked rounds run the same per-block work: identical
+    # sample counts, residuals and panels with the pipeline on and off.
+    ctx, g, part, (ro, ri, co, cix) = sphere
+    assert len(ro) - 1 >= 4 * 64  # enough jobs for 4 chunks
+    out = []
+    for chunks in (1, 4):
+        bt = farcur.Batch(ctx, ro, ri, co, cix)
+        bt.set_pipeline(chunks)
+        bt.compress(mode, tol=1e-4, seed=23, precision=farcur.F32, pinv_rtol=1e-6, heldout=8)
+        out.append((bt.info_arrays(), [bt.block(b) for b in range(0, bt.nblocks, 17)]))
+    (i1, b1), (i2, b2) = out
+    for k in ("r", "rounds", "kept", "converged", "err"):
+        np.testing.assert_array_equal(i1[k], i2[k])
+    for x, y in zip(b1, b2):
+        for k in ("C", "U", "R", "rs", "cs"):
+            np.testing.assert_array_equal(x[k], y[k])
+
+
+def test_round_graphs_replay_matches_eager(farcur, sphere):
+    # Rounds captured as hipGraphs and replayed on later compresses give the
+    # eager launches' results, every replay, with eager work in between.
+    ctx, g, part, (ro, ri, co, cix) = sphere
+    opts = dict(tol=1e-4, seed=29, precision=farcur.F32, pinv_rtol=1e-6, heldout=8)
+    eager = farcur.Batch(ctx, ro, ri, co, cix)
+    eager.set_graphs(False)
+    eager.compress(farcur.MODE_LEARNED, **opts)
+    ie = eager.info_arrays()
+    pick = list(range(0, eager.nblocks, 13))
+    be = [eager.block(b) for b in pick]
+    assert eager.graph_stats() == (0, 0)
+    gb = farcur.Batch(ctx, ro, ri, co, cix)
+    rounds = int(ie["rounds"].max())
+    for rep in range(3):
+        gb.compress(farcur.MODE_LEARNED, **opts)
+        ig = gb.info_arrays()
+        for k in ("r", "rounds", "kept", "converged", "err", "heldout_err"):
+            np.testing.assert_array_equal(ig[k], ie[k])
+        for b, x in zip(pick, be):
+            y = gb.block(b)
+            for k in ("C", "U", "R", "rs", "cs"):
+                np.testing.assert_array_equal(x[k], y[k])
+        # eager work of the program between replays
+        eager.compress(farcur.MODE_BATCHED, tol=1e-4, seed=3)
+        ctx.fill(ri[ro[0]:ro[1]], cix[co[0]:co[1]])
+    captures, replays = gb.graph_stats()
+    assert captures == rounds and replays == 2 * rounds
