@@ -32,28 +32,56 @@ PEAK_TRANSC_PER_S = 256 * 16 * 2.4e9
 FLOPS_PER_EVAL = 22.0  # algorithmic FP32 flops of one Green-function evaluation
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_sphere_learned.json"
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    --pmc summary (FETCH_SIZE scaled by the plain-copy calibration, plus
+    WRITE_SIZE), or None. The numbers come from a separate profiling run of
+    this file; see that summary for its command."""
+    try:
+        d = json.loads(TRAFFIC_FILE.read_text())
+        k = d["kernels"][name]
+        return {"bytes_per_launch": k["bytes_per_launch"], "fetch_bytes_per_launch": k["fetch_bytes_per_launch"],
+                "write_bytes_per_launch": k["write_bytes_per_launch"], "source": str(TRAFFIC_FILE.relative_to(ROOT))}
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def roofline(kt):
-    """Roofline of the kernel class with the largest device time."""
-    main = {k: v for k, v in kt.items() if k in ("fill", "pinv", "residual")}
+    """Roofline of the kernel class with the largest device time in the
+    profiled step (pinv counted per r class, each class being its own launch).
+    achieved = algorithmic work / device time, both summed over the class's
+    launches of the step (= per-launch work / average launch duration)."""
+    main = {k: v for k, v in kt.items() if k in ("fill", "residual") or (k.startswith("pinv_") and v["launches"])}
     name = max(main, key=lambda k: main[k]["ms"])
     v = main[name]
     sec = v["ms"] * 1e-3
+    common = {"kernel": name, "launches": v["launches"], "avg_launch_ms": v["ms"] / max(v["launches"], 1)}
+    traffic = measured_traffic(name)
     if name == "fill":
         achieved = v["work"] * FLOPS_PER_EVAL / sec / 1e12
-        return {"kernel": "fill", "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VALU_TFLOPS,
+        return {**common, "bound": "mfma", "compute_unit": "VALU FP32 + transcendental (no MFMA in the fill)",
+                "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_VALU_TFLOPS,
                 "work_unit": "Green evaluations x 22 flops (9 x distinct triangle pairs per panel)",
                 "evals_per_s": v["work"] / sec,
                 "evals_frac_of_transcendental_ceiling": v["work"] / sec / (PEAK_TRANSC_PER_S / 3.0),
-                "traffic": None}
-    if name == "pinv":
+                "traffic": traffic["bytes_per_launch"] if traffic else None, "traffic_detail": traffic,
+                "note": "compute roof: MI355X publishes 157.3 TFLOP/s for FP32 (vector and matrix); packed FP32 "
+                        "is disabled in this build, so the VALU tops out at half of it"}
+    if name.startswith("pinv"):
         achieved = v["work"] / sec / 1e12
-        return {"kernel": "pinv", "bound": "valu", "achieved": achieved, "peak": PEAK_FP64_VALU_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VALU_TFLOPS,
-                "work_unit": "96 r^3 FP64 flops per r x r complex SVD-pinv", "traffic": None}
+        return {**common, "bound": "mfma", "compute_unit": "VALU FP64 (no MFMA in the pinv)", "achieved": achieved,
+                "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VALU_TFLOPS,
+                "work_unit": "96 r^3 FP64 flops per r x r complex SVD-pinv",
+                "traffic": traffic["bytes_per_launch"] if traffic else None, "traffic_detail": traffic,
+                "note": "compute roof: 78.6 TFLOP/s FP64 (vector and matrix)"}
     achieved = v["work"] / sec / 1e9
-    return {"kernel": "residual", "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS, "work_unit": "panel + vector bytes read", "traffic": None}
+    return {**common, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, "work_unit": "panel + vector bytes read",
+            "traffic": traffic["bytes_per_launch"] if traffic else None, "traffic_detail": traffic}
 
 
 def build_workload(fc, level=5, radius=2.0, jitter=0.01, seed=1, leaf=0.5, eta_adm=2.0):
@@ -212,7 +240,25 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the per-block plain-loop comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--check-blocks", type=int, default=64, help="blocks checked against FP64 dense")
+    ap.add_argument("--no-product", action="store_true", help="skip the multi-RHS product timing")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched plain-loop comparison")
+    ap.add_argument("--tuning", default=None,
+                    help="pinv threads per r class, e.g. 64,256,256,256,512 (skips the autotuner's probes, "
+                         "for profiler runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 256 MiB device-to-device copy three times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device copy", "bytes_read": a.numel() * 4, "bytes_written": a.numel() * 4,
+                          "copies": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(far-field blocks shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
@@ -231,8 +277,13 @@ def main():
     n = np.diff(co)
     represented = float(np.sum(m.astype(np.float64) * n))
     ctx = fc.Context(0)
-    tuning = ctx.autotune()
-    log(f"# autotune: {tuning}")
+    if args.tuning:
+        ctx.set_tuning(pinv_threads=[int(x) for x in args.tuning.split(",")])
+        tuning = ctx.tuning()
+        log(f"# tuning (given): {tuning}")
+    else:
+        tuning = ctx.autotune()
+        log(f"# autotune: {tuning}")
     model, model_src = rank_model(fc, ctx, log)
     ctx.set_mesh(mesh)
     prec = fc.F32 if args.precision == "f32" else fc.F64
@@ -265,14 +316,19 @@ def main():
     log(f"# learned accuracy vs FP64 dense ({args.check_blocks} blocks): max {err_max:.2e} median {err_med:.2e}")
 
     # config 3: the compressed operator times 256 plane waves (theta_j = j pi / 255, phi = 0)
-    prod = product_bench(fc, learned, mesh, blocks, np.arange(256) * np.pi / 255, np.zeros(256), log)
+    prod = None
+    if not args.no_product:
+        prod = product_bench(fc, learned, mesh, blocks, np.arange(256) * np.pi / 255, np.zeros(256), log)
 
     # the same without the network: batched reference loop
+    batched_res = None
     batched = fc.Batch(ctx, ro, ri, co, cix)
-    batched.compress(fc.MODE_BATCHED, **opts)
-    bw = float(np.mean([batched.compress(fc.MODE_BATCHED, **opts) for _ in range(2)]))
-    bmax, bmed = accuracy(fc, ctx, batched, blocks, args.check_blocks)
-    log(f"# batched plain loop: {bw:.2f} ms/step, accuracy max {bmax:.2e} median {bmed:.2e}")
+    if not args.no_batched:
+        batched.compress(fc.MODE_BATCHED, **opts)
+        bw = float(np.mean([batched.compress(fc.MODE_BATCHED, **opts) for _ in range(2)]))
+        bmax, bmed = accuracy(fc, ctx, batched, blocks, args.check_blocks)
+        log(f"# batched plain loop: {bw:.2f} ms/step, accuracy max {bmax:.2e} median {bmed:.2e}")
+        batched_res = {"blocks_per_s": nb / (bw * 1e-3), "ms": bw, "max_block_relerr_sampled": bmax}
 
     plain = None
     if not args.no_plain:
@@ -302,12 +358,14 @@ def main():
         "represented_entries_per_s": represented / (ms * 1e-3),
         "filled_entries_per_s": filled / (ms * 1e-3),
         "miss_rate": misses / nb,
+        "pinv_sweeps_mean": float(np.mean(il["sweeps"])),
+        "round_graphs_captured_replayed": learned.graph_stats(),
         "max_block_relerr_sampled": err_max,
         "median_block_relerr_sampled": err_med,
-        "batched_plain": {"blocks_per_s": nb / (bw * 1e-3), "ms": bw, "max_block_relerr_sampled": bmax},
+        "batched_plain": batched_res,
         "plain_per_block": plain,
         "speedup_vs_plain_per_block": value / plain["blocks_per_s"] if plain else None,
-        "speedup_vs_batched_plain": value / (nb / (bw * 1e-3)),
+        "speedup_vs_batched_plain": value / batched_res["blocks_per_s"] if batched_res else None,
         "profiled_step_ms": prof_wall,
         "product_256": prod,
         "roofline": roofline(kt),

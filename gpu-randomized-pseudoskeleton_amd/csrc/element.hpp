@@ -103,8 +103,16 @@ __device__ __forceinline__ void entry_f64(const RwgRec<double>& a, const double*
 //   g - 1/R0 = [(cos k delta - 1) - j sin k delta] / R - delta / (R R0)
 // instead of g: the near-cancelling second difference of the scalar potential
 // is then formed from small quantities only. delta = X / (R + R0) with
-// X = 2 d.D + |D|^2 (d = c_m - c_n, D = u_a - u_b) is exact to rounding.
-// Sines take revolutions (hardware sine), arguments |delta / lambda| < 1.
+// X = R^2 - R0^2 = 2 d.D + |D|^2 (d = c_m - c_n, D = u_a - u_b) is exact to
+// rounding. Sines take revolutions (hardware sine), arguments |delta / lambda| < 1.
+//
+// The point-pair loop is VALU-issue bound, so everything that depends on one
+// point only is hoisted to the entry: in half-wavelength units (ihl = 1/(2 lambda)),
+//   X ihl = alpha_i + beta_j - 2 u_ai . u'_bj,
+//   alpha_i = (2 d.u_ai + |u_ai|^2) ihl,  beta_j = (|u_bj|^2 - 2 d.u_bj) ihl,  u'_bj = u_bj ihl,
+// and with th = delta / (2 lambda) (so k delta = 2 pi (2 th) in revolutions):
+//   cos(k delta) / R = (1 - 2 sin^2 th) / R,   sin(k delta) / R,
+//   -Re(g - 1/R0) = 2 sin^2 th / R + th (2 lambda / R0) / R.
 struct F32Consts {
     float inv_lambda;  // k / (2 pi)
     float c4k2;        // 4 / k^2
@@ -121,13 +129,32 @@ __device__ __forceinline__ void entry_f32(const RwgRec<float>& a, const double* 
     const float f0 = (float)(ph - rint(ph));
     const float s0 = __builtin_amdgcn_sinf(f0), c0 = __builtin_amdgcn_cosf(f0);
     const float dx = (float)dx64, dy = (float)dy64, dz = (float)dz64;
-    const float r0 = (float)r064, r0sq = r0 * r0, ir0 = 1.0f / r0;
-    float ar = 0, ai = 0;
+    const float r0 = (float)r064, r0sq = r0 * r0;
+    const float ihl = 0.5f * fc.inv_lambda;  // 1 / (2 lambda)
+    const float twol = 1.0f / ihl;           // 2 lambda
+    const float c2 = twol / r0;              // 2 lambda / R0
+    float alpha[2][3], beta[2][3], ub[2][3][3];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float* q = a.q[h][i];
+            const float du = dx * q[0] + dy * q[1] + dz * q[2];
+            const float uu = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+            alpha[h][i] = (2.0f * du + uu) * ihl;
+            const float* u = b + h * 9 + i * 3;
+            const float dv = dx * u[0] + dy * u[1] + dz * u[2];
+            const float vv = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+            beta[h][i] = (vv - 2.0f * dv) * ihl;
+            ub[h][i][0] = u[0] * ihl;
+            ub[h][i][1] = u[1] * ihl;
+            ub[h][i][2] = u[2] * ihl;
+        }
+    float ar = 0, ai = 0;  // ai accumulates -Im
 #pragma unroll
     for (int hb = 0; hb < 2; ++hb) {
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const float bx = b[hb * 9 + j * 3 + 0], by = b[hb * 9 + j * 3 + 1], bz = b[hb * 9 + j * 3 + 2];
             const float px = b[18 + hb * 9 + j * 3 + 0], py = b[18 + hb * 9 + j * 3 + 1];
             const float pz = b[18 + hb * 9 + j * 3 + 2];
 #pragma unroll
@@ -135,26 +162,27 @@ __device__ __forceinline__ void entry_f32(const RwgRec<float>& a, const double* 
                 const float ss = (ha == hb) ? fc.c4k2 : -fc.c4k2;
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    const float Dx = a.q[ha][i][0] - bx, Dy = a.q[ha][i][1] - by, Dz = a.q[ha][i][2] - bz;
-                    const float X = 2.0f * (dx * Dx + dy * Dy + dz * Dz) + (Dx * Dx + Dy * Dy + Dz * Dz);
-                    const float r2 = r0sq + X;
+                    const float w = a.q[ha][i][0] * ub[hb][j][0] + a.q[ha][i][1] * ub[hb][j][1] +
+                                    a.q[ha][i][2] * ub[hb][j][2];
+                    const float xl = fmaf(-2.0f, w, alpha[ha][i] + beta[hb][j]);  // X / (2 lambda)
+                    const float r2 = fmaf(xl, twol, r0sq);
                     const float ir = __builtin_amdgcn_rsqf(r2);
-                    const float r = r2 * ir;
-                    const float delta = X * __builtin_amdgcn_rcpf(r + r0);
-                    const float t = delta * fc.inv_lambda;
-                    const float sn = __builtin_amdgcn_sinf(t);
-                    const float sh = __builtin_amdgcn_sinf(0.5f * t);
-                    const float cm1 = -2.0f * sh * sh;  // cos(k delta) - 1
+                    const float th = xl * __builtin_amdgcn_rcpf(fmaf(r2, ir, r0));  // delta / (2 lambda)
+                    const float sh = __builtin_amdgcn_sinf(th);
+                    const float sn = __builtin_amdgcn_sinf(th + th);
+                    const float v = sh * sh * ir;
+                    const float gr = fmaf(-2.0f, v, ir);                  // Re g
+                    const float hneg = fmaf(th * ir, c2, v + v);          // -Re(g - 1/R0)
+                    const float gin = sn * ir;                            // -Im g
                     const float term = a.p[ha][i][0] * px + a.p[ha][i][1] * py + a.p[ha][i][2] * pz;
                     // term * g - ss * (g - 1/R0)
-                    const float gr = (1.0f + cm1) * ir, gi = -sn * ir;
-                    const float hr = cm1 * ir - delta * ir * ir0;
-                    ar += term * gr - ss * hr;
-                    ai += (term - ss) * gi;
+                    ar = fmaf(ss, hneg, fmaf(term, gr, ar));
+                    ai = fmaf(term - ss, gin, ai);
                 }
             }
         }
     }
+    ai = -ai;
     // Z = -j kappa e^{-jkR0} (ar + j ai)
     const float pr = c0 * ar + s0 * ai, pi = c0 * ai - s0 * ar;
     const float kap = fc.scale * a.l * b[36];
