@@ -210,7 +210,7 @@ struct KTimer {
 
 // Launch shapes, chosen by the autotuner (farcur_autotune) or defaults.
 struct Tuning {
-    int pinv_threads[kPinvClasses] = {64, 64, 128, 256, 256};
+    int pinv_threads[kPinvClasses] = {64, 256, 256, 256, 256, 256, 512};
     int fill_grid = 256 * 16;
     int resid_threads = 256;
 };
@@ -376,7 +376,7 @@ struct RoundPlan {
     int f32 = 0;
     bool pivot = false;
     int fill_grid = 0;
-    int pinv_threads[kPinvClasses] = {64, 64, 128, 256, 256};
+    int pinv_threads[kPinvClasses] = {64, 256, 256, 256, 256, 256, 512};
     int resid_threads = 256;
     double atol = 1e-10, rtol = 0, label_eps = 1e-4;
     uint64_t seed = 0;
@@ -1971,8 +1971,8 @@ extern "C" int farcur_batch_block(farcur_batch* B, int b, farcur_c128* C, farcur
 extern "C" int farcur_batch_kernel_times(farcur_batch* B, const char** names, double* ms, int* launches,
                                          double* work, int cap) {
     if (!B) return fail(FARCUR_EINVAL, "NULL batch");
-    static const char* kNames[KTimer::kClasses] = {"sample",       "fill",         "pinv",         "residual",
-                                                   "pinv_r16",     "pinv_r32",     "pinv_r48",     "pinv_r64",
+    static const char* kNames[KTimer::kClasses] = {"sample",   "fill",     "pinv",     "residual", "pinv_r16",
+                                                   "pinv_r24", "pinv_r32", "pinv_r40", "pinv_r48", "pinv_r64",
                                                    "pinv_global"};
     int n = 0;
     for (int i = 0; i < KTimer::kClasses && n < cap; ++i, ++n) {
@@ -2064,13 +2064,16 @@ extern "C" int farcur_autotune(farcur_ctx* ctx, farcur_tuning* out) {
     TRY(ctx->select());
     Tuning t = ctx->tune;
     farcur_tuning rep{};
-    const int cand[3] = {64, 128, 256};
     for (int c = 0; c < kPinvClasses; ++c) {
-        const int r = c < kPinvClasses - 1 ? kPinvClassMax[c] : 96;
-        const int nprob = c < kPinvClasses - 1 ? 4096 : 1024;
+        const bool glb = c == kPinvClasses - 1;
+        const int r = glb ? 96 : kPinvClassMax[c];
+        const int nprob = glb ? 1024 : 4096;
+        // cores up to 32: 64..256 threads; larger LDS classes and the global
+        // variant: 128..512 (64 threads would leave one lane pair per column pair)
+        const int base = glb || r > 32 ? 128 : 64;
         double best = 1e30;
         for (int k = 0; k < 3; ++k) {
-            const int th = c < kPinvClasses - 1 ? cand[k] : 2 * cand[k];
+            const int th = base << k;
             double ms;
             TRY(time_pinv(ctx, r, nprob, th, kPinvClassMax[c], &ms));
             if (ms < best) {

@@ -71,9 +71,16 @@ void launch_fill(const BlockJob* jobs, int njobs, const int32_t* tile_prefix, in
 // selects the LDS variant sized for rmax (every listed job has r <= rmax);
 // rmax = 0 selects the global-memory variant.
 constexpr int kPinvLdsMaxR = 64;
-constexpr int kPinvClasses = 5;  // r in (0,16], (16,32], (32,48], (48,64], > 64
-constexpr int kPinvClassMax[kPinvClasses] = {16, 32, 48, 64, 0};
-inline int pinv_class(int r) { return r <= 16 ? 0 : r <= 32 ? 1 : r <= 48 ? 2 : r <= 64 ? 3 : 4; }
+// r classes (0,16], (16,24], (24,32], (32,40], (40,48], (48,64], > 64: the LDS of
+// a launch is sized for its class maximum, so finer classes let small cores
+// pack more workgroups per CU.
+constexpr int kPinvClasses = 7;
+constexpr int kPinvClassMax[kPinvClasses] = {16, 24, 32, 40, 48, 64, 0};
+inline int pinv_class(int r) {
+    for (int c = 0; c < kPinvClasses - 1; ++c)
+        if (r <= kPinvClassMax[c]) return c;
+    return kPinvClasses - 1;
+}
 size_t pinv_lds_bytes(int rmax);
 // Complex elements of the Jacobi scratch V for sample count r (V plus, past
 // r = 1024, the r weights).

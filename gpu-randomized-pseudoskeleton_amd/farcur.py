@@ -49,6 +49,7 @@ def lib() -> ctypes.CDLL:
 
 
 vp, ci, cd, cu64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint64
+PINV_CLASSES = 7  # FARCUR_PINV_CLASSES
 
 
 class Opts(ctypes.Structure):
@@ -91,7 +92,8 @@ class BlockInfo(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("pinv_threads", ci * 5), ("fill_grid", ci), ("resid_threads", ci), ("pinv_ms", cd * 5)]
+    _fields_ = [("pinv_threads", ci * PINV_CLASSES), ("fill_grid", ci), ("resid_threads", ci),
+                ("pinv_ms", cd * PINV_CLASSES)]
 
     def as_dict(self):
         return {"pinv_threads": list(self.pinv_threads), "fill_grid": self.fill_grid,

@@ -318,7 +318,8 @@ int farcur_batch_matvec(farcur_batch* batch, const farcur_c128* V, int nrhs, far
 
 /* Device time (ms) and launch count of each kernel class over the last
  * batch_compress call with profiling on: names "sample", "fill", "pinv",
- * "residual". Returns the number of classes written (<= cap). */
+ * "residual", then per pinv class "pinv_r16" ... "pinv_r64", "pinv_global".
+ * Returns the number of classes written (<= cap). */
 int farcur_batch_profile(farcur_batch* batch, int enable);
 int farcur_batch_kernel_times(farcur_batch* batch, const char** names, double* ms, int* launches,
                               double* evals, int cap);
@@ -328,12 +329,14 @@ int farcur_batch_kernel_times(farcur_batch* batch, const char** names, double* m
  * blocks from a `divfactor` sweep (codes/GPU_implementation/run_experiment.sh:8,
  * compres_single_block_spheres_for_sweep2.jl:574-587); here farcur_autotune
  * times candidate shapes on the device and keeps the fastest per context.
- * pinv classes: sample counts (0,16], (16,32], (32,48], (48,64], > 64. */
+ * pinv classes: sample counts (0,16], (16,24], (24,32], (32,40], (40,48],
+ * (48,64] (LDS variant sized for the class maximum) and > 64 (global). */
+#define FARCUR_PINV_CLASSES 7
 typedef struct {
-    int pinv_threads[5];
+    int pinv_threads[FARCUR_PINV_CLASSES];
     int fill_grid;
     int resid_threads;
-    double pinv_ms[5]; /* autotune: best time of the class's probe (ms) */
+    double pinv_ms[FARCUR_PINV_CLASSES]; /* autotune: best time of the class's probe (ms) */
 } farcur_tuning;
 
 int farcur_autotune(farcur_ctx* ctx, farcur_tuning* out);
