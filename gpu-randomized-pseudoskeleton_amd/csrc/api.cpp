@@ -235,7 +235,7 @@ struct farcur_ctx {
     // Second stream of the batched rounds' two-stream pipeline (default
     // priority, like the first) and the events joining the two.
     hipStream_t stream2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_pfork = nullptr, ev_pjoin = nullptr;
     Tuning tune;
 
     int select() {
@@ -395,7 +395,10 @@ struct RoundPlan {
     }
 };
 
-void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt, hipStream_t s) {
+// side: a second stream for the pinv classes (nullptr: everything on s). The
+// classes are independent, so with a side stream the large-r classes (few cores,
+// each long: the tail of a launch) run beside the many small ones.
+void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt, hipStream_t s, hipStream_t side = nullptr) {
     const FillParams fp = ctx->fill_params(P.f32);
     hipEvent_t a;
     kt.begin(0, s, &a);
@@ -417,12 +420,23 @@ void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt, hipStream_
                     P.fill_grid, s);
         kt.end(1, s, a);
     }
-    for (int c = 0; c < kPinvClasses; ++c) {
+    if (side) {
+        (void)hipEventRecord(ctx->ev_pfork, s);
+        (void)hipStreamWaitEvent(side, ctx->ev_pfork, 0);
+    }
+    // largest classes first; with a side stream they alternate between the two
+    int k = 0;
+    for (int c = kPinvClasses - 1; c >= 0; --c) {
         if (P.nlist[c] == 0) continue;
-        kt.begin(4 + c, s, &a);
+        hipStream_t sc = side && (k++ & 1) == 0 ? side : s;
+        kt.begin(4 + c, sc, &a);
         launch_pinv(P.jobs, P.list[c], P.nlist[c], P.atol, P.rtol, P.label_eps, kPinvClassMax[c], P.pinv_threads[c],
-                    P.out, s);
-        kt.end(4 + c, s, a);
+                    P.out, sc);
+        kt.end(4 + c, sc, a);
+    }
+    if (side) {
+        (void)hipEventRecord(ctx->ev_pjoin, side);
+        (void)hipStreamWaitEvent(s, ctx->ev_pjoin, 0);
     }
     kt.begin(3, s, &a);
     launch_residual(P.jobs, P.njobs, P.f32, P.resid_threads, P.out, s);
@@ -661,6 +675,8 @@ int farcur_create(int device, farcur_ctx** out) {
     HIPTRY(hipEventCreate(&ctx->ev1));
     HIPTRY(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
     HIPTRY(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    HIPTRY(hipEventCreateWithFlags(&ctx->ev_pfork, hipEventDisableTiming));
+    HIPTRY(hipEventCreateWithFlags(&ctx->ev_pjoin, hipEventDisableTiming));
     *out = ctx.release();
     return FARCUR_OK;
 }
@@ -678,6 +694,8 @@ void farcur_destroy(farcur_ctx* ctx) {
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
+    if (ctx->ev_pfork) hipEventDestroy(ctx->ev_pfork);
+    if (ctx->ev_pjoin) hipEventDestroy(ctx->ev_pjoin);
     if (ctx->stream2) hipStreamDestroy(ctx->stream2);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1399,6 +1417,8 @@ struct farcur_batch {
     int first_round_misses = 0;
     // Two-stream pipeline of the batched rounds: number of chunks (1 = off).
     int pipeline_chunks = 1;
+    // pinv r classes split over the context's two streams within a round
+    bool pinv_side_stream = true;
     std::vector<char> h_plan;
     // Rounds captured as hipGraphs, keyed by plan content and launch
     // parameters (at most kMaxGraphs, least recently used evicted).
@@ -1641,7 +1661,10 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
             P.seed = o.seed;
             P.label_eps = o.label_eps;
             P.tune(ctx->tune);
-            run_round(ctx, P, B->kt, (c & 1) ? s1 : s0);
+            // one chunk: the pinv classes use the second stream (not while
+            // profiling, which times every class alone)
+            hipStream_t side = nchunks == 1 && !B->profile && B->pinv_side_stream ? s1 : nullptr;
+            run_round(ctx, P, B->kt, (c & 1) ? s1 : s0, side);
         }
         HIPTRY(hipGetLastError());
         if (nchunks > 1) {
