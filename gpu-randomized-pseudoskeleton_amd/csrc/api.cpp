@@ -2472,88 +2472,125 @@ extern "C" int farcur_batch_matvec(farcur_batch* B, const farcur_c128* V, int s,
     const int N = ctx->ne;
     double ms_interp = 0;
     TRY(batch_interp(B, &ms_interp));
-    // Target groups: identical id lists form one group; distinct groups must
-    // not share an RWG.
-    std::vector<std::vector<int>> group_contribs;  // contribution indices per group
-    std::vector<const int32_t*> group_ids_dev, group_ids_host;
-    std::vector<int> group_m;
-    std::unordered_map<uint64_t, std::vector<int>> by_hash;
-    std::vector<int> owner(N, -1);
-    // A group is an id list in one order: its contributions' rows follow it.
-    auto group_of = [&](const int32_t* host_ids, const int32_t* dev_ids, int cnt, int* out) -> int {
-        uint64_t h = fnv1a(0xcbf29ce484222325ull, host_ids, (size_t)cnt * 4) ^ (uint64_t)cnt;
-        for (int g : by_hash[h]) {
-            if (group_m[g] != cnt) continue;
-            if (std::memcmp(group_ids_host[g], host_ids, (size_t)cnt * 4) == 0) {
-                *out = g;
-                return FARCUR_OK;
+    // Groups of RWG id lists. A group is an id list in one order (a
+    // contribution's rows follow it). Target groups (rows of Y written) must
+    // be disjoint or identical, so that every row of Y has one writer; source
+    // groups (rows of V read) need only be identified.
+    struct Registry {
+        bool exclusive;
+        std::vector<const int32_t*> ids_dev, ids_host;
+        std::vector<int> m;
+        std::unordered_map<uint64_t, std::vector<int>> by_hash;
+        std::vector<int> owner;
+        int of(const int32_t* host_ids, const int32_t* dev_ids, int cnt, int* out) {
+            uint64_t h = fnv1a(0xcbf29ce484222325ull, host_ids, (size_t)cnt * 4) ^ (uint64_t)cnt;
+            for (int g : by_hash[h]) {
+                if (m[g] == cnt && std::memcmp(ids_host[g], host_ids, (size_t)cnt * 4) == 0) {
+                    *out = g;
+                    return FARCUR_OK;
+                }
             }
+            const int g = (int)m.size();
+            if (exclusive) {
+                for (int i = 0; i < cnt; ++i) {
+                    if (owner[host_ids[i]] != -1)
+                        return fail(FARCUR_EINVAL,
+                                    "row/column sets of the blocks overlap without being identical lists (RWG %d)",
+                                    host_ids[i]);
+                    owner[host_ids[i]] = g;
+                }
+            }
+            by_hash[h].push_back(g);
+            m.push_back(cnt);
+            ids_dev.push_back(dev_ids);
+            ids_host.push_back(host_ids);
+            *out = g;
+            return FARCUR_OK;
         }
-        const int g = (int)group_m.size();
-        for (int i = 0; i < cnt; ++i) {
-            if (owner[host_ids[i]] != -1)
-                return fail(FARCUR_EINVAL,
-                            "row/column sets of the blocks overlap without being identical lists (RWG %d)",
-                            host_ids[i]);
-            owner[host_ids[i]] = g;
-        }
-        by_hash[h].push_back(g);
-        group_m.push_back(cnt);
-        group_ids_dev.push_back(dev_ids);
-        group_ids_host.push_back(host_ids);
-        group_contribs.emplace_back();
-        *out = g;
-        return FARCUR_OK;
     };
-    // Pass A: one problem per contribution (block, orientation), T = A1 X.
-    // Pass B segments: one per contribution, L T into the contribution's group.
-    std::vector<MrhsProb> pa;
-    std::vector<MrhsSeg> sa, sb;
-    std::vector<int32_t> ta_p, ta_r;
-    size_t t2_total = 0;
+    Registry tgt{true}, src{false};
+    tgt.owner.assign(N, -1);
+    // Contributions (block, orientation): source group (rows of V read) and
+    // target group (rows of Y written).
+    struct Contrib {
+        int b, o, r, src, tgt;
+        const void* a1;  // pass A operand (r x |src|, column-major: R or P^T)
+        const void* l;   // pass B operand, read transposed (P^T or R)
+        size_t t2_off;
+    };
+    std::vector<Contrib> cons;
     for (int b = 0; b < B->nb; ++b) {
         const BState& S = B->st[b];
-        const int r = S.fin_r;
         const int32_t* hrows = &B->row_ids[B->row_off[b]];
         const int32_t* hcols = &B->col_ids[B->col_off[b]];
         const int32_t* drows = B->d_rows.as<int32_t>() + B->row_off[b];
         const int32_t* dcols = B->d_cols.as<int32_t>() + B->col_off[b];
-        for (int o = 0; o < (symmetric ? 2 : 1); ++o) {
-            MrhsSeg a{}, l{};
-            if (o == 0) {  // Y[rows] += P (R V[cols])
-                a = MrhsSeg{S.fin.R, r, 0, S.n, 0, dcols, 0};
-                l = MrhsSeg{B->interp[b], S.m, 0, r, 0, nullptr, t2_total};
-            } else {       // Y[cols] += R^T (P^T V[rows])
-                a = MrhsSeg{B->interp[b], S.m, 1, S.m, 0, drows, 0};
-                l = MrhsSeg{S.fin.R, r, 1, r, 0, nullptr, t2_total};
-            }
-            int g;
-            if (o == 0)
-                TRY(group_of(hrows, drows, S.m, &g));
-            else
-                TRY(group_of(hcols, dcols, S.n, &g));
-            const int prob = (int)pa.size();
-            pa.push_back(MrhsProb{nullptr, t2_total, r, (int)sa.size(), (int)sa.size() + 1, 0});
-            sa.push_back(a);
-            add_tiles(prob, r, ta_p, ta_r);
-            group_contribs[g].push_back((int)sb.size());
-            sb.push_back(l);
-            t2_total += (size_t)r * s;
+        // o = 0: Y[rows] += P (R V[cols]);  o = 1: Y[cols] += R^T (P^T V[rows])
+        int s0, t0;
+        TRY(src.of(hcols, dcols, S.n, &s0));
+        TRY(tgt.of(hrows, drows, S.m, &t0));
+        cons.push_back(Contrib{b, 0, S.fin_r, s0, t0, S.fin.R, B->interp[b], 0});
+        if (symmetric) {
+            int s1, t1;
+            TRY(src.of(hrows, drows, S.m, &s1));
+            TRY(tgt.of(hcols, dcols, S.n, &t1));
+            cons.push_back(Contrib{b, 1, S.fin_r, s1, t1, B->interp[b], S.fin.R, 0});
         }
     }
-    const int ng = (int)group_m.size();
-    std::vector<MrhsProb> pb(ng);
-    std::vector<MrhsSeg> sb_sorted;
+    const int nsrc = (int)src.m.size(), ntgt = (int)tgt.m.size();
+    // Pass A: one row-segmented problem per source group, its contributions'
+    // rows stacked (T of a contribution = r consecutive rows of the problem).
+    std::vector<std::vector<int>> by_src(nsrc), by_tgt(ntgt);
+    for (int c = 0; c < (int)cons.size(); ++c) {
+        by_src[cons[c].src].push_back(c);
+        by_tgt[cons[c].tgt].push_back(c);
+    }
+    std::vector<MrhsProb> pa;
+    std::vector<MrhsRowSeg> rsa;
+    std::vector<int32_t> ta_p, ta_r;
+    size_t t2_total = 0;
+    for (int g = 0; g < nsrc; ++g) {
+        if (by_src[g].empty()) continue;
+        MrhsProb P{};
+        P.out_off = t2_total;
+        P.b_ids = src.ids_dev[g];
+        P.k = src.m[g];
+        P.seg_begin = (int)rsa.size();
+        int rows = 0;
+        for (int c : by_src[g]) {
+            Contrib& C = cons[c];
+            C.t2_off = t2_total + (size_t)rows * s;
+            rsa.push_back(MrhsRowSeg{C.a1, C.r, C.r, rows, 0});
+            rows += C.r;
+        }
+        P.seg_end = (int)rsa.size();
+        P.m = rows;
+        const int prob = (int)pa.size();
+        pa.push_back(P);
+        add_tiles(prob, rows, ta_p, ta_r);
+        t2_total += (size_t)rows * s;
+    }
+    // Pass B: one K-segmented problem per target group.
+    std::vector<MrhsProb> pb;
+    std::vector<MrhsSeg> sb;
     std::vector<int32_t> tb_p, tb_r;
-    sb_sorted.reserve(sb.size());
-    for (int g = 0; g < ng; ++g) {
-        pb[g] = MrhsProb{group_ids_dev[g], 0, group_m[g], (int)sb_sorted.size(), 0, 0};
-        for (int c : group_contribs[g]) sb_sorted.push_back(sb[c]);
-        pb[g].seg_end = (int)sb_sorted.size();
-        add_tiles(g, group_m[g], tb_p, tb_r);
+    for (int g = 0; g < ntgt; ++g) {
+        if (by_tgt[g].empty()) continue;
+        MrhsProb P{};
+        P.out_ids = tgt.ids_dev[g];
+        P.m = tgt.m[g];
+        P.seg_begin = (int)sb.size();
+        for (int c : by_tgt[g]) {
+            const Contrib& C = cons[c];
+            sb.push_back(MrhsSeg{C.l, C.r, 1, C.r, 0, nullptr, C.t2_off});
+        }
+        P.seg_end = (int)sb.size();
+        const int prob = (int)pb.size();
+        pb.push_back(P);
+        add_tiles(prob, tgt.m[g], tb_p, tb_r);
     }
     // Device buffers: V and Y row-major complex64, T, descriptors.
-    DBuf dV, dY, dT, dpa, dsa, dtap, dtar, dpb, dsb, dtbp, dtbr;
+    DBuf dV, dY, dT, dpa, drsa, dtap, dtar, dpb, dsb, dtbp, dtbr;
     TRY(dV.ensure((size_t)N * s * 8));
     TRY(dY.ensure((size_t)N * s * 8));
     TRY(dT.ensure(std::max<size_t>(t2_total, 1) * 8));
@@ -2563,11 +2600,11 @@ extern "C" int farcur_batch_matvec(farcur_batch* B, const farcur_c128* V, int s,
         return FARCUR_OK;
     };
     TRY(up(dpa, pa.data(), pa.size() * sizeof(MrhsProb)));
-    TRY(up(dsa, sa.data(), sa.size() * sizeof(MrhsSeg)));
+    TRY(up(drsa, rsa.data(), rsa.size() * sizeof(MrhsRowSeg)));
     TRY(up(dtap, ta_p.data(), ta_p.size() * 4));
     TRY(up(dtar, ta_r.data(), ta_r.size() * 4));
     TRY(up(dpb, pb.data(), pb.size() * sizeof(MrhsProb)));
-    TRY(up(dsb, sb_sorted.data(), sb_sorted.size() * sizeof(MrhsSeg)));
+    TRY(up(dsb, sb.data(), sb.size() * sizeof(MrhsSeg)));
     TRY(up(dtbp, tb_p.data(), tb_p.size() * 4));
     TRY(up(dtbr, tb_r.data(), tb_r.size() * 4));
     std::vector<float2> hv((size_t)N * s);
@@ -2581,11 +2618,11 @@ extern "C" int farcur_batch_matvec(farcur_batch* B, const farcur_c128* V, int s,
     HIPTRY(hipEventCreate(&e1));
     HIPTRY(hipEventCreate(&e2));
     HIPTRY(hipEventRecord(e0, st));
-    launch_mrhs_gemm(dpa.as<MrhsProb>(), (int)ta_p.size(), dtap.as<int32_t>(), dtar.as<int32_t>(), dsa.as<MrhsSeg>(),
-                     B->f32, mode, dV.as<float2>(), s, dT.as<float2>(), st);
+    launch_mrhs_gemm(dpa.as<MrhsProb>(), (int)ta_p.size(), dtap.as<int32_t>(), dtar.as<int32_t>(), nullptr,
+                     drsa.as<MrhsRowSeg>(), B->f32, mode, dV.as<float2>(), s, dT.as<float2>(), st);
     HIPTRY(hipEventRecord(e1, st));
     launch_mrhs_gemm(dpb.as<MrhsProb>(), (int)tb_p.size(), dtbp.as<int32_t>(), dtbr.as<int32_t>(), dsb.as<MrhsSeg>(),
-                     B->f32, mode, dT.as<float2>(), s, dY.as<float2>(), st);
+                     nullptr, B->f32, mode, dT.as<float2>(), s, dY.as<float2>(), st);
     HIPTRY(hipEventRecord(e2, st));
     HIPTRY(hipGetLastError());
     std::vector<float2> hy((size_t)N * s);

@@ -47,8 +47,9 @@ __global__ __launch_bounds__(256) void k_mrhs_interp(const MrhsInterp* __restric
         const MrhsInterp J = jobs[b];
         const PC* C = (const PC*)J.c;
         PC* P = (PC*)J.p;
+        // P^T (r x m, column-major): entry P[i][k] at i * r + k
         for (int idx = threadIdx.x; idx < J.m * J.r; idx += blockDim.x) {
-            const int i = idx % J.m, k = idx / J.m;
+            const int i = idx / J.r, k = idx - i * J.r;
             double pr = 0, pi = 0;
             for (int l = 0; l < J.r; ++l) {
                 const c128 c = to_c128(C[(size_t)l * J.m + i]);
@@ -100,15 +101,68 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
     lo = (__bf16)(x - (float)hi);
 }
 
-template <typename PC, int kMode>
+// Loads rows [row0, row0 + 64) x K chunk [k0, k0 + 32) of a row-segmented A
+// (per-row pointer, leading dimension and local row in LDS; A column-major, so
+// consecutive rows of one segment are contiguous) and the gathered B chunk.
+template <typename PC>
+__device__ __forceinline__ void load_chunk_rows(const void* const* rt_a, const int* rt_lda, const int* rt_lrow,
+                                                const int32_t* __restrict__ b_ids, int K, int j0, int s, int k0,
+                                                const float2* __restrict__ B, GemmLds& L) {
+    for (int idx = threadIdx.x; idx < 64 * kKC; idx += blockDim.x) {
+        const int i = idx % 64, k = idx / 64;
+        float2 v = make_float2(0.f, 0.f);
+        const PC* A = (const PC*)rt_a[i];
+        if (A && k0 + k < K) v = ld_c(A, (size_t)(k0 + k) * rt_lda[i] + rt_lrow[i]);
+        L.ar[i][k] = v.x;
+        L.ai[i][k] = v.y;
+    }
+    for (int idx = threadIdx.x; idx < kKC * kNC; idx += blockDim.x) {
+        const int j = idx % kNC, k = idx / kNC;
+        float2 v = make_float2(0.f, 0.f);
+        if (k0 + k < K && j0 + j < s) v = B[(size_t)b_ids[k0 + k] * s + j0 + j];
+        L.br[k][j] = v.x;
+        L.bi[k][j] = v.y;
+    }
+}
+
+// Work item w of a persistent launch, mapped so that each XCD takes one
+// contiguous range of items (workgroups are dealt to the 8 XCDs round-robin):
+// consecutive tiles of one problem then share an L2.
+struct XcdItems {
+    int first, end, stride;
+    __device__ XcdItems(int total) {
+        const int G = gridDim.x;
+        if (G < 8) {  // some XCD would have no workgroup: plain striding
+            first = blockIdx.x;
+            end = total;
+            stride = G;
+            return;
+        }
+        const int x = blockIdx.x % 8, j = blockIdx.x / 8;
+        const int gx = G / 8 + (x < G % 8 ? 1 : 0);  // workgroups on this XCD (>= 1)
+        const int per = (total + 7) / 8;
+        first = x * per + j;
+        end = min(total, (x + 1) * per);
+        stride = gx;
+    }
+};
+
+// kRowSeg: pass A layout (a problem's rows are stacked contributions with
+// their own A, one gathered K range); otherwise pass B layout (one row range,
+// K concatenated over segments with their own A and dense B).
+template <typename PC, int kMode, bool kRowSeg>
 __global__ __launch_bounds__(256) void k_mrhs_gemm(const MrhsProb* __restrict__ probs, int ntiles_total,
                                                    const int32_t* __restrict__ tile_prob,
                                                    const int32_t* __restrict__ tile_row0,
-                                                   const MrhsSeg* __restrict__ segs, int nchunks,
+                                                   const MrhsSeg* __restrict__ segs,
+                                                   const MrhsRowSeg* __restrict__ rsegs, int nchunks,
                                                    const float2* __restrict__ B, int s, float2* __restrict__ Out) {
     __shared__ GemmLds L;
+    __shared__ const void* rt_a[64];
+    __shared__ int rt_lda[64], rt_lrow[64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int w = blockIdx.x; w < ntiles_total * nchunks; w += gridDim.x) {
+    const XcdItems it(ntiles_total * nchunks);
+    for (int w = it.first; w < it.end; w += it.stride) {
         const int tile = w / nchunks, j0 = (w % nchunks) * kNC;
         const MrhsProb& G = probs[tile_prob[tile]];
         const int row0 = tile_row0[tile], m = G.m;
@@ -123,12 +177,7 @@ __global__ __launch_bounds__(256) void k_mrhs_gemm(const MrhsProb* __restrict__ 
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b) accr[a][b] = acci[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-        for (int gi = G.seg_begin; gi < G.seg_end; ++gi) {
-            const MrhsSeg& g = segs[gi];
-            for (int k0 = 0; k0 < g.k; k0 += kKC) {
-                __syncthreads();
-                load_chunk<PC>(g, row0, m, j0, s, k0, B, L);
-                __syncthreads();
+        auto compute = [&]() {
                 if constexpr (kMode == kMrhsValu) {
                     const int ib = (tid / 16) * 4, jb = (tid % 16) * 4;
                     for (int k = 0; k < kKC; ++k) {
@@ -216,6 +265,42 @@ __global__ __launch_bounds__(256) void k_mrhs_gemm(const MrhsProb* __restrict__ 
 #undef FARCUR_BF3
                         }
                 }
+        };
+        if constexpr (kRowSeg) {
+            __syncthreads();  // the row table may still be read by the previous item
+            if (tid < 64) {
+                const int row = row0 + tid;
+                const void* pa = nullptr;
+                int lda = 0, lrow = 0;
+                if (row < m) {
+                    int lo = G.seg_begin, hi = G.seg_end - 1;  // last segment with row0 <= row
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (rsegs[mid].row0 <= row) lo = mid; else hi = mid - 1;
+                    }
+                    pa = rsegs[lo].a;
+                    lda = rsegs[lo].lda;
+                    lrow = row - rsegs[lo].row0;
+                }
+                rt_a[tid] = pa;
+                rt_lda[tid] = lda;
+                rt_lrow[tid] = lrow;
+            }
+            for (int k0 = 0; k0 < G.k; k0 += kKC) {
+                __syncthreads();
+                load_chunk_rows<PC>(rt_a, rt_lda, rt_lrow, G.b_ids, G.k, j0, s, k0, B, L);
+                __syncthreads();
+                compute();
+            }
+        } else {
+            for (int gi = G.seg_begin; gi < G.seg_end; ++gi) {
+                const MrhsSeg& g = segs[gi];
+                for (int k0 = 0; k0 < g.k; k0 += kKC) {
+                    __syncthreads();
+                    load_chunk<PC>(g, row0, m, j0, s, k0, B, L);
+                    __syncthreads();
+                    compute();
+                }
             }
         }
         // store
@@ -265,23 +350,25 @@ void launch_mrhs_interp(const MrhsInterp* jobs, int nb, int f32, hipStream_t st)
 }
 
 void launch_mrhs_gemm(const MrhsProb* probs, int ntiles, const int32_t* tile_prob, const int32_t* tile_row0,
-                      const MrhsSeg* segs, int f32, int mode, const float2* B, int s, float2* Out, hipStream_t st) {
+                      const MrhsSeg* segs, const MrhsRowSeg* rsegs, int f32, int mode, const float2* B, int s,
+                      float2* Out, hipStream_t st) {
     if (ntiles <= 0) return;
     const int nchunks = (s + kNC - 1) / kNC;
     const int work = ntiles * nchunks;
     const int grid = work < 65536 ? work : 65536;
-#define FARCUR_LAUNCH(PC, MODE)                                                                                \
-    hipLaunchKernelGGL((k_mrhs_gemm<PC, MODE>), dim3(grid), dim3(256), 0, st, probs, ntiles, tile_prob, tile_row0, \
-                       segs, nchunks, B, s, Out)
-    if (f32) {
-        if (mode == kMrhsValu) FARCUR_LAUNCH(c64, kMrhsValu);
-        else if (mode == kMrhsMfmaF32) FARCUR_LAUNCH(c64, kMrhsMfmaF32);
-        else FARCUR_LAUNCH(c64, kMrhsMfmaBf16x3);
+#define FARCUR_LAUNCH(PC, MODE, RS)                                                                            \
+    hipLaunchKernelGGL((k_mrhs_gemm<PC, MODE, RS>), dim3(grid), dim3(256), 0, st, probs, ntiles, tile_prob,     \
+                       tile_row0, segs, rsegs, nchunks, B, s, Out)
+#define FARCUR_LAUNCH_M(PC, RS)                                                                                \
+    if (mode == kMrhsValu) FARCUR_LAUNCH(PC, kMrhsValu, RS);                                                  \
+    else if (mode == kMrhsMfmaF32) FARCUR_LAUNCH(PC, kMrhsMfmaF32, RS);                                       \
+    else FARCUR_LAUNCH(PC, kMrhsMfmaBf16x3, RS);
+    if (rsegs) {
+        if (f32) { FARCUR_LAUNCH_M(c64, true) } else { FARCUR_LAUNCH_M(c128, true) }
     } else {
-        if (mode == kMrhsValu) FARCUR_LAUNCH(c128, kMrhsValu);
-        else if (mode == kMrhsMfmaF32) FARCUR_LAUNCH(c128, kMrhsMfmaF32);
-        else FARCUR_LAUNCH(c128, kMrhsMfmaBf16x3);
+        if (f32) { FARCUR_LAUNCH_M(c64, false) } else { FARCUR_LAUNCH_M(c128, false) }
     }
+#undef FARCUR_LAUNCH_M
 #undef FARCUR_LAUNCH
 }
 

@@ -9,12 +9,12 @@ namespace farcur {
 
 enum : int { kMrhsValu = 0, kMrhsMfmaF32 = 1, kMrhsMfmaBf16x3 = 2 };
 
-// Interpolation matrix of one block: P = C U (m x r, column-major), formed in
-// FP64 and stored in the panel precision.
+// Interpolation matrix of one block, P = C U, formed in FP64 and stored
+// transposed (P^T: r x m, column-major, the layout of R) in the panel precision.
 struct MrhsInterp {
     const void* c;     // C, m x r column-major (panel precision)
     const c128* u;     // U, r x r column-major
-    void* p;           // P, m x r column-major (panel precision)
+    void* p;           // P^T, r x m column-major (panel precision)
     int32_t m, r;
 };
 
@@ -29,21 +29,34 @@ struct MrhsSeg {
     size_t b_off;
 };
 
-// One problem: rows [0, m) of the output, the sum over segments
-// [seg_begin, seg_end). Row i goes to Out[out_ids[i] * s + j] when out_ids is
-// set, else to Out[out_off + i * s + j]. Every output row belongs to exactly one
-// problem, so the kernel stores without atomics.
+// Rows [row0, row0 + rows) of a row-segmented problem: A[i][k] =
+// a[k * lda + (i - row0)] (column-major, panel precision).
+struct MrhsRowSeg {
+    const void* a;
+    int32_t lda, rows, row0, pad;
+};
+
+// One problem: rows [0, m) of the output. Row i goes to Out[out_ids[i] * s + j]
+// when out_ids is set, else to Out[out_off + i * s + j]. Every output row
+// belongs to exactly one problem, so the kernel stores without atomics.
+// K-segmented problems (launched without row segments): the sum over the K
+// segments [seg_begin, seg_end) of MrhsSeg. Row-segmented problems: rows come
+// from the MrhsRowSeg range [seg_begin, seg_end) (sorted by row0), times one
+// gathered B, B[k][j] = B[b_ids[k] * s + j] for k below the problem's k.
 struct MrhsProb {
     const int32_t* out_ids;
+    const int32_t* b_ids;
     size_t out_off;
-    int32_t m, seg_begin, seg_end, pad;
+    int32_t m, k, seg_begin, seg_end;
 };
 
 int mrhs_chunk();
 void launch_mrhs_interp(const MrhsInterp* jobs, int nb, int f32, hipStream_t st);
 // ntiles 64-row tiles (problem tile_prob[t], first row tile_row0[t]) times
 // ceil(s / 64) right-hand-side chunks.
+// rsegs non-NULL: row-segmented problems (segs unused); else K-segmented.
 void launch_mrhs_gemm(const MrhsProb* probs, int ntiles, const int32_t* tile_prob, const int32_t* tile_row0,
-                      const MrhsSeg* segs, int f32, int mode, const float2* B, int s, float2* Out, hipStream_t st);
+                      const MrhsSeg* segs, const MrhsRowSeg* rsegs, int f32, int mode, const float2* B, int s,
+                      float2* Out, hipStream_t st);
 
 }  // namespace farcur
