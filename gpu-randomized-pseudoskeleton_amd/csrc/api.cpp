@@ -2580,11 +2580,14 @@ extern "C" int farcur_batch_matvec(farcur_batch* B, const farcur_c128* V, int s,
         P.out_ids = tgt.ids_dev[g];
         P.m = tgt.m[g];
         P.seg_begin = (int)sb.size();
+        int kbase = 0;
         for (int c : by_tgt[g]) {
             const Contrib& C = cons[c];
-            sb.push_back(MrhsSeg{C.l, C.r, 1, C.r, 0, nullptr, C.t2_off});
+            sb.push_back(MrhsSeg{C.l, C.r, 1, C.r, kbase, nullptr, C.t2_off});
+            kbase += C.r;
         }
         P.seg_end = (int)sb.size();
+        P.k = kbase;
         const int prob = (int)pb.size();
         pb.push_back(P);
         add_tiles(prob, tgt.m[g], tb_p, tb_r);

@@ -41,23 +41,51 @@ constexpr int kNC = 64;   // right-hand sides per tile
 constexpr int kKC = 32;   // K chunk
 
 // ---------------------------------------------------------------- P = C U
+// 64 rows of P at a time: computed with lanes over rows (C read coalesced, U
+// broadcast), staged in LDS, written as P^T with lanes over the r columns.
+constexpr int kInterpMaxR = 64;
 template <typename PC>
 __global__ __launch_bounds__(256) void k_mrhs_interp(const MrhsInterp* __restrict__ jobs, int nb) {
+    __shared__ PC tile[64][kInterpMaxR + 1];
     for (int b = blockIdx.x; b < nb; b += gridDim.x) {
         const MrhsInterp J = jobs[b];
         const PC* C = (const PC*)J.c;
-        PC* P = (PC*)J.p;
-        // P^T (r x m, column-major): entry P[i][k] at i * r + k
-        for (int idx = threadIdx.x; idx < J.m * J.r; idx += blockDim.x) {
-            const int i = idx / J.r, k = idx - i * J.r;
-            double pr = 0, pi = 0;
-            for (int l = 0; l < J.r; ++l) {
-                const c128 c = to_c128(C[(size_t)l * J.m + i]);
-                const c128 u = J.u[(size_t)k * J.r + l];
-                pr += c.re * u.re - c.im * u.im;
-                pi += c.re * u.im + c.im * u.re;
+        PC* Pt = (PC*)J.p;
+        const int m = J.m, r = J.r;
+        if (r > kInterpMaxR) {  // P^T entry P[i][k] at i * r + k, straight from global
+            for (int idx = threadIdx.x; idx < m * r; idx += blockDim.x) {
+                const int i = idx / r, k = idx - i * r;
+                double pr = 0, pi = 0;
+                for (int l = 0; l < r; ++l) {
+                    const c128 c = to_c128(C[(size_t)l * m + i]);
+                    const c128 u = J.u[(size_t)k * r + l];
+                    pr += c.re * u.re - c.im * u.im;
+                    pi += c.re * u.im + c.im * u.re;
+                }
+                store_c(Pt + idx, pr, pi);
             }
-            store_c(P + idx, pr, pi);
+            continue;
+        }
+        for (int i0 = 0; i0 < m; i0 += 64) {
+            const int rows = min(64, m - i0);
+            for (int idx = threadIdx.x; idx < 64 * r; idx += blockDim.x) {
+                const int i = idx % 64, k = idx / 64;
+                if (i >= rows) continue;
+                double pr = 0, pi = 0;
+                for (int l = 0; l < r; ++l) {
+                    const c128 c = to_c128(C[(size_t)l * m + i0 + i]);
+                    const c128 u = J.u[(size_t)k * r + l];
+                    pr += c.re * u.re - c.im * u.im;
+                    pi += c.re * u.im + c.im * u.re;
+                }
+                store_c(&tile[i][k], pr, pi);
+            }
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < rows * r; idx += blockDim.x) {
+                const int i = idx / r, k = idx - i * r;
+                Pt[(size_t)(i0 + i) * r + k] = tile[i][k];
+            }
+            __syncthreads();
         }
     }
 }
@@ -67,34 +95,6 @@ struct GemmLds {
     float ar[64][kKC + 1], ai[64][kKC + 1];  // A chunk, re / im planes
     float br[kKC][kNC + 1], bi[kKC][kNC + 1];
 };
-
-// Loads rows [row0, row0 + 64) x K chunk [k0, k0 + 32) of segment g's A and
-// the matching B chunk into LDS (zero padded).
-template <typename PC>
-__device__ __forceinline__ void load_chunk(const MrhsSeg& g, int row0, int m, int j0, int s, int k0,
-                                           const float2* __restrict__ B, GemmLds& L) {
-    const PC* A = (const PC*)g.a;
-    for (int idx = threadIdx.x; idx < 64 * kKC; idx += blockDim.x) {
-        // the contiguous index of A runs fastest: coalesced either way
-        const int i = g.a_trans ? idx / kKC : idx % 64;
-        const int k = g.a_trans ? idx % kKC : idx / 64;
-        float2 v = make_float2(0.f, 0.f);
-        if (row0 + i < m && k0 + k < g.k)
-            v = g.a_trans ? ld_c(A, (size_t)(row0 + i) * g.lda + k0 + k) : ld_c(A, (size_t)(k0 + k) * g.lda + row0 + i);
-        L.ar[i][k] = v.x;
-        L.ai[i][k] = v.y;
-    }
-    for (int idx = threadIdx.x; idx < kKC * kNC; idx += blockDim.x) {
-        const int j = idx % kNC, k = idx / kNC;
-        float2 v = make_float2(0.f, 0.f);
-        if (k0 + k < g.k && j0 + j < s) {
-            const size_t row = g.b_ids ? (size_t)g.b_ids[k0 + k] * s : g.b_off + (size_t)(k0 + k) * s;
-            v = B[row + j0 + j];
-        }
-        L.br[k][j] = v.x;
-        L.bi[k][j] = v.y;
-    }
-}
 
 __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
     hi = (__bf16)x;
@@ -120,6 +120,30 @@ __device__ __forceinline__ void load_chunk_rows(const void* const* rt_a, const i
         const int j = idx % kNC, k = idx / kNC;
         float2 v = make_float2(0.f, 0.f);
         if (k0 + k < K && j0 + j < s) v = B[(size_t)b_ids[k0 + k] * s + j0 + j];
+        L.br[k][j] = v.x;
+        L.bi[k][j] = v.y;
+    }
+}
+
+// Loads rows [row0, row0 + 64) x one packed K chunk: column k of the chunk is
+// local column kt_lk[k] of the transposed A at kt_a[k] (A[i][k] = a[i * lda + lk])
+// and B row kt_brow[k] (dense).
+template <typename PC>
+__device__ __forceinline__ void load_chunk_packed(const void* const* kt_a, const int* kt_lda, const int* kt_lk,
+                                                  const size_t* kt_brow, int row0, int m, int j0, int s,
+                                                  const float2* __restrict__ B, GemmLds& L) {
+    for (int idx = threadIdx.x; idx < 64 * kKC; idx += blockDim.x) {
+        const int i = idx / kKC, k = idx % kKC;  // k fastest: consecutive local columns are contiguous
+        float2 v = make_float2(0.f, 0.f);
+        const PC* A = (const PC*)kt_a[k];
+        if (A && row0 + i < m) v = ld_c(A, (size_t)(row0 + i) * kt_lda[k] + kt_lk[k]);
+        L.ar[i][k] = v.x;
+        L.ai[i][k] = v.y;
+    }
+    for (int idx = threadIdx.x; idx < kKC * kNC; idx += blockDim.x) {
+        const int j = idx % kNC, k = idx / kNC;
+        float2 v = make_float2(0.f, 0.f);
+        if (kt_a[k] && j0 + j < s) v = B[kt_brow[k] + j0 + j];
         L.br[k][j] = v.x;
         L.bi[k][j] = v.y;
     }
@@ -160,6 +184,9 @@ __global__ __launch_bounds__(256) void k_mrhs_gemm(const MrhsProb* __restrict__ 
     __shared__ GemmLds L;
     __shared__ const void* rt_a[64];
     __shared__ int rt_lda[64], rt_lrow[64];
+    __shared__ const void* kt_a[kKC];
+    __shared__ int kt_lda[kKC], kt_lk[kKC];
+    __shared__ size_t kt_brow[kKC];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const XcdItems it(ntiles_total * nchunks);
     for (int w = it.first; w < it.end; w += it.stride) {
@@ -293,14 +320,36 @@ __global__ __launch_bounds__(256) void k_mrhs_gemm(const MrhsProb* __restrict__ 
                 compute();
             }
         } else {
-            for (int gi = G.seg_begin; gi < G.seg_end; ++gi) {
-                const MrhsSeg& g = segs[gi];
-                for (int k0 = 0; k0 < g.k; k0 += kKC) {
-                    __syncthreads();
-                    load_chunk<PC>(g, row0, m, j0, s, k0, B, L);
-                    __syncthreads();
-                    compute();
+            // K packed across segments (segment g covers [g.kbase, g.kbase + g.k)):
+            // each 32-wide chunk gets a per-k table of (A column, B row).
+            for (int k0 = 0; k0 < G.k; k0 += kKC) {
+                __syncthreads();
+                if (tid < kKC) {
+                    const int kk = k0 + tid;
+                    const void* pa = nullptr;
+                    int lda = 0, lk = 0;
+                    size_t brow = 0;
+                    if (kk < G.k) {
+                        int lo = G.seg_begin, hi = G.seg_end - 1;  // last segment with kbase <= kk
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (segs[mid].kbase <= kk) lo = mid; else hi = mid - 1;
+                        }
+                        const MrhsSeg& g = segs[lo];
+                        pa = g.a;
+                        lda = g.lda;
+                        lk = kk - g.kbase;
+                        brow = g.b_off + (size_t)lk * s;
+                    }
+                    kt_a[tid] = pa;
+                    kt_lda[tid] = lda;
+                    kt_lk[tid] = lk;
+                    kt_brow[tid] = brow;
                 }
+                __syncthreads();
+                load_chunk_packed<PC>(kt_a, kt_lda, kt_lk, kt_brow, row0, m, j0, s, B, L);
+                __syncthreads();
+                compute();
             }
         }
         // store

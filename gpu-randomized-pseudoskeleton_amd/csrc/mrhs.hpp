@@ -24,7 +24,7 @@ struct MrhsInterp {
 struct MrhsSeg {
     const void* a;
     int32_t lda, a_trans;
-    int32_t k, pad;
+    int32_t k, kbase;  // kbase: first packed K index of the segment in its problem
     const int32_t* b_ids;
     size_t b_off;
 };
@@ -40,7 +40,8 @@ struct MrhsRowSeg {
 // when out_ids is set, else to Out[out_off + i * s + j]. Every output row
 // belongs to exactly one problem, so the kernel stores without atomics.
 // K-segmented problems (launched without row segments): the sum over the K
-// segments [seg_begin, seg_end) of MrhsSeg. Row-segmented problems: rows come
+// segments [seg_begin, seg_end) of MrhsSeg, packed along K (k = total K,
+// segments sorted by kbase, A read transposed, B dense). Row-segmented problems: rows come
 // from the MrhsRowSeg range [seg_begin, seg_end) (sorted by row0), times one
 // gathered B, B[k][j] = B[b_ids[k] * s + j] for k below the problem's k.
 struct MrhsProb {
