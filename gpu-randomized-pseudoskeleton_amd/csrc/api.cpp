@@ -1505,6 +1505,15 @@ BlockJob make_job(farcur_batch* B, int b, const RoundBufs& rb, const farcur_opts
     return J;
 }
 
+// Held-out rows of a learned round: at least `heldout`, one per 32 block rows
+// on tall blocks (a fixed count under-samples the error of blocks with
+// hundreds of rows), at most 4 x heldout, and no more than the rows left
+// over by the r samples.
+int heldout_rows(int m, int r, int heldout) {
+    const int h = std::min(std::max(heldout, (m + 31) / 32), 4 * heldout);
+    return std::max(0, std::min(h, m - r));
+}
+
 // Learned mode: the predicted round is accepted when its held-out rows meet
 // tol; otherwise the sample count doubles (capped) and the block runs again.
 void advance_learned(BState& s, const JobOut& out, const RoundBufs& rb, double accept, double tol, int heldout,
@@ -1534,7 +1543,7 @@ void advance_learned(BState& s, const JobOut& out, const RoundBufs& rb, double a
     }
     // grow by the factor, at least by the oversampling
     s.r = std::min(std::max((int)std::ceil(growth * s.r), s.r + std::max(oversample, 1)), s.cap);
-    s.h = std::min(heldout, s.m - s.r);
+    s.h = heldout_rows(s.m, s.r, heldout);
 }
 
 // Applies one round's result to a block's state (cuda_cur_compress3 control flow).
@@ -1895,7 +1904,7 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
         for (int b = 0; b < B->nb; ++b) {
             BState& s = B->st[b];
             s.r = std::max(1, std::min(rp[b], s.cap));
-            s.h = std::min(o.heldout, s.m - s.r);
+            s.h = heldout_rows(s.m, s.r, o.heldout);
             active[b] = b;
         }
         std::vector<RoundBufs> bufs;

@@ -185,7 +185,8 @@ typedef struct {
     int r_start;     /* 0: reference rule r0 = D < 10 ? 1 : D / 10 (SW2:889-894) */
     int max_rank;    /* cap on the sample count (0: min(m, n), the reference cap) */
     int block_id;    /* stream id of a single-block call */
-    int heldout;     /* held-out rows per block in FARCUR_MODE_LEARNED (default 8) */
+    int heldout;     /* held-out rows per block in FARCUR_MODE_LEARNED (default 8; blocks of
+                        m > 32 * heldout rows use ceil(m / 32), at most 4 * heldout) */
     int oversample;  /* p added to the predicted rank in FARCUR_MODE_LEARNED (default 14) */
     double label_eps;/* singular values >= label_eps * sigma_max are counted as rank_eps */
     double heldout_safety; /* learned mode accepts a round when held-out error <= heldout_safety * tol (0.5) */

@@ -85,7 +85,7 @@ def test_learned_mode_heldout_sampling_parity(farcur, gctx, ellipsoid, oracle):
         m, n, r = ro[b + 1] - ro[b], co[b + 1] - co[b], info["r"][b]
         last = info["rounds"][b] - 1  # round index of the final samples
         rows, cols = ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]]
-        h = min(8, m - r)
+        h = max(0, min(min(max(8, -(-m // 32)), 32), m - r))  # held-out rows: >= 8, one per 32 rows, <= 32
         hp, rsp = oracle.sample_heldout(11, b, last, m, h, r)
         out = bt.block(b)
         np.testing.assert_array_equal(out["rs"], rows[rsp])
