@@ -228,7 +228,7 @@ struct farcur_ctx {
     DBuf rec64, rec32, cen;
     uint64_t geom_hash = 0;
     // single-block scratch
-    DBuf d_rows, d_cols, d_cs, d_rs, d_C, d_R, d_W, d_V, d_U, d_sigma, d_v, d_y0, d_y1, d_plan, d_out;
+    DBuf d_rows, d_cols, d_cs, d_rs, d_rpos, d_C, d_R, d_W, d_V, d_U, d_sigma, d_v, d_y0, d_y1, d_plan, d_out;
     int last_m = 0, last_n = 0, last_r = 0, last_f32 = 0, last_y = 0;
     bool has_result = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -431,7 +431,7 @@ void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt, hipStream_
         hipStream_t sc = side && (k++ & 1) == 0 ? side : s;
         kt.begin(4 + c, sc, &a);
         launch_pinv(P.jobs, P.list[c], P.nlist[c], P.atol, P.rtol, P.label_eps, kPinvClassMax[c], P.pinv_threads[c],
-                    P.out, sc);
+                    P.f32, P.out, sc);
         kt.end(4 + c, sc, a);
     }
     if (side) {
@@ -461,10 +461,11 @@ void pack_round(const std::vector<BlockJob>& jobs, bool pivot, PackedRound& pk) 
     pwr[0] = 0;
     for (int j = 0; j < nj; ++j) {
         const BlockJob& J = jobs[j];
+        const int pw = J.rpos ? 0 : kPanelW;  // with row positions W is gathered from C, not filled
         const int all = pivot ? fill_tiles(J.m, J.n, J.r, J.h, kPanelC)
-                              : fill_tiles(J.m, J.n, J.r, J.h, kPanelC | kPanelW | kPanelR | kPanelH);
+                              : fill_tiles(J.m, J.n, J.r, J.h, kPanelC | pw | kPanelR | kPanelH);
         pa[j + 1] = pa[j] + all;
-        pwr[j + 1] = pwr[j] + (pivot ? fill_tiles(J.m, J.n, J.r, J.h, kPanelW | kPanelR | kPanelH) : 0);
+        pwr[j + 1] = pwr[j] + (pivot ? fill_tiles(J.m, J.n, J.r, J.h, pw | kPanelR | kPanelH) : 0);
         lists[pinv_class(J.r)].push_back(j);
     }
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -685,7 +686,7 @@ void farcur_destroy(farcur_ctx* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    DBuf* bufs[] = {&ctx->rec64, &ctx->rec32, &ctx->cen, &ctx->d_rows, &ctx->d_cols, &ctx->d_cs,    &ctx->d_rs,
+    DBuf* bufs[] = {&ctx->rec64, &ctx->rec32, &ctx->cen, &ctx->d_rows, &ctx->d_cols, &ctx->d_cs,    &ctx->d_rs, &ctx->d_rpos,
                     &ctx->d_C,   &ctx->d_R,   &ctx->d_W,    &ctx->d_V,    &ctx->d_U,     &ctx->d_sigma,
                     &ctx->d_v,   &ctx->d_y0,  &ctx->d_y1,   &ctx->d_plan, &ctx->d_out};
     for (DBuf* b : bufs) b->release();
@@ -785,7 +786,7 @@ int farcur_pinv(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, doubl
     Plan* dp = ctx->d_plan.as<Plan>();
     const int cls = pinv_class(r);
     const int threads = cls < kPinvClasses - 1 ? ctx->tune.pinv_threads[cls] : (r > 128 ? 1024 : 256);
-    launch_pinv(&dp->job, dp->list, 1, atol, rtol, 0.0, kPinvClassMax[cls], threads, &dp->out, ctx->stream);
+    launch_pinv(&dp->job, dp->list, 1, atol, rtol, 0.0, kPinvClassMax[cls], threads, 0, &dp->out, ctx->stream);
     HIPTRY(hipGetLastError());
     HIPTRY(hipMemcpyAsync(U, ctx->d_U.p, rr, hipMemcpyDeviceToHost, ctx->stream));
     std::vector<double> s(r);
@@ -923,6 +924,7 @@ namespace {
 // Device buffers of one block's round.
 struct RoundBufs {
     int32_t *cs, *rs;
+    int32_t* rpos;  // sampled row positions (W gathered from C); NULL for given samples
     void *C, *R;
     c128 *W, *V, *U;
     double* sigma;
@@ -968,6 +970,7 @@ int block_loop(farcur_ctx* ctx, const BlockLoopIO& io, const farcur_opts& o, All
         J.cols = io.d_cols;
         J.cs = rb.cs;
         J.rs = rb.rs;
+        J.rpos = rb.rpos;
         J.C = rb.C;
         J.R = rb.R;
         J.W = rb.W;
@@ -1068,15 +1071,23 @@ extern "C" int farcur_compress_block(farcur_ctx* ctx, const int* rows, int m, co
     auto alloc = [&](int r, RoundBufs& rb) -> int {
         TRY(ctx->d_cs.ensure(r * 4));
         TRY(ctx->d_rs.ensure(r * 4));
+        TRY(ctx->d_rpos.ensure(r * 4));
         TRY(ctx->d_C.ensure((size_t)m * r * psize(f32)));
         TRY(ctx->d_R.ensure((size_t)r * n * psize(f32)));
         TRY(ctx->d_W.ensure((size_t)r * r * 16));
         TRY(ctx->d_V.ensure(pinv_v_elems(r) * 16));
         TRY(ctx->d_U.ensure((size_t)r * r * 16));
         TRY(ctx->d_sigma.ensure(r * 8));
-        rb = RoundBufs{ctx->d_cs.as<int32_t>(), ctx->d_rs.as<int32_t>(), ctx->d_C.p,         ctx->d_R.p,
-                       ctx->d_W.as<c128>(),    ctx->d_V.as<c128>(),    ctx->d_U.as<c128>(), ctx->d_sigma.as<double>(),
-                       nullptr,                nullptr,                nullptr};
+        rb = RoundBufs{};
+        rb.cs = ctx->d_cs.as<int32_t>();
+        rb.rs = ctx->d_rs.as<int32_t>();
+        rb.rpos = ctx->d_rpos.as<int32_t>();
+        rb.C = ctx->d_C.p;
+        rb.R = ctx->d_R.p;
+        rb.W = ctx->d_W.as<c128>();
+        rb.V = ctx->d_V.as<c128>();
+        rb.U = ctx->d_U.as<c128>();
+        rb.sigma = ctx->d_sigma.as<double>();
         return FARCUR_OK;
     };
     KTimer kt;
@@ -1441,6 +1452,8 @@ int batch_alloc_round(farcur_batch* B, int r, int m, int n, RoundBufs& rb) {
     rb.cs = (int32_t*)p;
     TRY(B->arena.alloc((size_t)r * 4, &p));
     rb.rs = (int32_t*)p;
+    TRY(B->arena.alloc((size_t)r * 4, &p));
+    rb.rpos = (int32_t*)p;
     TRY(B->arena.alloc((size_t)m * r * ps, &p));
     rb.C = p;
     TRY(B->arena.alloc((size_t)r * n * ps, &p));
@@ -1491,6 +1504,7 @@ BlockJob make_job(farcur_batch* B, int b, const RoundBufs& rb, const farcur_opts
     J.cols = B->d_cols.as<int32_t>() + B->col_off[b];
     J.cs = rb.cs;
     J.rs = rb.rs;
+    J.rpos = rb.rpos;
     J.C = rb.C;
     J.R = rb.R;
     J.W = rb.W;
@@ -1773,7 +1787,14 @@ int batch_account(farcur_batch* B) {
             HIPTRY(hipMemcpy(rs.data(), J.rs, J.r * 4, hipMemcpyDeviceToHost));
             const double tcs = distinct_triangles(ctx->tri_of_rwg, cs.data(), J.r, scratch);
             const double trs = distinct_triangles(ctx->tri_of_rwg, rs.data(), J.r, scratch);
-            B->work[1] += 9.0 * (trows[J.b] * tcs + trs * tcs + trs * tcols[J.b]);
+            double ths = 0;
+            if (J.h > 0) {
+                std::vector<int32_t> hs(J.h);
+                HIPTRY(hipMemcpy(hs.data(), J.hs, J.h * 4, hipMemcpyDeviceToHost));
+                ths = distinct_triangles(ctx->tri_of_rwg, hs.data(), J.h, scratch);
+            }
+            // panels C, R, the held-out rows H and, unless gathered from C, W
+            B->work[1] += 9.0 * (trows[J.b] * tcs + (J.rpos ? 0.0 : trs * tcs) + trs * tcols[J.b] + ths * tcols[J.b]);
             B->work[2] += 96.0 * (double)J.r * J.r * J.r;
             B->work[4 + pinv_class(J.r)] += 96.0 * (double)J.r * J.r * J.r;
             B->work[3] += ps * ((double)J.m * J.r + (double)J.r * J.n) + 16.0 * J.r * J.r + 16.0 * (J.n + 2.0 * J.m);
@@ -2060,7 +2081,7 @@ int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* 
     for (int rep = 0; rep < 3; ++rep) {
         HIPTRY(hipMemcpyAsync(W.p, W0.p, nprob * rr * 16, hipMemcpyDeviceToDevice, s));
         HIPTRY(hipEventRecord(ctx->ev0, s));
-        launch_pinv((const BlockJob*)d, (const int32_t*)(d + bj), nprob, 1e-10, 0.0, 0.0, rmax, threads,
+        launch_pinv((const BlockJob*)d, (const int32_t*)(d + bj), nprob, 1e-10, 0.0, 0.0, rmax, threads, 0,
                     (JobOut*)(d + bj + bl + 256), s);
         HIPTRY(hipEventRecord(ctx->ev1, s));
         HIPTRY(hipGetLastError());

@@ -58,7 +58,10 @@ __device__ void sample_rows_heldout(const BlockJob& J, uint64_t seed) {
         if ((cand & 3u) == 0) o = stream4(seed, J.b, J.round, kStreamRows, cand);
         const uint32_t l = cand & 3u;
         const uint32_t u = l == 0 ? o.x : (l == 1 ? o.y : (l == 2 ? o.z : o.w));
-        if (select_step(u, N - cand, r - got)) J.rs[got++] = J.rows[t];
+        if (select_step(u, N - cand, r - got)) {
+            if (J.rpos) J.rpos[got] = t;
+            J.rs[got++] = J.rows[t];
+        }
         ++cand;
     }
 }
@@ -91,7 +94,10 @@ __global__ void k_sample(const BlockJob* __restrict__ jobs, int njobs, uint64_t 
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
             const uint32_t tt = t4 + l;
-            if (tt < N && got < need && select_step(w[l], N - tt, need - got)) out[got++] = ids[tt];
+            if (tt < N && got < need && select_step(w[l], N - tt, need - got)) {
+                if (kind == 1 && J.rpos) J.rpos[got] = tt;
+                out[got++] = ids[tt];
+            }
         }
     }
 }
@@ -133,7 +139,10 @@ __global__ __launch_bounds__(256) void k_pivot_rows(const BlockJob* __restrict__
                 }
             }
             if (besti >= J.m) besti = 0;  // all entries NaN: keep a valid row
-            if (lane == 0) J.rs[col] = J.rows[besti];
+            if (lane == 0) {
+                J.rs[col] = J.rows[besti];
+                if (J.rpos) J.rpos[col] = besti;
+            }
         }
     }
 }
@@ -196,7 +205,8 @@ __global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restric
         const BlockJob& J = jobs[lo];
         int local = t - prefix[lo];
         const int m = J.m, n = J.n, r = J.r;
-        if (panels & kPanelC) {
+        const int panels_j = J.rpos ? (panels & ~kPanelW) : panels;  // W is gathered from C
+        if (panels_j & kPanelC) {
             const int mt = cdiv(m, kTileRows), nC = mt * cdiv(r, kTileCols);
             if (local < nC) {
                 fill_tile<T, PC, decltype(fc), true>((PC*)J.C, m, J.rows, m, J.cs, r, (local % mt) * kTileRows,
@@ -206,7 +216,7 @@ __global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restric
             local -= nC;
         }
         const int nt = cdiv(n, kTileRows);
-        if (panels & kPanelW) {
+        if (panels_j & kPanelW) {
             const int rt = cdiv(r, kTileRows);
             const int nW = rt * cdiv(r, kTileCols);
             if (local < nW) {
@@ -216,7 +226,7 @@ __global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restric
             }
             local -= nW;
         }
-        if (panels & kPanelR) {
+        if (panels_j & kPanelR) {
             const int nR = nt * cdiv(r, kTileCols);
             if (local < nR) {
                 fill_tile<T, PC, decltype(fc), false>((PC*)J.R, r, J.rs, r, J.cols, n, (local / nt) * kTileCols,
@@ -268,6 +278,15 @@ void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32
 }
 
 // ---------------------------------------------------------------- pinv
+// Element idx (column-major, r x r) of the core W: the W panel, or the rows
+// rpos of C (the same entries, filled once).
+__device__ __forceinline__ c128 load_core(const BlockJob& J, int idx, int f32) {
+    if (!J.rpos) return J.W[idx];
+    const int i = idx % J.r, j = idx / J.r;
+    const size_t at = (size_t)j * J.m + J.rpos[i];
+    return f32 ? to_c128(((const c64*)J.C)[at]) : ((const c128*)J.C)[at];
+}
+
 // LDS variant: W (then X = R^H), the accumulated rotations and the small
 // per-column arrays of one problem in dynamic LDS sized for the largest r of
 // the launch (r <= kPinvLdsMaxR); one workgroup per problem. The pinv is
@@ -296,14 +315,14 @@ size_t pinv_lds_bytes(int rmax) {
 
 template <int kT, int kE>
 __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist, int rmax,
-                           double atol, double rtol, double label_eps, JobOut* __restrict__ out) {
+                           double atol, double rtol, double label_eps, int f32, JobOut* __restrict__ out) {
     extern __shared__ __align__(16) char smem[];
     const PinvLds L(smem, rmax);
     for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
         const int jb = list[li];
         const BlockJob& J = jobs[jb];
         const int r = J.r;
-        for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) L.a[idx] = J.W[idx];
+        for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) L.a[idx] = load_core(J, idx, f32);
         int kept, reps, sweeps;
         double smax;
         pinv_qr_lds<kT, kE>(L.a, L.b, r, atol, rtol, J.U, r, J.sigma, J.W, L.flag, L.w, L.phys, L.norm, L.tau,
@@ -338,12 +357,16 @@ __device__ void pinv_global_one(const BlockJob& J, int jb, double atol, double r
 }
 
 __global__ void k_pinv_global(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist,
-                              double atol, double rtol, double label_eps, JobOut* __restrict__ out) {
+                              double atol, double rtol, double label_eps, int f32, JobOut* __restrict__ out) {
     __shared__ double s_w[1024];
     __shared__ int s_flag;
     for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
         const int jb = list[li];
         const BlockJob& J = jobs[jb];
+        if (J.rpos) {  // the Jacobi works in place on the W panel
+            for (int idx = threadIdx.x; idx < J.r * J.r; idx += blockDim.x) J.W[idx] = load_core(J, idx, f32);
+            __syncthreads();
+        }
         switch (jacobi_team(J.r, blockDim.x)) {
             case 1: pinv_global_one<1>(J, jb, atol, rtol, label_eps, s_w, &s_flag, out); break;
             case 2: pinv_global_one<2>(J, jb, atol, rtol, label_eps, s_w, &s_flag, out); break;
@@ -368,7 +391,7 @@ namespace {
 
 template <int kT, int kE>
 void launch_lds(int grid, int threads, size_t lds, hipStream_t s, const BlockJob* jobs, const int32_t* list,
-                int nlist, int rmax, double atol, double rtol, double label_eps, JobOut* out) {
+                int nlist, int rmax, double atol, double rtol, double label_eps, int f32, JobOut* out) {
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute((const void*)k_pinv_lds<kT, kE>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -376,26 +399,27 @@ void launch_lds(int grid, int threads, size_t lds, hipStream_t s, const BlockJob
         attr_set = true;
     }
     hipLaunchKernelGGL((k_pinv_lds<kT, kE>), dim3(grid), dim3(threads), lds, s, jobs, list, nlist, rmax, atol, rtol,
-                       label_eps, out);
+                       label_eps, f32, out);
 }
 
 // E rounded up to a compiled register count; past 16 elements the team
 // streams its columns from LDS (kE = 0).
 template <int kT>
 void launch_lds_t(int E, int grid, int threads, size_t lds, hipStream_t s, const BlockJob* jobs,
-                  const int32_t* list, int nlist, int rmax, double atol, double rtol, double label_eps, JobOut* out) {
-    if (E <= 1) launch_lds<kT, 1>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out);
-    else if (E <= 2) launch_lds<kT, 2>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out);
-    else if (E <= 4) launch_lds<kT, 4>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out);
-    else if (E <= 8) launch_lds<kT, 8>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out);
-    else if (E <= 16) launch_lds<kT, 16>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out);
-    else launch_lds<kT, 0>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out);
+                  const int32_t* list, int nlist, int rmax, double atol, double rtol, double label_eps, int f32,
+                  JobOut* out) {
+    if (E <= 1) launch_lds<kT, 1>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out);
+    else if (E <= 2) launch_lds<kT, 2>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out);
+    else if (E <= 4) launch_lds<kT, 4>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out);
+    else if (E <= 8) launch_lds<kT, 8>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out);
+    else if (E <= 16) launch_lds<kT, 16>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out);
+    else launch_lds<kT, 0>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out);
 }
 
 }  // namespace
 
 void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, double label_eps,
-                 int rmax, int threads, JobOut* out, hipStream_t s) {
+                 int rmax, int threads, int f32, JobOut* out, hipStream_t s) {
     if (nlist <= 0) return;
     const int grid = nlist < 16384 ? nlist : 16384;
     if (rmax > 0 && rmax <= kPinvLdsMaxR) {
@@ -404,15 +428,15 @@ void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double at
         const size_t lds = pinv_lds_bytes(rmax);
         switch (T) {
             case 1:
-            case 2: launch_lds_t<2>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out); break;
-            case 4: launch_lds_t<4>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out); break;
-            case 8: launch_lds_t<8>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out); break;
-            case 16: launch_lds_t<16>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out); break;
-            default: launch_lds_t<32>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, out); break;
+            case 2: launch_lds_t<2>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out); break;
+            case 4: launch_lds_t<4>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out); break;
+            case 8: launch_lds_t<8>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out); break;
+            case 16: launch_lds_t<16>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out); break;
+            default: launch_lds_t<32>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out); break;
         }
     } else {
         hipLaunchKernelGGL(k_pinv_global, dim3(grid), dim3(threads), 0, s, jobs, list, nlist, atol, rtol, label_eps,
-                           out);
+                           f32, out);
     }
 }
 

@@ -24,6 +24,8 @@ struct BlockJob {
     const int32_t* cols;  // n RWG ids
     int32_t* cs;          // r sampled col ids (RWG ids)
     int32_t* rs;          // r sampled row ids (RWG ids)
+    int32_t* rpos;        // r sampled row positions in the block, written by the sampler;
+                          // when set, W = C[rpos, :] is gathered from C instead of filled
     void* C;              // c128 or c64 (panel precision)
     void* R;
     c128* W;
@@ -87,8 +89,9 @@ size_t pinv_lds_bytes(int rmax);
 inline size_t pinv_v_elems(int r) { return (size_t)r * r + (size_t)(r + 1) / 2; }
 // Largest sample count any entry point accepts.
 constexpr int kMaxSamples = 8192;
+// f32: panel precision of C (for jobs whose W is gathered from C).
 void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, double label_eps,
-                 int rmax, int threads, JobOut* out, hipStream_t s);
+                 int rmax, int threads, int f32, JobOut* out, hipStream_t s);
 void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, JobOut* out, hipStream_t s);
 
 // Plain panel fill Z[i + j*ld] = Z(rows[i], cols[j]) (standalone, one panel).
