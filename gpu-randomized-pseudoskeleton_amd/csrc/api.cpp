@@ -444,53 +444,75 @@ void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt, hipStream_
 }
 
 // Host-side packing of a round: jobs, fill prefixes, pinv lists by r class,
-// into one device upload.
+// into one device upload. pack_layout computes the sections (prefix sums and
+// class lists into `scratch`), pack_write writes them at `dst` with the
+// padding zeroed, so equal plans are equal bytes.
 struct PackedRound {
-    std::vector<char> bytes;
-    size_t off_jobs = 0, off_pa = 0, off_pwr = 0, off_out = 0;
+    size_t off_jobs = 0, off_pa = 0, off_pwr = 0, off_out = 0, size = 0;
     size_t off_list[kPinvClasses] = {};
     int total_all = 0, total_wr = 0;
     int nlist[kPinvClasses] = {};
 };
 
-void pack_round(const std::vector<BlockJob>& jobs, bool pivot, PackedRound& pk) {
-    const int nj = (int)jobs.size();
-    std::vector<int32_t> pa(nj + 1), pwr(nj + 1);
+struct PackScratch {
+    std::vector<int32_t> pa, pwr;
     std::vector<int32_t> lists[kPinvClasses];
-    pa[0] = 0;
-    pwr[0] = 0;
+};
+
+void pack_layout(const BlockJob* jobs, int nj, bool pivot, PackedRound& pk, PackScratch& sc) {
+    sc.pa.assign(nj + 1, 0);
+    sc.pwr.assign(nj + 1, 0);
+    for (auto& l : sc.lists) l.clear();
     for (int j = 0; j < nj; ++j) {
         const BlockJob& J = jobs[j];
         const int pw = J.rpos ? 0 : kPanelW;  // with row positions W is gathered from C, not filled
         const int all = pivot ? fill_tiles(J.m, J.n, J.r, J.h, kPanelC)
                               : fill_tiles(J.m, J.n, J.r, J.h, kPanelC | pw | kPanelR | kPanelH);
-        pa[j + 1] = pa[j] + all;
-        pwr[j + 1] = pwr[j] + (pivot ? fill_tiles(J.m, J.n, J.r, J.h, pw | kPanelR | kPanelH) : 0);
-        lists[pinv_class(J.r)].push_back(j);
+        sc.pa[j + 1] = sc.pa[j] + all;
+        sc.pwr[j + 1] = sc.pwr[j] + (pivot ? fill_tiles(J.m, J.n, J.r, J.h, pw | kPanelR | kPanelH) : 0);
+        sc.lists[pinv_class(J.r)].push_back(j);
     }
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t o = 0;
     pk.off_jobs = o;
     o = align(o + nj * sizeof(BlockJob));
     pk.off_pa = o;
-    o = align(o + pa.size() * 4);
+    o = align(o + sc.pa.size() * 4);
     pk.off_pwr = o;
-    o = align(o + pwr.size() * 4);
+    o = align(o + sc.pwr.size() * 4);
     for (int c = 0; c < kPinvClasses; ++c) {
         pk.off_list[c] = o;
-        o = align(o + std::max<size_t>(lists[c].size(), 1) * 4);
-        pk.nlist[c] = (int)lists[c].size();
+        o = align(o + std::max<size_t>(sc.lists[c].size(), 1) * 4);
+        pk.nlist[c] = (int)sc.lists[c].size();
     }
     pk.off_out = o;
     o = align(o + nj * sizeof(JobOut));
-    pk.bytes.assign(o, 0);
-    std::memcpy(pk.bytes.data() + pk.off_jobs, jobs.data(), nj * sizeof(BlockJob));
-    std::memcpy(pk.bytes.data() + pk.off_pa, pa.data(), pa.size() * 4);
-    std::memcpy(pk.bytes.data() + pk.off_pwr, pwr.data(), pwr.size() * 4);
-    for (int c = 0; c < kPinvClasses; ++c)
-        if (!lists[c].empty()) std::memcpy(pk.bytes.data() + pk.off_list[c], lists[c].data(), lists[c].size() * 4);
-    pk.total_all = pa[nj];
-    pk.total_wr = pwr[nj];
+    pk.size = o;
+    pk.total_all = sc.pa[nj];
+    pk.total_wr = sc.pwr[nj];
+}
+
+void pack_write(const BlockJob* jobs, int nj, const PackedRound& pk, const PackScratch& sc, char* dst) {
+    size_t at = 0;
+    auto put = [&](size_t off, const void* src, size_t bytes) {
+        if (off > at) std::memset(dst + at, 0, off - at);  // padding
+        if (bytes) std::memcpy(dst + off, src, bytes);
+        at = off + bytes;
+    };
+    put(pk.off_jobs, jobs, nj * sizeof(BlockJob));
+    put(pk.off_pa, sc.pa.data(), sc.pa.size() * 4);
+    put(pk.off_pwr, sc.pwr.data(), sc.pwr.size() * 4);
+    for (int c = 0; c < kPinvClasses; ++c) put(pk.off_list[c], sc.lists[c].data(), sc.lists[c].size() * 4);
+    put(pk.off_out, nullptr, 0);
+    std::memset(dst + pk.off_out, 0, pk.size - pk.off_out);  // JobOut slots (written by the kernels)
+}
+
+// Single-job rounds (per-block loop, drop-in, standalone calls).
+void pack_round(const std::vector<BlockJob>& jobs, bool pivot, PackedRound& pk, std::vector<char>& bytes) {
+    PackScratch sc;
+    pack_layout(jobs.data(), (int)jobs.size(), pivot, pk, sc);
+    bytes.resize(pk.size);
+    pack_write(jobs.data(), (int)jobs.size(), pk, sc, bytes.data());
 }
 
 RoundPlan plan_from(const PackedRound& pk, void* dev, int njobs) {
@@ -982,9 +1004,10 @@ int block_loop(farcur_ctx* ctx, const BlockLoopIO& io, const farcur_opts& o, All
         J.y_new = io.d_y[ycur ^ 1];
         std::vector<BlockJob> jobs{J};
         PackedRound pk;
-        pack_round(jobs, o.row_mode == FARCUR_ROWS_PIVOT, pk);
-        TRY(plan_buf.ensure(pk.bytes.size()));
-        HIPTRY(hipMemcpyAsync(plan_buf.p, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
+        std::vector<char> bytes;
+        pack_round(jobs, o.row_mode == FARCUR_ROWS_PIVOT, pk, bytes);
+        TRY(plan_buf.ensure(bytes.size()));
+        HIPTRY(hipMemcpyAsync(plan_buf.p, bytes.data(), bytes.size(), hipMemcpyHostToDevice, ctx->stream));
         RoundPlan P = plan_from(pk, plan_buf.p, 1);
         P.f32 = f32;
         P.pivot = o.row_mode == FARCUR_ROWS_PIVOT;
@@ -1237,9 +1260,10 @@ int compat_impl(farcur_c128* Cc, farcur_c128* Uc, farcur_c128* Rc, int r, double
     J.y_new = ctx->d_y1.as<c128>();
     std::vector<BlockJob> jobs{J};
     PackedRound pk;
-    pack_round(jobs, false, pk);
-    TRY(ctx->d_plan.ensure(pk.bytes.size()));
-    HIPTRY(hipMemcpyAsync(ctx->d_plan.p, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice, s));
+    std::vector<char> bytes;
+    pack_round(jobs, false, pk, bytes);
+    TRY(ctx->d_plan.ensure(bytes.size()));
+    HIPTRY(hipMemcpyAsync(ctx->d_plan.p, bytes.data(), bytes.size(), hipMemcpyHostToDevice, s));
     RoundPlan P = plan_from(pk, ctx->d_plan.p, 1);
     P.atol = pinv_tol;
     P.rtol = 0;
@@ -1426,11 +1450,15 @@ struct farcur_batch {
     farcur_model* model = nullptr;
     bool learned = false;
     int first_round_misses = 0;
+    // Host-side wall time of the last compress (ms): rank prediction round
+    // trip, round planning, waiting for the device, advancing the block states.
+    double host_ms[4] = {};
     // Two-stream pipeline of the batched rounds: number of chunks (1 = off).
     int pipeline_chunks = 1;
     // pinv r classes split over the context's two streams within a round
     bool pinv_side_stream = true;
     std::vector<char> h_plan;
+    std::vector<PackScratch> pack_sc;  // per chunk, reused across rounds
     // Rounds captured as hipGraphs, keyed by plan content and launch
     // parameters (at most kMaxGraphs, least recently used evicted).
     static constexpr int kMaxGraphs = 8;
@@ -1606,6 +1634,7 @@ double job_cost(const BlockJob& J) {
 int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_opts& o, std::vector<RoundBufs>& bufs,
                 std::vector<JobOut>& outs) {
     farcur_ctx* ctx = B->ctx;
+    const auto t_plan0 = std::chrono::steady_clock::now();
     std::vector<BlockJob> jobs;
     jobs.reserve(active.size());
     bufs.resize(active.size());
@@ -1637,14 +1666,15 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
     nchunks = (int)cut.size() - 1;
     std::vector<PackedRound> pk(nchunks);
     std::vector<size_t> off(nchunks + 1, 0);
+    if ((int)B->pack_sc.size() < nchunks) B->pack_sc.resize(nchunks);
     for (int c = 0; c < nchunks; ++c) {
-        std::vector<BlockJob> part(jobs.begin() + cut[c], jobs.begin() + cut[c + 1]);
-        pack_round(part, o.row_mode == FARCUR_ROWS_PIVOT, pk[c]);
-        off[c + 1] = off[c] + ((pk[c].bytes.size() + 255) & ~(size_t)255);
+        pack_layout(jobs.data() + cut[c], cut[c + 1] - cut[c], o.row_mode == FARCUR_ROWS_PIVOT, pk[c], B->pack_sc[c]);
+        off[c + 1] = off[c] + pk[c].size;  // sizes are multiples of 256
     }
     const size_t plan_bytes = off[nchunks];
-    B->h_plan.resize(plan_bytes);
-    for (int c = 0; c < nchunks; ++c) std::memcpy(B->h_plan.data() + off[c], pk[c].bytes.data(), pk[c].bytes.size());
+    if (B->h_plan.size() < plan_bytes) B->h_plan.resize(plan_bytes);
+    for (int c = 0; c < nchunks; ++c)
+        pack_write(jobs.data() + cut[c], cut[c + 1] - cut[c], pk[c], B->pack_sc[c], B->h_plan.data() + off[c]);
     hipStream_t s0 = ctx->stream, s1 = ctx->stream2;
     // Launch parameters that the kernels take by value.
     struct KeyParams {
@@ -1700,6 +1730,8 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
         return FARCUR_OK;
     };
     outs.resize(nj);
+    const auto t_plan1 = std::chrono::steady_clock::now();
+    B->host_ms[1] += std::chrono::duration<double, std::milli>(t_plan1 - t_plan0).count();
     if (!B->use_graphs || B->profile) {
         TRY(B->d_plan.ensure(plan_bytes));
         TRY(enqueue(B->h_plan.data(), B->d_plan.p, outs.data()));
@@ -1760,6 +1792,7 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
         HIPTRY(hipStreamSynchronize(s0));
         std::memcpy(outs.data(), g->h_out, (size_t)nj * sizeof(JobOut));
     }
+    B->host_ms[2] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan1).count();
     B->kt.collect();
     if (B->profile) B->rounds.push_back(std::move(jobs));
     return FARCUR_OK;
@@ -1888,6 +1921,7 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
     B->kt.clear();
     B->kt.on = B->profile;
     for (double& w : B->work) w = 0;
+    for (double& h : B->host_ms) h = 0;
     for (int b = 0; b < B->nb; ++b) {
         BState& s = B->st[b];
         s = BState{};
@@ -1910,6 +1944,7 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
         // every block whose held-out rows still miss tol.
         // Without a model the first round uses the reference's rule (the
         // method without the network, for comparison).
+        const auto tp0 = std::chrono::steady_clock::now();
         std::vector<int32_t> cap(B->nb), rp(B->nb);
         for (int b = 0; b < B->nb; ++b) cap[b] = B->st[b].cap;
         if (B->model) {
@@ -1921,6 +1956,7 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
         } else {
             for (int b = 0; b < B->nb; ++b) rp[b] = B->st[b].r;
         }
+        B->host_ms[0] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp0).count();
         std::vector<int> active(B->nb);
         for (int b = 0; b < B->nb; ++b) {
             BState& s = B->st[b];
@@ -1933,6 +1969,7 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
         bool first = true;
         while (!active.empty()) {
             TRY(batch_round(B, active, o, bufs, outs));
+            const auto ta0 = std::chrono::steady_clock::now();
             std::vector<int> next;
             for (size_t i = 0; i < active.size(); ++i) {
                 BState& s = B->st[active[i]];
@@ -1940,6 +1977,7 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
                                 o.oversample);
                 if (s.status == 0) next.push_back(active[i]);
             }
+            B->host_ms[3] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta0).count();
             if (first) B->first_round_misses = (int)next.size();
             first = false;
             active.swap(next);
@@ -2027,12 +2065,19 @@ extern "C" int farcur_batch_kernel_times(farcur_batch* B, const char** names, do
     static const char* kNames[KTimer::kClasses] = {"sample",   "fill",     "pinv",     "residual", "pinv_r16",
                                                    "pinv_r24", "pinv_r32", "pinv_r40", "pinv_r48", "pinv_r64",
                                                    "pinv_global"};
+    static const char* kHost[4] = {"host_predict", "host_plan", "host_wait", "host_advance"};
     int n = 0;
     for (int i = 0; i < KTimer::kClasses && n < cap; ++i, ++n) {
         if (names) names[n] = kNames[i];
         if (ms) ms[n] = B->kt.ms[i];
         if (launches) launches[n] = B->kt.launches[i];
         if (work) work[n] = B->work[i];
+    }
+    for (int i = 0; i < 4 && n < cap; ++i, ++n) {  // host wall time, measured on every compress
+        if (names) names[n] = kHost[i];
+        if (ms) ms[n] = B->host_ms[i];
+        if (launches) launches[n] = 0;
+        if (work) work[n] = 0;
     }
     return n;
 }

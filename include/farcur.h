@@ -319,8 +319,12 @@ int farcur_batch_matvec(farcur_batch* batch, const farcur_c128* V, int nrhs, far
 
 /* Device time (ms) and launch count of each kernel class over the last
  * batch_compress call with profiling on: names "sample", "fill", "pinv",
- * "residual", then per pinv class "pinv_r16" ... "pinv_r64", "pinv_global".
- * Returns the number of classes written (<= cap). */
+ * "residual", then per pinv class "pinv_r16" ... "pinv_r64", "pinv_global",
+ * then the host wall time of the last compress (measured with or without
+ * profiling): "host_predict" (rank inference round trip), "host_plan" (round
+ * planning and upload staging), "host_wait" (waiting for the device),
+ * "host_advance" (block state updates). Returns the number of entries written
+ * (<= cap). */
 int farcur_batch_profile(farcur_batch* batch, int enable);
 int farcur_batch_kernel_times(farcur_batch* batch, const char** names, double* ms, int* launches,
                               double* evals, int cap);
