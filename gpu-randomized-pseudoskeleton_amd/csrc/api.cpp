@@ -226,6 +226,8 @@ struct farcur_ctx {
     std::vector<int32_t> h_topol, h_edges;
     std::vector<double> h_ln;
     DBuf rec64, rec32, cen;
+    DBuf tri32, tcen, lrec32, rtri;  // triangle tables of the FP32 fill
+    std::vector<double> h_cen;       // RWG centres (kCenStride), for the fill's tile order
     uint64_t geom_hash = 0;
     // single-block scratch
     DBuf d_rows, d_cols, d_cs, d_rs, d_rpos, d_C, d_R, d_W, d_V, d_U, d_sigma, d_v, d_y0, d_y1, d_plan, d_out;
@@ -246,6 +248,7 @@ struct farcur_ctx {
         FillParams fp;
         fp.rec = f32 ? rec32.p : rec64.p;
         fp.cen = cen.as<double>();
+        fp.tri = TriGeom{tri32.as<float>(), tcen.as<double>(), lrec32.as<float>(), rtri.as<int32_t>()};
         fp.ne = ne;
         fp.f32 = f32;
         fp.k = k;
@@ -299,6 +302,88 @@ void build_records(const std::vector<double>& vertex, const std::vector<int32_t>
     }
 }
 
+// Triangle tables of the FP32 fill: per triangle the quadrature points as
+// offsets from its centroid and their squared norms (FP32, kTriStride) and the
+// centroid (FP64, kCenStride); per RWG the free vertex of each half relative
+// to that triangle's centroid and the edge length (FP32, kLaneStride), and the
+// two triangle ids.
+void build_triangles(const std::vector<double>& vertex, const std::vector<int32_t>& topol,
+                     const std::vector<int32_t>& edges, const std::vector<double>& ln, std::vector<float>& tri,
+                     std::vector<double>& tcen, std::vector<float>& lrec, std::vector<int32_t>& rtri) {
+    const int nt = (int)(topol.size() / 3), ne = (int)ln.size();
+    const double b[3][3] = {{2.0 / 3, 1.0 / 6, 1.0 / 6}, {1.0 / 6, 2.0 / 3, 1.0 / 6}, {1.0 / 6, 1.0 / 6, 2.0 / 3}};
+    tri.assign((size_t)kTriStride * nt, 0.0f);
+    tcen.assign((size_t)kCenStride * nt, 0.0);
+    for (int t = 0; t < nt; ++t) {
+        double q[3][3], c[3] = {0, 0, 0};
+        for (int i = 0; i < 3; ++i)
+            for (int x = 0; x < 3; ++x) {
+                q[i][x] = b[i][0] * vertex[3 * topol[3 * t] + x] + b[i][1] * vertex[3 * topol[3 * t + 1] + x] +
+                          b[i][2] * vertex[3 * topol[3 * t + 2] + x];
+                c[x] += q[i][x] / 3.0;
+            }
+        float* T = &tri[(size_t)t * kTriStride];
+        for (int i = 0; i < 3; ++i) {
+            double uu = 0;
+            for (int x = 0; x < 3; ++x) {
+                const double u = q[i][x] - c[x];
+                T[i * 3 + x] = (float)u;
+                uu += u * u;
+            }
+            T[9 + i] = (float)uu;
+        }
+        for (int x = 0; x < 3; ++x) tcen[(size_t)t * kCenStride + x] = c[x];
+    }
+    lrec.assign((size_t)kLaneStride * ne, 0.0f);
+    rtri.assign(2 * (size_t)ne, 0);
+    for (int e = 0; e < ne; ++e) {
+        float* L = &lrec[(size_t)e * kLaneStride];
+        for (int h = 0; h < 2; ++h) {
+            const int t = edges[4 * e + h], vf = edges[4 * e + 2 + h];
+            rtri[2 * (size_t)e + h] = t;
+            for (int x = 0; x < 3; ++x) L[h * 3 + x] = (float)(vertex[3 * vf + x] - tcen[(size_t)t * kCenStride + x]);
+        }
+        L[6] = (float)ln[e];
+    }
+}
+
+// Tile order of an RWG list for the FP32 fill: positions 0..n-1 arranged so
+// that each run of kTileRows consecutive entries is a compact patch (the
+// tile's RWGs then share most of their triangles). Recursive bisection along
+// the longest extent of the RWG centres, split at the multiple of kTileRows
+// nearest the middle; runs of at most kTileRows keep the list order.
+void tile_order(const int32_t* ids, int n, const double* cen, int32_t* perm) {
+    std::vector<int32_t> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    struct Range {
+        int lo, hi;
+    };
+    std::vector<Range> stack{{0, n}};
+    while (!stack.empty()) {
+        const Range rg = stack.back();
+        stack.pop_back();
+        const int len = rg.hi - rg.lo;
+        if (len <= kTileRows) continue;
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int i = rg.lo; i < rg.hi; ++i)
+            for (int x = 0; x < 3; ++x) {
+                const double v = cen[(size_t)ids[idx[i]] * kCenStride + x];
+                lo[x] = std::min(lo[x], v);
+                hi[x] = std::max(hi[x], v);
+            }
+        int ax = 0;
+        for (int x = 1; x < 3; ++x)
+            if (hi[x] - lo[x] > hi[ax] - lo[ax]) ax = x;
+        std::stable_sort(idx.begin() + rg.lo, idx.begin() + rg.hi, [&](int32_t a, int32_t b) {
+            return cen[(size_t)ids[a] * kCenStride + ax] < cen[(size_t)ids[b] * kCenStride + ax];
+        });
+        const int k = std::max(kTileRows, (len / 2 + kTileRows / 2) / kTileRows * kTileRows);
+        stack.push_back({rg.lo, rg.lo + k});
+        stack.push_back({rg.lo + k, rg.hi});
+    }
+    std::copy(idx.begin(), idx.end(), perm);
+}
+
 uint64_t fnv1a(uint64_t h, const void* data, size_t n) {
     const unsigned char* p = (const unsigned char*)data;
     // 8 bytes at a time for speed; still position-sensitive.
@@ -337,7 +422,23 @@ int set_geometry_impl(farcur_ctx* ctx, const std::vector<double>& vertex, const 
     std::vector<double> rec, cen;
     std::vector<float> rec32;
     build_records(vertex, topol, edges, ln, rec, rec32, cen);
+    std::vector<float> tri32, lrec32;
+    std::vector<double> tcen;
+    std::vector<int32_t> rtri;
+    build_triangles(vertex, topol, edges, ln, tri32, tcen, lrec32, rtri);
     TRY(ctx->select());
+    TRY(ctx->tri32.ensure(tri32.size() * sizeof(float)));
+    TRY(ctx->tcen.ensure(tcen.size() * sizeof(double)));
+    TRY(ctx->lrec32.ensure(lrec32.size() * sizeof(float)));
+    TRY(ctx->rtri.ensure(rtri.size() * sizeof(int32_t)));
+    HIPTRY(hipMemcpyAsync(ctx->tri32.p, tri32.data(), tri32.size() * sizeof(float), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPTRY(hipMemcpyAsync(ctx->tcen.p, tcen.data(), tcen.size() * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPTRY(hipMemcpyAsync(ctx->lrec32.p, lrec32.data(), lrec32.size() * sizeof(float), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPTRY(hipMemcpyAsync(ctx->rtri.p, rtri.data(), rtri.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                          ctx->stream));
     TRY(ctx->rec64.ensure(rec.size() * sizeof(double)));
     TRY(ctx->rec32.ensure(rec32.size() * sizeof(float)));
     TRY(ctx->cen.ensure(cen.size() * sizeof(double)));
@@ -348,6 +449,7 @@ int set_geometry_impl(farcur_ctx* ctx, const std::vector<double>& vertex, const 
     HIPTRY(hipStreamSynchronize(ctx->stream));
     ctx->ne = ne;
     ctx->nt = nt;
+    ctx->h_cen = cen;
     ctx->h_vertex = vertex;
     ctx->h_topol = topol;
     ctx->h_edges = edges;
@@ -708,7 +810,7 @@ void farcur_destroy(farcur_ctx* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    DBuf* bufs[] = {&ctx->rec64, &ctx->rec32, &ctx->cen, &ctx->d_rows, &ctx->d_cols, &ctx->d_cs,    &ctx->d_rs, &ctx->d_rpos,
+    DBuf* bufs[] = {&ctx->rec64, &ctx->rec32, &ctx->cen, &ctx->tri32, &ctx->tcen, &ctx->lrec32, &ctx->rtri, &ctx->d_rows, &ctx->d_cols, &ctx->d_cs,    &ctx->d_rs, &ctx->d_rpos,
                     &ctx->d_C,   &ctx->d_R,   &ctx->d_W,    &ctx->d_V,    &ctx->d_U,     &ctx->d_sigma,
                     &ctx->d_v,   &ctx->d_y0,  &ctx->d_y1,   &ctx->d_plan, &ctx->d_out};
     for (DBuf* b : bufs) b->release();
@@ -1434,7 +1536,7 @@ struct farcur_batch {
     farcur_ctx* ctx = nullptr;
     int nb = 0;
     std::vector<int32_t> row_off, col_off, row_ids, col_ids;
-    DBuf d_rows, d_cols, d_v, d_ya, d_yb, d_vo, d_plan;
+    DBuf d_rows, d_cols, d_rperm, d_cperm, d_v, d_ya, d_yb, d_vo, d_plan;
     Arena arena;
     std::vector<BState> st;
     KTimer kt;
@@ -1540,6 +1642,8 @@ BlockJob make_job(farcur_batch* B, int b, const RoundBufs& rb, const farcur_opts
     J.U = rb.U;
     J.sigma = rb.sigma;
     J.v = B->d_v.as<c128>() + B->col_off[b];
+    J.row_perm = B->d_rperm.as<int32_t>() + B->row_off[b];
+    J.col_perm = B->d_cperm.as<int32_t>() + B->col_off[b];
     c128* ya = B->d_ya.as<c128>() + B->row_off[b];
     c128* yb = B->d_yb.as<c128>() + B->row_off[b];
     J.y_old = s.ycur ? yb : ya;
@@ -1862,6 +1966,18 @@ extern "C" int farcur_batch_create(farcur_ctx* ctx, int nblocks, const int* row_
     B->col_ids.assign(col_ids, col_ids + nc);
     TRY(B->d_rows.ensure((size_t)nr * 4));
     TRY(B->d_cols.ensure((size_t)nc * 4));
+    {
+        std::vector<int32_t> rp(nr), cp(nc);
+        for (int b = 0; b < nblocks; ++b) {
+            tile_order(row_ids + row_off[b], row_off[b + 1] - row_off[b], ctx->h_cen.data(), rp.data() + row_off[b]);
+            tile_order(col_ids + col_off[b], col_off[b + 1] - col_off[b], ctx->h_cen.data(), cp.data() + col_off[b]);
+        }
+        TRY(B->d_rperm.ensure((size_t)nr * 4));
+        TRY(B->d_cperm.ensure((size_t)nc * 4));
+        HIPTRY(hipMemcpyAsync(B->d_rperm.p, rp.data(), (size_t)nr * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIPTRY(hipMemcpyAsync(B->d_cperm.p, cp.data(), (size_t)nc * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIPTRY(hipStreamSynchronize(ctx->stream));
+    }
     TRY(B->d_v.ensure((size_t)nc * 16));
     TRY(B->d_vo.ensure((size_t)(nblocks + 1) * 4));
     TRY(B->d_ya.ensure((size_t)nr * 16));
@@ -1888,7 +2004,7 @@ extern "C" void farcur_batch_destroy(farcur_batch* B) {
     if (!B) return;
     hipSetDevice(B->ctx->device);
     hipStreamSynchronize(B->ctx->stream);
-    DBuf* bufs[] = {&B->d_rows, &B->d_cols, &B->d_v,    &B->d_ya,  &B->d_yb,
+    DBuf* bufs[] = {&B->d_rows, &B->d_cols, &B->d_rperm, &B->d_cperm, &B->d_v,    &B->d_ya,  &B->d_yb,
                     &B->d_vo,   &B->d_plan, &B->d_feat, &B->d_cap, &B->d_rpred};
     for (DBuf* b : bufs) b->release();
     for (RoundGraph& g : B->graphs) g.release();

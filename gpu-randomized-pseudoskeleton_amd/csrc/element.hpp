@@ -97,21 +97,29 @@ __device__ __forceinline__ void entry_f64(const RwgRec<double>& a, const double*
 }
 
 // ---------------------------------------------------------------- FP32
-// With R0 = |c_m - c_n| and delta = R - R0 for each point pair,
-//   e^{-jkR}/R = e^{-jkR0} g,   g = e^{-jk delta} / R,
-// and because sum_{a,b} s_a s_b = 0 the charge (div-div) term may use
-//   g - 1/R0 = [(cos k delta - 1) - j sin k delta] / R - delta / (R R0)
-// instead of g: the near-cancelling second difference of the scalar potential
-// is then formed from small quantities only. delta = X / (R + R0) with
-// X = R^2 - R0^2 = 2 d.D + |D|^2 (d = c_m - c_n, D = u_a - u_b) is exact to
-// rounding. Sines take revolutions (hardware sine), arguments |delta / lambda| < 1.
+// The FP32 fill shares work between RWGs that share a triangle. For a test
+// triangle a (of test RWGs m, sign s_a, free vertex v_ma) and a source RWG n
+// (six points q_j, rho_nj = s_b (q_j - v_nb) with signs), write the test
+// points q_ai = c_a + u_ai (c_a the centroid) and w_ma = v_ma - c_a. Then
+//   Z_mn = -j kappa_mn sum_{a in {a+, a-}} s_a [A(a,n) - w_ma . B(a,n) - (4/k^2) C(a,n)],
+//   T_i = sum_j rho_nj G_ij,  A = sum_i u_ai . T_i,  B = sum_i T_i,
+//   C = sum_{i,j} s_b(j) (G_ij - e^{-jkR0} / R0),
+// with kappa_mn = k eta l_m l_n / (144 pi), G_ij = e^{-jkR_ij} / R_ij (the
+// areas cancel against the 1/3 weights as in the FP64 element), and R0 =
+// |c_a - c_n| (c_n the RWG centre). The subtracted term cancels in C because
+// sum_j s_b(j) = 0; it keeps the source-side difference of the charge term
+// formed from small quantities. A, B, C of one (triangle, RWG) item serve
+// every test RWG on that triangle, so a tile of 64 test RWGs evaluates its
+// distinct triangles (about 0.9 per RWG on a mesh patch) instead of four
+// triangle pairs per entry.
 //
-// The point-pair loop is VALU-issue bound, so everything that depends on one
-// point only is hoisted to the entry: in half-wavelength units (ihl = 1/(2 lambda)),
-//   X ihl = alpha_i + beta_j - 2 u_ai . u'_bj,
-//   alpha_i = (2 d.u_ai + |u_ai|^2) ihl,  beta_j = (|u_bj|^2 - 2 d.u_bj) ihl,  u'_bj = u_bj ihl,
-// and with th = delta / (2 lambda) (so k delta = 2 pi (2 th) in revolutions):
-//   cos(k delta) / R = (1 - 2 sin^2 th) / R,   sin(k delta) / R,
+// Per point pair, with delta = R - R0 = X / (R + R0), X = R^2 - R0^2 =
+// 2 d.D + |D|^2 (d = c_a - c_n, D = u_ai - u_j, u_j = q_j - c_n) exact to
+// rounding, and G = e^{-jkR0} g, g = e^{-jk delta} / R:
+//   X ihl = alpha_i + beta_j - 2 ihl u_ai . u_j  (ihl = 1/(2 lambda)),
+//   alpha_i = (2 d.u_ai + |u_ai|^2) ihl,  beta_j = (|u_j|^2 - 2 d.u_j) ihl,
+//   th = delta / (2 lambda) (sines take revolutions, so k delta = 2 pi (2 th)),
+//   Re g = (1 - 2 sin^2 th) / R,  -Im g = sin(2 pi 2 th) / R,
 //   -Re(g - 1/R0) = 2 sin^2 th / R + th (2 lambda / R0) / R.
 struct F32Consts {
     float inv_lambda;  // k / (2 pi)
@@ -120,104 +128,47 @@ struct F32Consts {
     double inv_lambda64;
 };
 
-__device__ __forceinline__ void entry_f32(const RwgRec<float>& a, const double* __restrict__ ac3,
-                                          const float* __restrict__ b, const double* __restrict__ bc3,
-                                          const F32Consts& fc, float& zr, float& zi) {
-    const double dx64 = ac3[0] - bc3[0], dy64 = ac3[1] - bc3[1], dz64 = ac3[2] - bc3[2];
-    const double r064 = sqrt(dx64 * dx64 + dy64 * dy64 + dz64 * dz64);
-    const double ph = r064 * fc.inv_lambda64;
-    const float f0 = (float)(ph - rint(ph));
-    const float s0 = __builtin_amdgcn_sinf(f0), c0 = __builtin_amdgcn_cosf(f0);
-    const float dx = (float)dx64, dy = (float)dy64, dz = (float)dz64;
-    const float r0 = (float)r064, r0sq = r0 * r0;
-    const float ihl = 0.5f * fc.inv_lambda;  // 1 / (2 lambda)
-    const float twol = 1.0f / ihl;           // 2 lambda
-    const float c2 = twol / r0;              // 2 lambda / R0
-    float alpha[2][3], beta[2][3], ub[2][3][3];
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const float* q = a.q[h][i];
-            const float du = dx * q[0] + dy * q[1] + dz * q[2];
-            const float uu = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
-            alpha[h][i] = (2.0f * du + uu) * ihl;
-            const float* u = b + h * 9 + i * 3;
-            const float dv = dx * u[0] + dy * u[1] + dz * u[2];
-            const float vv = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-            beta[h][i] = (vv - 2.0f * dv) * ihl;
-            ub[h][i][0] = u[0] * ihl;
-            ub[h][i][1] = u[1] * ihl;
-            ub[h][i][2] = u[2] * ihl;
-        }
-    float ar = 0, ai = 0;  // ai accumulates -Im
-#pragma unroll
-    for (int hb = 0; hb < 2; ++hb) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const float px = b[18 + hb * 9 + j * 3 + 0], py = b[18 + hb * 9 + j * 3 + 1];
-            const float pz = b[18 + hb * 9 + j * 3 + 2];
-#pragma unroll
-            for (int ha = 0; ha < 2; ++ha) {
-                const float ss = (ha == hb) ? fc.c4k2 : -fc.c4k2;
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    const float w = a.q[ha][i][0] * ub[hb][j][0] + a.q[ha][i][1] * ub[hb][j][1] +
-                                    a.q[ha][i][2] * ub[hb][j][2];
-                    const float xl = fmaf(-2.0f, w, alpha[ha][i] + beta[hb][j]);  // X / (2 lambda)
-                    const float r2 = fmaf(xl, twol, r0sq);
-                    const float ir = __builtin_amdgcn_rsqf(r2);
-                    const float th = xl * __builtin_amdgcn_rcpf(fmaf(r2, ir, r0));  // delta / (2 lambda)
-                    const float sh = __builtin_amdgcn_sinf(th);
-                    const float sn = __builtin_amdgcn_sinf(th + th);
-                    const float v = sh * sh * ir;
-                    const float gr = fmaf(-2.0f, v, ir);                  // Re g
-                    const float hneg = fmaf(th * ir, c2, v + v);          // -Re(g - 1/R0)
-                    const float gin = sn * ir;                            // -Im g
-                    const float term = a.p[ha][i][0] * px + a.p[ha][i][1] * py + a.p[ha][i][2] * pz;
-                    // term * g - ss * (g - 1/R0)
-                    ar = fmaf(ss, hneg, fmaf(term, gr, ar));
-                    ai = fmaf(term - ss, gin, ai);
-                }
-            }
-        }
-    }
-    ai = -ai;
-    // Z = -j kappa e^{-jkR0} (ar + j ai)
-    const float pr = c0 * ar + s0 * ai, pi = c0 * ai - s0 * ar;
-    const float kap = fc.scale * a.l * b[36];
-    zr = kap * pi;
-    zi = -kap * pr;
-}
+// Triangle tables (built with the RWG records):
+//   tri[t * kTriStride + i * 3 + c]  u_ti = q_ti - c_t (quadrature points of
+//                                    triangle t relative to its centroid)
+//   tri[t * kTriStride + 9 + i]      |u_ti|^2
+//   tcen[t * kCenStride + c]         c_t (FP64)
+//   lrec[e * kLaneStride + h * 3 + c] w_eh = v_eh - c_{t_eh} (free vertex of
+//                                    RWG e in its half h, from that centroid)
+//   lrec[e * kLaneStride + 6]        l_e
+//   rtri[2 e + h]                    t_eh, the triangle of half h (0: T+, 1: T-)
+constexpr int kTriStride = 12;
+constexpr int kLaneStride = 8;
+struct TriGeom {
+    const float* tri;
+    const double* tcen;
+    const float* lrec;
+    const int32_t* rtri;
+};
 
-// Tile geometry of the panel fill: a workgroup of 4 waves covers 64 rows x up
-// to 64 columns; lane = row, and the waves take the tile's columns round-robin
-// (wave w: columns w, w + 4, ...). Sampled panels have r of a few tens of
-// columns, so one tile spans all of them and the waves stay balanced (a 16-
-// column tile left whole waves idle on the last partial column group). The
-// column (source) records of the tile are staged in LDS and read as broadcasts.
+// Tile geometry of the panel fill: a workgroup of 4 waves covers 64 RWGs of
+// one index set (the lane set: rows for the tall panels C and W, columns for
+// the wide panels R and H, using reciprocity Z(m, n) = Z(n, m) of the
+// symmetric Galerkin element) times up to 64 RWGs of the other set (the LDS
+// set), whose records are staged in LDS and read as broadcasts.
 constexpr int kTileRows = 64;
 constexpr int kTileCols = 64;
 constexpr int kFillWaves = 4;
 constexpr int kFillThreads = kFillWaves * 64;
 
-template <typename T>
+// ---------------------------------------------------------------- FP64 tile
 struct FillTileLds {
-    T col[kTileCols][kRecFields];
-    double cen[kTileCols][3];
+    double col[kTileCols][kRecFields];
 };
 
-// One tile of a panel out[i + j*ld] = Z(rows[i], cols[j]). Lanes run over
-// 64 consecutive entries of one index set (the lane set), up to 64 entries of
-// the other set (the LDS set) are staged in LDS and read as broadcasts.
-// kLanesAreRows: lanes over rows (tall panels, C and W); otherwise lanes over
-// columns (wide panels R and H), using reciprocity Z(m, n) = Z(n, m) of the
-// symmetric Galerkin element. Must be called by all 256 threads.
-template <typename T, typename OutC, typename Consts, bool kLanesAreRows>
+// One tile of a panel out[i + j*ld] = Z(rows[i], cols[j]) in FP64: lane =
+// entry of the lane set, the waves take the LDS set round-robin. Must be
+// called by all 256 threads.
+template <typename OutC, bool kLanesAreRows>
 __device__ __forceinline__ void fill_tile(OutC* __restrict__ out, int ld, const int* __restrict__ rows, int nr,
                                           const int* __restrict__ cols, int nc, int row0, int col0,
-                                          const T* __restrict__ rec, const double* __restrict__ cen, int ne,
-                                          const Consts& fc, FillTileLds<T>& lds) {
+                                          const double* __restrict__ rec, double k, double c4k2, double scale,
+                                          FillTileLds& lds) {
     const int* lane_ids = kLanesAreRows ? rows : cols;
     const int* lds_ids = kLanesAreRows ? cols : rows;
     const int nlane = kLanesAreRows ? nr : nc;
@@ -231,38 +182,265 @@ __device__ __forceinline__ void fill_tile(OutC* __restrict__ out, int ld, const 
         const int c = idx / kRecFields, f = idx - c * kRecFields;
         lds.col[c][f] = rec[(size_t)lds_ids[lds0 + c] * kRecStride + f];
     }
-    if constexpr (sizeof(T) == 4) {
-        for (int idx = tid; idx < ncol * 3; idx += kFillThreads) {
-            const int c = idx / 3, x = idx - 3 * c;
-            lds.cen[c][x] = cen[(size_t)lds_ids[lds0 + c] * kCenStride + x];
-        }
-    }
     __syncthreads();
     const int lane = tid & (kWave - 1);
     const int w = tid / kWave;
     const int i = lane0 + lane;
     if (i >= nlane) return;
-    const int e = lane_ids[i];
-    RwgRec<T> a;
-    load_rec(rec, e, a);
-    double ac3[3] = {0, 0, 0};
-    if constexpr (sizeof(T) == 4) {
-        ac3[0] = cen[(size_t)e * kCenStride];
-        ac3[1] = cen[(size_t)e * kCenStride + 1];
-        ac3[2] = cen[(size_t)e * kCenStride + 2];
-    }
+    RwgRec<double> a;
+    load_rec(rec, lane_ids[i], a);
 #pragma unroll 1
     for (int c = w; c < ncol; c += kFillWaves) {
         const int j = lds0 + c;
-        T zr, zi;
-        if constexpr (sizeof(T) == 8)
-            entry_f64(a, lds.col[c], fc.k, fc.c4k2, fc.scale, zr, zi);
-        else
-            entry_f32(a, ac3, lds.col[c], lds.cen[c], fc, zr, zi);
+        double zr, zi;
+        entry_f64(a, lds.col[c], k, c4k2, scale, zr, zi);
         if constexpr (kLanesAreRows)
             store_c(&out[(size_t)j * ld + i], zr, zi);
         else
             store_c(&out[(size_t)i * ld + j], zr, zi);
+    }
+}
+
+// ---------------------------------------------------------------- FP32 tile
+// The tile's distinct lane-set triangles (at most two per RWG) are found with
+// an LDS hash table. Items (triangle a, LDS-set RWG c), numbered it = c nt + a
+// over the tile's nt triangles, are evaluated 256 per round (one per thread)
+// into a ring of kTriRing moment slots; after each round every column whose
+// items are all done is assembled, each entry gathering the items of its two
+// triangles. A column's unassembled items start less than nt <= kTriMax items
+// before the round's end, so a ring of 256 + kTriMax slots is never
+// overwritten before it is read.
+constexpr int kTriMax = 2 * kTileRows;
+constexpr int kTriHash = 256;                    // open addressing, load factor <= 1/2
+constexpr int kTriRing = kFillThreads + kTriMax;  // moment slots
+constexpr int kSrcFields = 44;   // u_j (18), rho_j (18), l (1), |u_j|^2 (6), pad
+constexpr int kMoments = 10;     // A, B (3), C: complex
+
+struct FillTriLds {
+    float src[kTileCols][kSrcFields];
+    double srcc[kTileCols][3];
+    float tu[9][kTriMax];   // u_ti of the distinct triangles (structure of arrays)
+    float tuu[3][kTriMax];  // |u_ti|^2
+    double tc[3][kTriMax];  // centroids
+    int hkey[kTriHash];
+    int hval[kTriHash];
+    int tid_of[kTriMax];    // triangle id of each local triangle
+    int loc[kTriMax];       // local triangle of (lane RWG i, half h) at 2 i + h
+    int ntri;
+    float mom[kMoments][kTriRing];
+};
+
+// Moments A, B, C of item (local triangle a, LDS-set RWG c), already rotated
+// by e^{-jkR0}, into L.mom[.][it].
+__device__ __forceinline__ void tri_item(FillTriLds& L, int a, int c, int it, const F32Consts& fc) {
+    const float* S = L.src[c];
+    const double dx64 = L.tc[0][a] - L.srcc[c][0], dy64 = L.tc[1][a] - L.srcc[c][1],
+                 dz64 = L.tc[2][a] - L.srcc[c][2];
+    const double r064 = sqrt(dx64 * dx64 + dy64 * dy64 + dz64 * dz64);
+    const double ph = r064 * fc.inv_lambda64;
+    const float f0 = (float)(ph - rint(ph));
+    const float s0 = __builtin_amdgcn_sinf(f0), c0 = __builtin_amdgcn_cosf(f0);
+    const float dx = (float)dx64, dy = (float)dy64, dz = (float)dz64;
+    const float r0 = (float)r064, r0sq = r0 * r0;
+    const float ihl = 0.5f * fc.inv_lambda;  // 1 / (2 lambda)
+    const float twol = 1.0f / ihl;           // 2 lambda
+    const float c2 = twol / r0;              // 2 lambda / R0
+    const float m2ihl = -2.0f * ihl;
+    float ua[3][3], alpha[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        ua[i][0] = L.tu[i * 3 + 0][a];
+        ua[i][1] = L.tu[i * 3 + 1][a];
+        ua[i][2] = L.tu[i * 3 + 2][a];
+        const float du = dx * ua[i][0] + dy * ua[i][1] + dz * ua[i][2];
+        alpha[i] = (2.0f * du + L.tuu[i][a]) * ihl;
+    }
+    float tr[3][3] = {}, tn[3][3] = {};  // T_i = tr - j tn
+    float chn = 0, cgn = 0;              // C = -(chn + j cgn)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const float ux = S[j * 3 + 0], uy = S[j * 3 + 1], uz = S[j * 3 + 2];
+        const float px = S[18 + j * 3 + 0], py = S[18 + j * 3 + 1], pz = S[18 + j * 3 + 2];
+        const float beta = fmaf(m2ihl, dx * ux + dy * uy + dz * uz, S[37 + j] * ihl);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float wd = ua[i][0] * ux + ua[i][1] * uy + ua[i][2] * uz;
+            const float xl = fmaf(m2ihl, wd, alpha[i] + beta);  // X / (2 lambda)
+            const float r2 = fmaf(xl, twol, r0sq);
+            const float ir = __builtin_amdgcn_rsqf(r2);
+            const float th = xl * __builtin_amdgcn_rcpf(fmaf(r2, ir, r0));  // delta / (2 lambda)
+            const float sh = __builtin_amdgcn_sinf(th);
+            const float sn = __builtin_amdgcn_sinf(th + th);
+            const float v = sh * sh * ir;
+            const float gr = fmaf(-2.0f, v, ir);          // Re g
+            const float hneg = fmaf(th * ir, c2, v + v);  // -Re(g - 1/R0)
+            const float gin = sn * ir;                    // -Im g
+            tr[i][0] = fmaf(px, gr, tr[i][0]);
+            tr[i][1] = fmaf(py, gr, tr[i][1]);
+            tr[i][2] = fmaf(pz, gr, tr[i][2]);
+            tn[i][0] = fmaf(px, gin, tn[i][0]);
+            tn[i][1] = fmaf(py, gin, tn[i][1]);
+            tn[i][2] = fmaf(pz, gin, tn[i][2]);
+            if (j < 3) {
+                chn += hneg;
+                cgn += gin;
+            } else {
+                chn -= hneg;
+                cgn -= gin;
+            }
+        }
+    }
+    // A = sum_i u_ai . T_i, B = sum_i T_i, C; then times e^{-jkR0} = c0 - j s0.
+    float ar = 0, an = 0, br[3], bn[3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        br[x] = tr[0][x] + tr[1][x] + tr[2][x];
+        bn[x] = tn[0][x] + tn[1][x] + tn[2][x];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            ar = fmaf(ua[i][x], tr[i][x], ar);
+            an = fmaf(ua[i][x], tn[i][x], an);
+        }
+    }
+    // (xr - j xn)(c0 - j s0) = (xr c0 - xn s0) - j (xn c0 + xr s0)
+    auto rot = [&](float xr, float xn, int f) {
+        L.mom[f][it] = xr * c0 - xn * s0;
+        L.mom[f + 1][it] = -(xn * c0 + xr * s0);
+    };
+    rot(ar, an, 0);
+    rot(br[0], bn[0], 2);
+    rot(br[1], bn[1], 4);
+    rot(br[2], bn[2], 6);
+    rot(-chn, cgn, 8);  // C = -chn - j cgn
+}
+
+// One tile of a panel, FP32 (panels out of OutC = c64). Same tile geometry
+// and arguments as fill_tile; lane_perm (or NULL for the list order) orders the
+// lane set so that a tile's 64 RWGs form a compact patch sharing triangles:
+// tile lane i takes position lane_perm[lane0 + i] of the lane set. Must be
+// called by all 256 threads.
+template <typename OutC, bool kLanesAreRows>
+__device__ __forceinline__ void fill_tile_tri(OutC* __restrict__ out, int ld, const int* __restrict__ rows, int nr,
+                                              const int* __restrict__ cols, int nc, int row0, int col0,
+                                              const int32_t* __restrict__ lane_perm, const float* __restrict__ rec,
+                                              const double* __restrict__ cen, const TriGeom& tg,
+                                              const F32Consts& fc, FillTriLds& L) {
+    const int* lane_ids = kLanesAreRows ? rows : cols;
+    const int* lds_ids = kLanesAreRows ? cols : rows;
+    const int nlane = kLanesAreRows ? nr : nc;
+    const int nlds = kLanesAreRows ? nc : nr;
+    const int lane0 = kLanesAreRows ? row0 : col0;
+    const int lds0 = kLanesAreRows ? col0 : row0;
+    const int tid = threadIdx.x;
+    const int nl = min(kTileRows, nlane - lane0);
+    const int ncol = min(kTileCols, nlds - lds0);
+    __syncthreads();  // the LDS of a previous tile may still be read
+    for (int idx = tid; idx < ncol * kRecFields; idx += kFillThreads) {
+        const int c = idx / kRecFields, f = idx - c * kRecFields;
+        L.src[c][f] = rec[(size_t)lds_ids[lds0 + c] * kRecStride + f];
+    }
+    for (int idx = tid; idx < ncol * 6; idx += kFillThreads) {
+        const int c = idx / 6, j = idx - 6 * c;
+        const float* u = rec + (size_t)lds_ids[lds0 + c] * kRecStride + j * 3;
+        L.src[c][37 + j] = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    }
+    for (int idx = tid; idx < ncol * 3; idx += kFillThreads) {
+        const int c = idx / 3, x = idx - 3 * c;
+        L.srcc[c][x] = cen[(size_t)lds_ids[lds0 + c] * kCenStride + x];
+    }
+    for (int idx = tid; idx < kTriHash; idx += kFillThreads) L.hkey[idx] = -1;
+    if (tid == 0) L.ntri = 0;
+    __syncthreads();
+    // Distinct triangles of the lane set: slot 2 i + h inserts the triangle of
+    // half h of lane RWG i; the first inserter numbers it. The numbering
+    // depends on timing, the results do not (every item is evaluated alike).
+    int hslot = 0;
+    auto lane_pos = [&](int i) { return lane_perm ? lane_perm[lane0 + i] : lane0 + i; };
+    if (tid < 2 * nl) {
+        const int t = tg.rtri[2 * (size_t)lane_ids[lane_pos(tid >> 1)] + (tid & 1)];
+        unsigned h = ((unsigned)t * 2654435761u) >> 24;
+        while (true) {
+            const int old = atomicCAS(&L.hkey[h], -1, t);
+            if (old == -1) {
+                const int k = atomicAdd(&L.ntri, 1);
+                L.hval[h] = k;
+                L.tid_of[k] = t;
+                break;
+            }
+            if (old == t) break;
+            h = (h + 1) & (kTriHash - 1);
+        }
+        hslot = (int)h;
+    }
+    __syncthreads();
+    if (tid < 2 * nl) L.loc[tid] = L.hval[hslot];
+    const int nt = L.ntri;
+    for (int idx = tid; idx < nt * kTriStride; idx += kFillThreads) {
+        const int a = idx / kTriStride, f = idx - a * kTriStride;
+        const float v = tg.tri[(size_t)L.tid_of[a] * kTriStride + f];
+        if (f < 9)
+            L.tu[f][a] = v;
+        else
+            L.tuu[f - 9][a] = v;
+    }
+    for (int idx = tid; idx < nt * 3; idx += kFillThreads) {
+        const int a = idx / 3, x = idx - 3 * a;
+        L.tc[x][a] = tg.tcen[(size_t)L.tid_of[a] * kCenStride + x];
+    }
+    __syncthreads();
+    const int lane = tid & (kWave - 1);
+    const int w = tid / kWave;
+    const bool has_lane = lane < nl;
+    int la[2] = {0, 0}, i = 0;
+    float wv[2][3] = {}, lm = 0;
+    if (has_lane) {
+        la[0] = L.loc[2 * lane];
+        la[1] = L.loc[2 * lane + 1];
+        i = lane_pos(lane);
+        const float* lr = tg.lrec + (size_t)lane_ids[i] * kLaneStride;
+#pragma unroll
+        for (int x = 0; x < 6; ++x) wv[x / 3][x % 3] = lr[x];
+        lm = lr[6];
+    }
+    const int nitems = nt * ncol;
+    int cdone = 0;
+    for (int i0 = 0; i0 < nitems; i0 += kFillThreads) {
+        const int it = i0 + tid;
+        if (it < nitems) {
+            const int c = it / nt;
+            tri_item(L, it - c * nt, c, it % kTriRing, fc);
+        }
+        __syncthreads();
+        const int cnew = min(ncol, (i0 + kFillThreads) / nt);
+        if (has_lane) {
+            for (int c = cdone + w; c < cnew; c += kFillWaves) {
+                float zr = 0, zi = 0;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int q = (c * nt + la[h]) % kTriRing;
+                    const float vr = L.mom[0][q] - wv[h][0] * L.mom[2][q] - wv[h][1] * L.mom[4][q] -
+                                     wv[h][2] * L.mom[6][q] - fc.c4k2 * L.mom[8][q];
+                    const float vi = L.mom[1][q] - wv[h][0] * L.mom[3][q] - wv[h][1] * L.mom[5][q] -
+                                     wv[h][2] * L.mom[7][q] - fc.c4k2 * L.mom[9][q];
+                    if (h == 0) {
+                        zr += vr;
+                        zi += vi;
+                    } else {
+                        zr -= vr;
+                        zi -= vi;
+                    }
+                }
+                // Z = -j kappa (zr + j zi)
+                const float kap = fc.scale * lm * L.src[c][36];
+                const int j = lds0 + c;
+                if constexpr (kLanesAreRows)
+                    store_c(&out[(size_t)j * ld + i], kap * zi, -kap * zr);
+                else
+                    store_c(&out[(size_t)i * ld + j], kap * zi, -kap * zr);
+            }
+        }
+        cdone = cnew;
+        __syncthreads();
     }
 }
 

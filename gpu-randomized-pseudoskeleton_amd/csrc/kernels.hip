@@ -169,23 +169,41 @@ int fill_tiles(int m, int n, int r, int h, int panels) {
     return t;
 }
 
+// Fill constants for the wavenumber k and impedance eta.
+__device__ inline FillConsts fill_consts_f64(double k, double eta) {
+    return FillConsts{k, 4.0 / (k * k), k * eta / (144.0 * M_PI)};
+}
+__device__ inline F32Consts fill_consts_f32(double k, double eta) {
+    const double il = k / (2.0 * M_PI);
+    return F32Consts{(float)il, (float)(4.0 / (k * k)), (float)(k * eta / (144.0 * M_PI)), il};
+}
+
+// A tile of a panel in the fill's precision: FP64 per entry, FP32 through the
+// tile's distinct triangles.
 template <typename T>
-struct ConstsOf;
+struct FillLdsOf;
 template <>
-struct ConstsOf<double> {
-    using type = FillConsts;
-    static __device__ FillConsts make(double k, double eta) {
-        return FillConsts{k, 4.0 / (k * k), k * eta / (144.0 * M_PI)};
-    }
+struct FillLdsOf<double> {
+    using type = FillTileLds;
 };
 template <>
-struct ConstsOf<float> {
-    using type = F32Consts;
-    static __device__ F32Consts make(double k, double eta) {
-        const double il = k / (2.0 * M_PI);
-        return F32Consts{(float)il, (float)(4.0 / (k * k)), (float)(k * eta / (144.0 * M_PI)), il};
-    }
+struct FillLdsOf<float> {
+    using type = FillTriLds;
 };
+
+template <typename T, typename OutC, bool kLanesAreRows>
+__device__ __forceinline__ void fill_any(OutC* out, int ld, const int* rows, int nr, const int* cols, int nc,
+                                         int row0, int col0, const int32_t* lane_perm, const T* rec,
+                                         const double* cen, const TriGeom& tg, double k, double eta,
+                                         typename FillLdsOf<T>::type& lds) {
+    if constexpr (sizeof(T) == 8) {
+        const FillConsts fc = fill_consts_f64(k, eta);
+        fill_tile<OutC, kLanesAreRows>(out, ld, rows, nr, cols, nc, row0, col0, rec, fc.k, fc.c4k2, fc.scale, lds);
+    } else {
+        fill_tile_tri<OutC, kLanesAreRows>(out, ld, rows, nr, cols, nc, row0, col0, lane_perm, rec, cen, tg,
+                                           fill_consts_f32(k, eta), lds);
+    }
+}
 
 // Persistent fill over the tiles of all jobs. T = double: C, W, R in c128.
 // T = float: C, R in c64 and W widened to c128 for the FP64 pinv.
@@ -193,9 +211,8 @@ template <typename T, typename PC>
 __global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restrict__ jobs, int njobs,
                                                        const int32_t* __restrict__ prefix, int total, int panels,
                                                        const T* __restrict__ rec, const double* __restrict__ cen,
-                                                       int ne, double k, double eta) {
-    __shared__ FillTileLds<T> lds;
-    const auto fc = ConstsOf<T>::make(k, eta);
+                                                       TriGeom tg, double k, double eta) {
+    __shared__ typename FillLdsOf<T>::type lds;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         int lo = 0, hi = njobs - 1;
         while (lo < hi) {
@@ -209,8 +226,8 @@ __global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restric
         if (panels_j & kPanelC) {
             const int mt = cdiv(m, kTileRows), nC = mt * cdiv(r, kTileCols);
             if (local < nC) {
-                fill_tile<T, PC, decltype(fc), true>((PC*)J.C, m, J.rows, m, J.cs, r, (local % mt) * kTileRows,
-                                                     (local / mt) * kTileCols, rec, cen, ne, fc, lds);
+                fill_any<T, PC, true>((PC*)J.C, m, J.rows, m, J.cs, r, (local % mt) * kTileRows,
+                                      (local / mt) * kTileCols, J.row_perm, rec, cen, tg, k, eta, lds);
                 continue;
             }
             local -= nC;
@@ -220,8 +237,8 @@ __global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restric
             const int rt = cdiv(r, kTileRows);
             const int nW = rt * cdiv(r, kTileCols);
             if (local < nW) {
-                fill_tile<T, c128, decltype(fc), true>(J.W, r, J.rs, r, J.cs, r, (local % rt) * kTileRows,
-                                                       (local / rt) * kTileCols, rec, cen, ne, fc, lds);
+                fill_any<T, c128, true>(J.W, r, J.rs, r, J.cs, r, (local % rt) * kTileRows, (local / rt) * kTileCols,
+                                        nullptr, rec, cen, tg, k, eta, lds);
                 continue;
             }
             local -= nW;
@@ -229,14 +246,14 @@ __global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restric
         if (panels_j & kPanelR) {
             const int nR = nt * cdiv(r, kTileCols);
             if (local < nR) {
-                fill_tile<T, PC, decltype(fc), false>((PC*)J.R, r, J.rs, r, J.cols, n, (local / nt) * kTileCols,
-                                                      (local % nt) * kTileRows, rec, cen, ne, fc, lds);
+                fill_any<T, PC, false>((PC*)J.R, r, J.rs, r, J.cols, n, (local / nt) * kTileCols,
+                                       (local % nt) * kTileRows, J.col_perm, rec, cen, tg, k, eta, lds);
                 continue;
             }
             local -= nR;
         }
-        fill_tile<T, PC, decltype(fc), false>((PC*)J.H, J.h, J.hs, J.h, J.cols, n, (local / nt) * kTileCols,
-                                              (local % nt) * kTileRows, rec, cen, ne, fc, lds);
+        fill_any<T, PC, false>((PC*)J.H, J.h, J.hs, J.h, J.cols, n, (local / nt) * kTileCols, (local % nt) * kTileRows,
+                               J.col_perm, rec, cen, tg, k, eta, lds);
     }
 }
 
@@ -246,10 +263,10 @@ void launch_fill(const BlockJob* jobs, int njobs, const int32_t* tile_prefix, in
     if (grid <= 0 || grid > total_tiles) grid = total_tiles;
     if (fp.f32)
         hipLaunchKernelGGL((k_fill<float, c64>), dim3(grid), dim3(kFillThreads), 0, s, jobs, njobs, tile_prefix,
-                           total_tiles, panels, (const float*)fp.rec, fp.cen, fp.ne, fp.k, fp.eta);
+                           total_tiles, panels, (const float*)fp.rec, fp.cen, fp.tri, fp.k, fp.eta);
     else
         hipLaunchKernelGGL((k_fill<double, c128>), dim3(grid), dim3(kFillThreads), 0, s, jobs, njobs, tile_prefix,
-                           total_tiles, panels, (const double*)fp.rec, fp.cen, fp.ne, fp.k, fp.eta);
+                           total_tiles, panels, (const double*)fp.rec, fp.cen, fp.tri, fp.k, fp.eta);
 }
 
 template <typename T, typename PC>
@@ -257,12 +274,11 @@ __global__ __launch_bounds__(kFillThreads) void k_fill_panel(PC* __restrict__ Z,
                                                              const int32_t* __restrict__ rows, int nr,
                                                              const int32_t* __restrict__ cols, int nc,
                                                              const T* __restrict__ rec,
-                                                             const double* __restrict__ cen, int ne, double k,
+                                                             const double* __restrict__ cen, TriGeom tg, double k,
                                                              double eta) {
-    __shared__ FillTileLds<T> lds;
-    const auto fc = ConstsOf<T>::make(k, eta);
-    fill_tile<T, PC, decltype(fc), true>(Z, ld, rows, nr, cols, nc, blockIdx.x * kTileRows, blockIdx.y * kTileCols,
-                                         rec, cen, ne, fc, lds);
+    __shared__ typename FillLdsOf<T>::type lds;
+    fill_any<T, PC, true>(Z, ld, rows, nr, cols, nc, blockIdx.x * kTileRows, blockIdx.y * kTileCols, nullptr, rec,
+                          cen, tg, k, eta, lds);
 }
 
 void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32_t* cols, int nc,
@@ -271,10 +287,10 @@ void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32
     const dim3 grid(cdiv(nr, kTileRows), cdiv(nc, kTileCols));
     if (fp.f32)
         hipLaunchKernelGGL((k_fill_panel<float, c64>), grid, dim3(kFillThreads), 0, s, (c64*)Z, ld, rows, nr, cols,
-                           nc, (const float*)fp.rec, fp.cen, fp.ne, fp.k, fp.eta);
+                           nc, (const float*)fp.rec, fp.cen, fp.tri, fp.k, fp.eta);
     else
         hipLaunchKernelGGL((k_fill_panel<double, c128>), grid, dim3(kFillThreads), 0, s, (c128*)Z, ld, rows, nr,
-                           cols, nc, (const double*)fp.rec, fp.cen, fp.ne, fp.k, fp.eta);
+                           cols, nc, (const double*)fp.rec, fp.cen, fp.tri, fp.k, fp.eta);
 }
 
 // ---------------------------------------------------------------- pinv

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "common.hpp"
+#include "element.hpp"
 
 namespace farcur {
 
@@ -38,6 +39,8 @@ struct BlockJob {
     int32_t* hs;          // h held-out row ids (RWG ids)
     int32_t* hpos;        // h held-out row positions in the block
     void* H;              // h x n exact rows Z[hs, cols] (ld h), panel precision
+    const int32_t* row_perm;  // FP32 fill order of the rows (positions, 64 per tile), or NULL
+    const int32_t* col_perm;  // same for the columns
 };
 
 // Per-job results written by the kernels.
@@ -54,6 +57,7 @@ struct JobOut {
 struct FillParams {
     const void* rec;     // RWG records (double or float SoA, kRecFields x ne)
     const double* cen;   // RWG centres, 3 x ne SoA (used by the FP32 fill)
+    TriGeom tri;         // triangle tables (used by the FP32 fill)
     int32_t ne;
     int32_t f32;      // 1: FP32 fill, panels c64; 0: FP64 fill, panels c128
     double k, eta;
