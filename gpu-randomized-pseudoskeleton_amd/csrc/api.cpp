@@ -210,7 +210,7 @@ struct KTimer {
 
 // Launch shapes, chosen by the autotuner (farcur_autotune) or defaults.
 struct Tuning {
-    int pinv_threads[kPinvClasses] = {64, 256, 256, 256, 256, 256, 512};
+    int pinv_threads[kPinvClasses] = {64, 128, 128, 256, 256, 256, 512, 512, 512, 512};
     int fill_grid = 256 * 16;
     int resid_threads = 256;
 };
@@ -478,7 +478,7 @@ struct RoundPlan {
     int f32 = 0;
     bool pivot = false;
     int fill_grid = 0;
-    int pinv_threads[kPinvClasses] = {64, 256, 256, 256, 256, 256, 512};
+    int pinv_threads[kPinvClasses] = {64, 128, 128, 256, 256, 256, 512, 512, 512, 512};
     int resid_threads = 256;
     double atol = 1e-10, rtol = 0, label_eps = 1e-4;
     uint64_t seed = 0;
@@ -2178,9 +2178,10 @@ extern "C" int farcur_batch_block(farcur_batch* B, int b, farcur_c128* C, farcur
 extern "C" int farcur_batch_kernel_times(farcur_batch* B, const char** names, double* ms, int* launches,
                                          double* work, int cap) {
     if (!B) return fail(FARCUR_EINVAL, "NULL batch");
-    static const char* kNames[KTimer::kClasses] = {"sample",   "fill",     "pinv",     "residual", "pinv_r16",
-                                                   "pinv_r24", "pinv_r32", "pinv_r40", "pinv_r48", "pinv_r64",
-                                                   "pinv_global"};
+    static const char* kNames[KTimer::kClasses] = {"sample",   "fill",     "pinv",     "residual",
+                                                   "pinv_r16", "pinv_r20", "pinv_r24", "pinv_r28",
+                                                   "pinv_r32", "pinv_r36", "pinv_r40", "pinv_r48",
+                                                   "pinv_r64", "pinv_global"};
     static const char* kHost[4] = {"host_predict", "host_plan", "host_wait", "host_advance"};
     int n = 0;
     for (int i = 0; i < KTimer::kClasses && n < cap; ++i, ++n) {

@@ -107,13 +107,13 @@ __device__ __forceinline__ void rotate_pair(const Rot& R, c128& a, c128& b) {
     b = c128{R.s * x.re + R.c * btr, R.s * x.im + R.c * bti};
 }
 
-// Sweeps of the circle schedule over the columns of A (r x r, col-major, LDS
-// or global), rotations accumulated into V. All threads of the workgroup must
-// call; teams of kT lanes; with kE > 0 every r <= kE * kT (elements cached in
-// registers), with kE = 0 any r (elements streamed). s_flag: one int of LDS.
-// Returns the number of sweeps.
+// Sweeps of the circle schedule over the columns of A (r x r, col-major with
+// leading dimension ld, LDS or global), rotations accumulated into V (same
+// layout). All threads of the workgroup must call; teams of kT lanes; with
+// kE > 0 every r <= kE * kT (elements cached in registers), with kE = 0 any r
+// (elements streamed). s_flag: one int of LDS. Returns the number of sweeps.
 template <int kT, int kE>
-__device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int* s_flag) {
+__device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int ld, int* s_flag) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = r + (r & 1);
     const int npairs = n / 2;
@@ -139,10 +139,10 @@ __device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int* s_fla
                 int p, q;
                 circle_pair(n, step, pi, p, q);
                 if (p >= r || q >= r) continue;
-                c128* ap = A + (size_t)p * r;
-                c128* aq = A + (size_t)q * r;
-                c128* vp = V + (size_t)p * r;
-                c128* vq = V + (size_t)q * r;
+                c128* ap = A + (size_t)p * ld;
+                c128* aq = A + (size_t)q * ld;
+                c128* vp = V + (size_t)p * ld;
+                c128* vq = V + (size_t)q * ld;
                 double al = 0, be = 0, gr = 0, gi = 0;
                 if constexpr (kE > 0) {
                     c128 a[kE], b[kE];
@@ -222,13 +222,13 @@ __device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int* s_fla
 // (else 0), *kept the number of weights set, *smax sigma_max and *reps the
 // number of sigma_j >= label_eps * sigma_max. All threads must call.
 template <int kT>
-__device__ __forceinline__ void jacobi_weights(const c128* A, int r, double atol, double rtol, double label_eps,
+__device__ __forceinline__ void jacobi_weights(const c128* A, int r, int ld, double atol, double rtol, double label_eps,
                                                double* __restrict__ sigma, double* s_w, int* kept_out,
                                                double* smax_out, int* reps_out) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nteams = nt / kT, team = tid / kT, tl = tid - team * kT;
     for (int j = team; j < r; j += nteams) {
-        const c128* aj = A + (size_t)j * r;
+        const c128* aj = A + (size_t)j * ld;
         double nn = 0;
         for (int i = tl; i < r; i += kT) nn += aj[i].re * aj[i].re + aj[i].im * aj[i].im;
         nn = team_sum_t<kT>(nn);
@@ -257,9 +257,10 @@ __device__ __forceinline__ void jacobi_weights(const c128* A, int r, double atol
     *reps_out = reps;
 }
 
-// U[k, map(i)] = sum_j L[i, j] w_j conj(M[k, j]) (L, M r x r col-major; map the
-// identity when rowmap is NULL). All threads must call.
-__device__ __forceinline__ void pinv_form(const c128* L, const c128* M, const double* s_w, int r,
+// U[k, map(i)] = sum_j L[i, j] w_j conj(M[k, j]) (L, M r x r col-major with
+// leading dimension ld; map the identity when rowmap is NULL). All threads
+// must call.
+__device__ __forceinline__ void pinv_form(const c128* L, const c128* M, const double* s_w, int r, int ld,
                                           const int* rowmap, c128* __restrict__ U, int ldu) {
     for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
         const int i = idx % r, k = idx / r;
@@ -267,8 +268,8 @@ __device__ __forceinline__ void pinv_form(const c128* L, const c128* M, const do
         for (int j = 0; j < r; ++j) {
             const double w = s_w[j];
             if (w == 0.0) continue;
-            const c128 v = L[(size_t)j * r + i];
-            const c128 a = M[(size_t)j * r + k];
+            const c128 v = L[(size_t)j * ld + i];
+            const c128 a = M[(size_t)j * ld + k];
             // v * conj(a) * w
             ur += w * (v.re * a.re + v.im * a.im);
             ui += w * (v.im * a.re - v.re * a.im);
@@ -291,33 +292,34 @@ __device__ __forceinline__ void jacobi_pinv_t(c128* A, c128* V, int r, double at
         const int i = idx % r, j = idx / r;
         V[idx] = c128{i == j ? 1.0 : 0.0, 0.0};
     }
-    *sweeps_out = jacobi_sweeps<kT, kE>(A, V, r, s_flag);
-    jacobi_weights<kT>(A, r, atol, rtol, label_eps, sigma, s_w, kept_out, smax_out, rank_eps_out);
-    pinv_form(V, A, s_w, r, nullptr, U, ldu);
+    *sweeps_out = jacobi_sweeps<kT, kE>(A, V, r, r, s_flag);
+    jacobi_weights<kT>(A, r, r, atol, rtol, label_eps, sigma, s_w, kept_out, smax_out, rank_eps_out);
+    pinv_form(V, A, s_w, r, r, nullptr, U, ldu);
     __syncthreads();
 }
 
 // ---------------------------------------------------------------- QR-preconditioned
-// Householder QR with column pivoting, A P = Q R (A r x r col-major in LDS).
+// Householder QR with column pivoting, A P = Q R (A r x r col-major in LDS,
+// leading dimension ld).
 // Step k takes the trailing column of largest norm (first on ties) as logical
 // column k and reflects it onto alpha_k e_k with
 //   H_k = I - tau_k v_k v_k^H,  v_k = x - alpha_k e_k,  alpha_k = -e^{i arg x_k} |x|,
 //   tau_k = 2 / |v_k|^2 = 1 / (|x|^2 + |x| |x_k|)   (real; H_k Hermitian and unitary).
 // The map from logical to physical columns and the trailing norms are double
 // buffered, so each step needs one barrier. On exit:
-//   R[i][j] = A[phys[j] * r + i] for i < j, R[j][j] = s_alpha[j],
+//   R[i][j] = A[phys[j] * ld + i] for i < j, R[j][j] = s_alpha[j],
 //   v_k (rows k..r-1) in Wg[k * r + i] (Wg global, r x r), tau_k in s_tau[k],
 //   phys = s_phys (the final map).
 // Returns the buffer (0 or 1) of s_phys / s_norm that holds the final map.
 template <int kT>
-__device__ __forceinline__ int qrcp_lds(c128* A, int r, int* s_phys, double* s_norm, double* s_tau, c128* s_alpha,
-                                        c128* __restrict__ Wg) {
+__device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, double* s_norm, double* s_tau,
+                                        c128* s_alpha, c128* __restrict__ Wg) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nteams = nt / kT, team = tid / kT, tl = tid - team * kT;
     __syncthreads();  // A was just written by every thread of the workgroup
     for (int j = tid; j < r; j += nt) s_phys[j] = j;
     for (int j = team; j < r; j += nteams) {
-        const c128* aj = A + (size_t)j * r;
+        const c128* aj = A + (size_t)j * ld;
         double nn = 0;
         for (int i = tl; i < r; i += kT) nn += aj[i].re * aj[i].re + aj[i].im * aj[i].im;
         nn = team_sum_t<kT>(nn);
@@ -330,28 +332,45 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int* s_phys, double* s_n
         int* phys_next = s_phys + (cur ^ 1) * r;
         const double* norm = s_norm + cur * r;
         double* norm_next = s_norm + (cur ^ 1) * r;
-        int p = k;
-        double best = norm[phys[k]];
-        for (int j = k + 1; j < r; ++j) {
-            const double v = norm[phys[j]];
-            if (v > best) {
-                best = v;
-                p = j;
+        // Pivot: the largest trailing norm, first logical column on ties, by a
+        // wave-wide argmax (one lane per column, r <= 64) that every wave
+        // computes alike. (value, column) is totally ordered (a non-finite or
+        // negative norm counts as -1), so all lanes agree on the result.
+        double best;
+        int p;
+        {
+            const int j = k + (tid & (kWave - 1));
+            double v = j < r ? norm[phys[j]] : -1.0;
+            if (!(v >= 0.0)) v = -1.0;
+            int pj = j;
+#pragma unroll
+            for (int off = kWave / 2; off >= 1; off >>= 1) {
+                const double ov = __shfl_xor(v, off, kWave);
+                const int oj = __shfl_xor(pj, off, kWave);
+                if (ov > v || (ov == v && oj < pj)) {
+                    v = ov;
+                    pj = oj;
+                }
             }
+            best = v;
+            p = pj;
         }
         const int ck = phys[p], co = phys[k];
-        const c128 x0 = A[(size_t)ck * r + k];
-        const double xn = sqrt(best), ax0 = sqrt(x0.re * x0.re + x0.im * x0.im);
+        const c128 x0 = A[(size_t)ck * ld + k];
+        const double a2 = x0.re * x0.re + x0.im * x0.im;
+        const double xn = best > 0 ? best * rsqrt_nr(best) : 0.0;
+        const double ia = a2 > 0 ? rsqrt_nr(a2) : 0.0;  // 1 / |x_k|
+        const double ax0 = a2 * ia;
         c128 alpha;
         if (ax0 > 0) {
-            const double f = -xn / ax0;
+            const double f = -xn * ia;
             alpha = c128{f * x0.re, f * x0.im};
         } else {
             alpha = c128{-xn, 0.0};
         }
         const c128 v0{x0.re - alpha.re, x0.im - alpha.im};
         const double den = best + xn * ax0;
-        const double tau = den > 0 ? 1.0 / den : 0.0;
+        const double tau = den > 0 ? rcp_nr(den) : 0.0;
         for (int j = tid; j < r; j += nt) phys_next[j] = j == k ? ck : (j == p ? co : phys[j]);
         if (tid == 0) {
             s_alpha[k] = alpha;
@@ -361,8 +380,8 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int* s_phys, double* s_n
         for (int j = k + 1 + team; j < r; j += nteams) {
             int c = co;
             if (j != p) c = phys[j];
-            c128* a = A + (size_t)c * r;
-            const c128* v = A + (size_t)ck * r;
+            c128* a = A + (size_t)c * ld;
+            const c128* v = A + (size_t)ck * ld;
             // w = v^H a (row k uses v0)
             double wr = 0, wi = 0;
             for (int i = k + tl; i < r; i += kT) {
@@ -392,7 +411,7 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int* s_phys, double* s_n
     const int* phys = s_phys + cur * r;
     for (int idx = tid; idx < r * r; idx += nt) {
         const int i = idx % r, k = idx / r;
-        if (i > k) Wg[idx] = A[(size_t)phys[k] * r + i];
+        if (i > k) Wg[idx] = A[(size_t)phys[k] * ld + i];
     }
     return cur;
 }
@@ -401,11 +420,12 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int* s_phys, double* s_n
 // tau_k in s_tau). Columns of Z are independent: a team applies every
 // reflector to its columns with team reductions only.
 template <int kT>
-__device__ __forceinline__ void apply_q_left(c128* Z, int r, const c128* __restrict__ Wg, const double* s_tau) {
+__device__ __forceinline__ void apply_q_left(c128* Z, int r, int ld, const c128* __restrict__ Wg,
+                                             const double* s_tau) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nteams = nt / kT, team = tid / kT, tl = tid - team * kT;
     for (int j = team; j < r; j += nteams) {
-        c128* z = Z + (size_t)j * r;
+        c128* z = Z + (size_t)j * ld;
         for (int k = r - 1; k >= 0; --k) {
             const c128* v = Wg + (size_t)k * r;
             const double tau = s_tau[k];
@@ -435,15 +455,16 @@ __device__ __forceinline__ void apply_q_left(c128* Z, int r, const c128* __restr
 // since R = V Sigma U^H gives W = (Q V) Sigma (P U)^H. X = R^H of a
 // column-pivoted R has nearly orthogonal columns, so the sweeps drop to
 // about 5 from about 13 for the graded core matrices of far-field blocks.
-// B: second r x r LDS buffer; Wg: the global W panel (receives the
-// reflectors). Outputs as jacobi_pinv_t.
+// A, B: r x r LDS buffers with leading dimension ld; Wg: the global W panel
+// (receives the reflectors, leading dimension r). Outputs as jacobi_pinv_t.
 template <int kT, int kE>
-__device__ __forceinline__ void pinv_qr_lds(c128* A, c128* B, int r, double atol, double rtol, c128* __restrict__ U,
+__device__ __forceinline__ void pinv_qr_lds(c128* A, c128* B, int r, int ld, double atol, double rtol,
+                                            c128* __restrict__ U,
                                             int ldu, double* __restrict__ sigma, c128* __restrict__ Wg, int* s_flag,
                                             double* s_w, int* s_phys, double* s_norm, double* s_tau,
                                             c128* s_alpha, int* kept_out, double* smax_out, double label_eps,
                                             int* rank_eps_out, int* sweeps_out) {
-    const int cur = qrcp_lds<kT>(A, r, s_phys, s_norm, s_tau, s_alpha, Wg);
+    const int cur = qrcp_lds<kT>(A, r, ld, s_phys, s_norm, s_tau, s_alpha, Wg);
     const int* phys = s_phys + cur * r;
     __syncthreads();
     // X = R^H into B (X[i][j] = conj(R[j][i]) for i >= j)
@@ -453,21 +474,21 @@ __device__ __forceinline__ void pinv_qr_lds(c128* A, c128* B, int r, double atol
         if (i == j) {
             x = c128{s_alpha[j].re, -s_alpha[j].im};
         } else if (i > j) {
-            const c128 rji = A[(size_t)phys[i] * r + j];
+            const c128 rji = A[(size_t)phys[i] * ld + j];
             x = c128{rji.re, -rji.im};
         }
-        B[idx] = x;
+        B[(size_t)j * ld + i] = x;
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
         const int i = idx % r, j = idx / r;
-        A[idx] = c128{i == j ? 1.0 : 0.0, 0.0};
+        A[(size_t)j * ld + i] = c128{i == j ? 1.0 : 0.0, 0.0};
     }
-    *sweeps_out = jacobi_sweeps<kT, kE>(B, A, r, s_flag);
-    jacobi_weights<kT>(B, r, atol, rtol, label_eps, sigma, s_w, kept_out, smax_out, rank_eps_out);
-    apply_q_left<kT>(A, r, Wg, s_tau);
+    *sweeps_out = jacobi_sweeps<kT, kE>(B, A, r, ld, s_flag);
+    jacobi_weights<kT>(B, r, ld, atol, rtol, label_eps, sigma, s_w, kept_out, smax_out, rank_eps_out);
+    apply_q_left<kT>(A, r, ld, Wg, s_tau);
     __syncthreads();
-    pinv_form(B, A, s_w, r, phys, U, ldu);
+    pinv_form(B, A, s_w, r, ld, phys, U, ldu);
     __syncthreads();
 }
 

@@ -325,6 +325,7 @@ struct PinvLds {
     }
 };
 
+// Two r x r complex buffers and the small arrays.
 size_t pinv_lds_bytes(int rmax) {
     return (size_t)32 * rmax * rmax + (size_t)16 * rmax + (size_t)8 * 4 * rmax + (size_t)4 * 2 * rmax + 16;
 }
@@ -337,11 +338,11 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
     for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
         const int jb = list[li];
         const BlockJob& J = jobs[jb];
-        const int r = J.r;
+        const int r = J.r, ld = r;
         for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) L.a[idx] = load_core(J, idx, f32);
         int kept, reps, sweeps;
         double smax;
-        pinv_qr_lds<kT, kE>(L.a, L.b, r, atol, rtol, J.U, r, J.sigma, J.W, L.flag, L.w, L.phys, L.norm, L.tau,
+        pinv_qr_lds<kT, kE>(L.a, L.b, r, ld, atol, rtol, J.U, r, J.sigma, J.W, L.flag, L.w, L.phys, L.norm, L.tau,
                             L.alpha, &kept, &smax, label_eps, &reps, &sweeps);
         if (threadIdx.x == 0) {
             out[jb].kept = kept;

@@ -80,8 +80,8 @@ constexpr int kPinvLdsMaxR = 64;
 // r classes (0,16], (16,24], (24,32], (32,40], (40,48], (48,64], > 64: the LDS of
 // a launch is sized for its class maximum, so finer classes let small cores
 // pack more workgroups per CU.
-constexpr int kPinvClasses = 7;
-constexpr int kPinvClassMax[kPinvClasses] = {16, 24, 32, 40, 48, 64, 0};
+constexpr int kPinvClasses = 10;
+constexpr int kPinvClassMax[kPinvClasses] = {16, 20, 24, 28, 32, 36, 40, 48, 64, 0};
 inline int pinv_class(int r) {
     for (int c = 0; c < kPinvClasses - 1; ++c)
         if (r <= kPinvClassMax[c]) return c;

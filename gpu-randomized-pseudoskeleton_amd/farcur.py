@@ -49,7 +49,7 @@ def lib() -> ctypes.CDLL:
 
 
 vp, ci, cd, cu64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint64
-PINV_CLASSES = 7  # FARCUR_PINV_CLASSES
+PINV_CLASSES = 10  # FARCUR_PINV_CLASSES
 
 
 class Opts(ctypes.Structure):

@@ -319,7 +319,8 @@ int farcur_batch_matvec(farcur_batch* batch, const farcur_c128* V, int nrhs, far
 
 /* Device time (ms) and launch count of each kernel class over the last
  * batch_compress call with profiling on: names "sample", "fill", "pinv",
- * "residual", then per pinv class "pinv_r16" ... "pinv_r64", "pinv_global",
+ * "residual", then per pinv class "pinv_r16", "pinv_r20", ... "pinv_r64" (the
+ * largest core size of the class) and "pinv_global",
  * then the host wall time of the last compress (measured with or without
  * profiling): "host_predict" (rank inference round trip), "host_plan" (round
  * planning and upload staging), "host_wait" (waiting for the device),
@@ -334,9 +335,10 @@ int farcur_batch_kernel_times(farcur_batch* batch, const char** names, double* m
  * blocks from a `divfactor` sweep (codes/GPU_implementation/run_experiment.sh:8,
  * compres_single_block_spheres_for_sweep2.jl:574-587); here farcur_autotune
  * times candidate shapes on the device and keeps the fastest per context.
- * pinv classes: sample counts (0,16], (16,24], (24,32], (32,40], (40,48],
- * (48,64] (LDS variant sized for the class maximum) and > 64 (global). */
-#define FARCUR_PINV_CLASSES 7
+ * pinv classes: sample counts (0,16], (16,20], (20,24], (24,28], (28,32],
+ * (32,36], (36,40], (40,48], (48,64] (LDS variant sized for the class
+ * maximum, so that small cores fit more workgroups per CU) and > 64 (global). */
+#define FARCUR_PINV_CLASSES 10
 typedef struct {
     int pinv_threads[FARCUR_PINV_CLASSES];
     int fill_grid;
