@@ -28,23 +28,29 @@ namespace farcur {
 constexpr int kJacobiMaxSweeps = 40;
 
 // Column pair (p, q) of pair index pi at step `step` of the circle schedule
-// over n (even) players.
+// over n (even) players: p = (step + pi) mod (n - 1), q = (step - pi) mod
+// (n - 1) for pi > 0, (step, n - 1) for pi = 0. With step < n - 1 and
+// pi < n / 2 both sums lie in [0, 2 (n - 1)), so one conditional subtraction
+// reduces them.
 __device__ __forceinline__ void circle_pair(int n, int step, int pi, int& p, int& q) {
+    const int m = n - 1;
     if (pi == 0) {
         p = step;
-        q = n - 1;
+        q = m;
     } else {
-        p = (step + pi) % (n - 1);
-        q = (step - pi + (n - 1)) % (n - 1);
+        p = step + pi;
+        p -= p >= m ? m : 0;
+        q = step - pi + m;
+        q -= q >= m ? m : 0;
     }
 }
 
 // Row-local DPP move of a double (both halves); every source lane is inside
-// the row, so no lane reads out of range.
+// the row, so no lane reads out of range and no fill value is needed.
 template <int kCtrl>
 __device__ __forceinline__ double dpp_move(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), kCtrl, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), kCtrl, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 
@@ -134,6 +140,7 @@ __device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int ld, in
         ++sweeps;
         if (tid == 0) *s_flag = 0;
         __syncthreads();
+        float cmax = 0.0f;  // this thread's largest rotated coupling of the sweep
         for (int step = 0; step < n - 1; ++step) {
             for (int pi = team; pi < npairs; pi += nteams) {
                 int p, q;
@@ -179,7 +186,7 @@ __device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int ld, in
                             vq[i] = y;
                         }
                     }
-                    if (tl == 0) atomicMax(s_flag, __float_as_int(fmaxf((float)(g2 * rcp_nr(al * be)), 1e-30f)));
+                    cmax = fmaxf(cmax, fmaxf((float)(g2 * __builtin_amdgcn_rcp(al * be)), 1e-30f));
                 } else {
                     for (int i = tl; i < r; i += kT) {
                         const c128 a = ap[i], b = aq[i];
@@ -205,11 +212,14 @@ __device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int ld, in
                         vp[i] = x;
                         vq[i] = y;
                     }
-                    if (tl == 0) atomicMax(s_flag, __float_as_int(fmaxf((float)(g2 * rcp_nr(al * be)), 1e-30f)));
+                    cmax = fmaxf(cmax, fmaxf((float)(g2 * __builtin_amdgcn_rcp(al * be)), 1e-30f));
                 }
             }
             __syncthreads();
         }
+        // non-negative floats order like their bit patterns
+        if (cmax > 0.0f) atomicMax(s_flag, __float_as_int(cmax));
+        __syncthreads();
         const int maxc = *s_flag;
         __syncthreads();
         if (maxc == 0 || __int_as_float(maxc) < 1e-10f) break;
