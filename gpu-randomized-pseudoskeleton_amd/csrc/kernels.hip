@@ -458,7 +458,7 @@ void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double at
 }
 
 // ---------------------------------------------------------------- residual
-constexpr int kResidMaxR = 1024;
+constexpr int kResidMaxR = 512;
 
 // Relative Frobenius error of the CUR on the held-out rows:
 // ||H - C[hpos, :] U R||_F / ||H||_F. The rows go in groups of up to
@@ -481,31 +481,27 @@ __device__ double heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, 
             sT[idx] = to_c128(C[(size_t)l * m + J.hpos[i0 + i]]);
         }
         __syncthreads();
-        for (int k = tid; k < r; k += nt) {  // Y[i][k] = sum_l T[i][l] U[l][k]
+        // Y[i][k] = sum_l T[i][l] U[l][k], one entry per thread; the unrolled
+        // loop keeps several loads of U's column in flight.
+        for (int idx = tid; idx < g * r; idx += nt) {
+            const int i = idx / r, k = idx - i * r;
             const c128* uc = J.U + (size_t)k * r;
-            double ar[kHeldGroup], ai[kHeldGroup];
-#pragma unroll
-            for (int i = 0; i < kHeldGroup; ++i) ar[i] = ai[i] = 0.0;
+            const c128* ti = sT + i * r;
+            double ar = 0, ai = 0;
+#pragma unroll 8
             for (int l = 0; l < r; ++l) {
-                const c128 u = uc[l];
-#pragma unroll
-                for (int i = 0; i < kHeldGroup; ++i) {
-                    if (i < g) {
-                        const c128 t = sT[i * r + l];
-                        ar[i] += t.re * u.re - t.im * u.im;
-                        ai[i] += t.re * u.im + t.im * u.re;
-                    }
-                }
+                const c128 u = uc[l], t = ti[l];
+                ar += t.re * u.re - t.im * u.im;
+                ai += t.re * u.im + t.im * u.re;
             }
-#pragma unroll
-            for (int i = 0; i < kHeldGroup; ++i)
-                if (i < g) sY[i * r + k] = c128{ar[i], ai[i]};
+            sY[idx] = c128{ar, ai};
         }
         __syncthreads();
         for (int j = tid; j < n; j += nt) {  // z_i = sum_k Y[i][k] R[k, j]
             double zr[kHeldGroup], zi[kHeldGroup];
 #pragma unroll
             for (int i = 0; i < kHeldGroup; ++i) zr[i] = zi[i] = 0.0;
+#pragma unroll 8
             for (int k = 0; k < r; ++k) {
                 const c128 b = to_c128(R[(size_t)j * r + k]);
 #pragma unroll
