@@ -28,42 +28,38 @@ __device__ __forceinline__ U4 stream4(uint64_t seed, uint32_t block, uint32_t ro
     return philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
 }
 
-// Held-out rows first (h of m, stream kStreamHeldout), then r sample rows among
-// the other m - h positions (stream kStreamRows, word index = candidate index).
+// Held-out rows (h of m, stream kStreamHeldout) and r sample rows among the
+// other m - h positions (stream kStreamRows, word index = candidate index),
+// in one pass over the positions: both are selection scans in position order,
+// so interleaving them makes the same decisions as running them one after the
+// other. Positions are recorded first and mapped to RWG ids afterwards, so the
+// scan carries no dependent global loads.
 __device__ void sample_rows_heldout(const BlockJob& J, uint64_t seed) {
-    const uint32_t m = J.m, h = J.h, r = J.r;
-    uint32_t got = 0;
-    for (uint32_t t4 = 0; t4 < m && got < h; t4 += 4) {
-        const U4 o = stream4(seed, J.b, J.round, kStreamHeldout, t4);
-        const uint32_t w[4] = {o.x, o.y, o.z, o.w};
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            const uint32_t tt = t4 + l;
-            if (tt < m && got < h && select_step(w[l], m - tt, h - got)) {
-                J.hpos[got] = tt;
-                J.hs[got] = J.rows[tt];
-                ++got;
+    const uint32_t m = J.m, h = J.h, r = J.r, N = m - h;
+    uint32_t goth = 0, got = 0, cand = 0;
+    U4 oh{}, orow{};
+    for (uint32_t t = 0; t < m && (goth < h || got < r); ++t) {
+        if ((t & 3u) == 0 && goth < h) oh = stream4(seed, J.b, J.round, kStreamHeldout, t);
+        if (goth < h) {
+            const uint32_t l = t & 3u;
+            const uint32_t u = l == 0 ? oh.x : (l == 1 ? oh.y : (l == 2 ? oh.z : oh.w));
+            if (select_step(u, m - t, h - goth)) {
+                J.hpos[goth++] = t;
+                continue;
             }
         }
-    }
-    const uint32_t N = m - h;
-    uint32_t cand = 0, hk = 0;
-    got = 0;
-    U4 o{};
-    for (uint32_t t = 0; t < m && got < r; ++t) {
-        if (hk < h && (uint32_t)J.hpos[hk] == t) {
-            ++hk;
-            continue;
-        }
-        if ((cand & 3u) == 0) o = stream4(seed, J.b, J.round, kStreamRows, cand);
-        const uint32_t l = cand & 3u;
-        const uint32_t u = l == 0 ? o.x : (l == 1 ? o.y : (l == 2 ? o.z : o.w));
-        if (select_step(u, N - cand, r - got)) {
-            if (J.rpos) J.rpos[got] = t;
-            J.rs[got++] = J.rows[t];
+        if (got < r) {
+            if ((cand & 3u) == 0) orow = stream4(seed, J.b, J.round, kStreamRows, cand);
+            const uint32_t l = cand & 3u;
+            const uint32_t u = l == 0 ? orow.x : (l == 1 ? orow.y : (l == 2 ? orow.z : orow.w));
+            if (select_step(u, N - cand, r - got)) J.rpos[got++] = t;
         }
         ++cand;
     }
+#pragma unroll 8
+    for (uint32_t k = 0; k < goth; ++k) J.hs[k] = J.rows[J.hpos[k]];
+#pragma unroll 8
+    for (uint32_t k = 0; k < got; ++k) J.rs[k] = J.rows[J.rpos[k]];
 }
 
 __global__ void k_sample(const BlockJob* __restrict__ jobs, int njobs, uint64_t seed) {
@@ -80,6 +76,9 @@ __global__ void k_sample(const BlockJob* __restrict__ jobs, int njobs, uint64_t 
     const uint32_t N = kind ? J.m : J.n;
     const int32_t* ids = kind ? J.rows : J.cols;
     int32_t* out = kind ? J.rs : J.cs;
+    // positions first (into rpos for rows when the job keeps them, else into
+    // the output), RWG ids after the scan
+    int32_t* pos = kind == 1 && J.rpos ? J.rpos : out;
     const uint32_t purpose = kind ? kStreamRows : kStreamCols;
     const uint32_t need = J.r;
     uint32_t got = 0;
@@ -94,12 +93,11 @@ __global__ void k_sample(const BlockJob* __restrict__ jobs, int njobs, uint64_t 
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
             const uint32_t tt = t4 + l;
-            if (tt < N && got < need && select_step(w[l], N - tt, need - got)) {
-                if (kind == 1 && J.rpos) J.rpos[got] = tt;
-                out[got++] = ids[tt];
-            }
+            if (tt < N && got < need && select_step(w[l], N - tt, need - got)) pos[got++] = tt;
         }
     }
+#pragma unroll 8
+    for (uint32_t k = 0; k < got; ++k) out[k] = ids[pos[k]];
 }
 
 void launch_sample(const BlockJob* jobs, int njobs, uint64_t seed, hipStream_t s) {
