@@ -115,7 +115,7 @@ __device__ __forceinline__ void rotate_pair(const Rot& R, c128& a, c128& b) {
 
 // Sweeps of the circle schedule over the columns of A (r x r, col-major with
 // leading dimension ld, LDS or global), rotations accumulated into V (same
-// layout). All threads of the workgroup must call; teams of kT lanes; with
+// layout) unless V is NULL. All threads of the workgroup must call; teams of kT lanes; with
 // kE > 0 every r <= kE * kT (elements cached in registers), with kE = 0 any r
 // (elements streamed). s_flag: one int of LDS. Returns the number of sweeps.
 template <int kT, int kE>
@@ -148,8 +148,8 @@ __device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int ld, in
                 if (p >= r || q >= r) continue;
                 c128* ap = A + (size_t)p * ld;
                 c128* aq = A + (size_t)q * ld;
-                c128* vp = V + (size_t)p * ld;
-                c128* vq = V + (size_t)q * ld;
+                c128* vp = V ? V + (size_t)p * ld : nullptr;
+                c128* vq = V ? V + (size_t)q * ld : nullptr;
                 double al = 0, be = 0, gr = 0, gi = 0;
                 if constexpr (kE > 0) {
                     c128 a[kE], b[kE];
@@ -180,10 +180,12 @@ __device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int ld, in
                             rotate_pair(R, a[e], b[e]);
                             ap[i] = a[e];
                             aq[i] = b[e];
-                            c128 x = vp[i], y = vq[i];
-                            rotate_pair(R, x, y);
-                            vp[i] = x;
-                            vq[i] = y;
+                            if (vp) {
+                                c128 x = vp[i], y = vq[i];
+                                rotate_pair(R, x, y);
+                                vp[i] = x;
+                                vq[i] = y;
+                            }
                         }
                     }
                     cmax = fmaxf(cmax, fmaxf((float)(g2 * __builtin_amdgcn_rcp(al * be)), 1e-30f));
@@ -207,10 +209,12 @@ __device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int ld, in
                         rotate_pair(R, a, b);
                         ap[i] = a;
                         aq[i] = b;
-                        c128 x = vp[i], y = vq[i];
-                        rotate_pair(R, x, y);
-                        vp[i] = x;
-                        vq[i] = y;
+                        if (vp) {
+                            c128 x = vp[i], y = vq[i];
+                            rotate_pair(R, x, y);
+                            vp[i] = x;
+                            vq[i] = y;
+                        }
                     }
                     cmax = fmaxf(cmax, fmaxf((float)(g2 * __builtin_amdgcn_rcp(al * be)), 1e-30f));
                 }
@@ -228,13 +232,14 @@ __device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int ld, in
 }
 
 // Singular values sigma[j] = |a_j| of the converged columns, the pinv weights
-// s_w[j] = 1 / sigma_j^2 for sigma_j >= tau = max(atol, rtol * sigma_max)
-// (else 0), *kept the number of weights set, *smax sigma_max and *reps the
-// number of sigma_j >= label_eps * sigma_max. All threads must call.
+// s_w[j] = 1 / sigma_j^2 (1 / sigma_j^4 with quartic) for sigma_j >= tau =
+// max(atol, rtol * sigma_max) (else 0), *kept the number of weights set,
+// *smax sigma_max and *reps the number of sigma_j >= label_eps * sigma_max.
+// All threads must call.
 template <int kT>
 __device__ __forceinline__ void jacobi_weights(const c128* A, int r, int ld, double atol, double rtol, double label_eps,
                                                double* __restrict__ sigma, double* s_w, int* kept_out,
-                                               double* smax_out, int* reps_out) {
+                                               double* smax_out, int* reps_out, bool quartic = false) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nteams = nt / kT, team = tid / kT, tl = tid - team * kT;
     for (int j = team; j < r; j += nteams) {
@@ -259,7 +264,7 @@ __device__ __forceinline__ void jacobi_weights(const c128* A, int r, int ld, dou
     __syncthreads();
     for (int j = tid; j < r; j += nt) {
         const double nn = s_w[j];
-        s_w[j] = (sigma[j] >= tau && nn > 0) ? 1.0 / nn : 0.0;
+        s_w[j] = (sigma[j] >= tau && nn > 0) ? (quartic ? 1.0 / (nn * nn) : 1.0 / nn) : 0.0;
     }
     __syncthreads();
     *kept_out = kept;
@@ -318,7 +323,8 @@ __device__ __forceinline__ void jacobi_pinv_t(c128* A, c128* V, int r, double at
 // The map from logical to physical columns and the trailing norms are double
 // buffered, so each step needs one barrier. On exit:
 //   R[i][j] = A[phys[j] * ld + i] for i < j, R[j][j] = s_alpha[j],
-//   v_k (rows k..r-1) in Wg[k * r + i] (Wg global, r x r), tau_k in s_tau[k],
+//   v_k (rows k..r-1) in Wg[k * r + i] (Wg global, r x r; not stored when
+//   Wg is NULL), tau_k in s_tau[k],
 //   phys = s_phys (the final map).
 // Returns the buffer (0 or 1) of s_phys / s_norm that holds the final map.
 template <int kT>
@@ -385,7 +391,7 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, dou
         if (tid == 0) {
             s_alpha[k] = alpha;
             s_tau[k] = tau;
-            Wg[(size_t)k * r + k] = v0;
+            if (Wg) Wg[(size_t)k * r + k] = v0;
         }
         for (int j = k + 1 + team; j < r; j += nteams) {
             int c = co;
@@ -419,9 +425,11 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, dou
     }
     // reflector bodies (rows below k) to global
     const int* phys = s_phys + cur * r;
-    for (int idx = tid; idx < r * r; idx += nt) {
-        const int i = idx % r, k = idx / r;
-        if (i > k) Wg[idx] = A[(size_t)phys[k] * ld + i];
+    if (Wg) {
+        for (int idx = tid; idx < r * r; idx += nt) {
+            const int i = idx % r, k = idx / r;
+            if (i > k) Wg[idx] = A[(size_t)phys[k] * ld + i];
+        }
     }
     return cur;
 }
@@ -459,22 +467,33 @@ __device__ __forceinline__ void apply_q_left(c128* Z, int r, int ld, const c128*
 }
 
 // Pinv of W (in A, LDS; destroyed) preconditioned by QR with column pivoting:
-// W P = Q R, one-sided Jacobi on X = R^H (X V = X_f, orthogonal columns
-// x_j = sigma_j u_j), and
-//   W^+ = P X_f diag(w) (Q V)^H,  w_j = [sigma_j >= tau] / sigma_j^2,
-// since R = V Sigma U^H gives W = (Q V) Sigma (P U)^H. X = R^H of a
-// column-pivoted R has nearly orthogonal columns, so the sweeps drop to
-// about 5 from about 13 for the graded core matrices of far-field blocks.
+// W P = Q R, then one-sided Jacobi on X = R^H (X V = X_f, orthogonal columns
+// x_j = sigma_j u_j). X = R^H of a column-pivoted R has nearly orthogonal
+// columns, so the sweeps drop to about 5 from about 13 for the graded core
+// matrices of far-field blocks. Two ways to finish:
+//  - with V (accumulated rotations): R = V Sigma U^H gives
+//      W^+ = P X_f diag(w) (Q V)^H,  w_j = [sigma_j >= tau] / sigma_j^2.
+//    The weak components keep the relative accuracy of the Jacobi; needed
+//    when tau may sit many orders below sigma_max (absolute thresholds).
+//  - without V (vfree): W^H W = P X X^H P^T = P X_f X_f^H P^T, so
+//      W^+ = (W^H W)^+ W^H = P X_f diag(w) Y^H,  Y = W P X_f,
+//      w_j = [sigma_j >= tau] / sigma_j^4.
+//    No rotations are accumulated and Q is not needed, but Y = W P x_j,
+//    of norm sigma_j^2, forms from entries of size sigma_max sigma_j: the
+//    weak components carry a relative error ~ eps sigma_max / sigma_j. The
+//    caller uses this form only when tau >= 1e-8 sigma_max (rtol >= 1e-8).
 // A, B: r x r LDS buffers with leading dimension ld; Wg: the global W panel
-// (receives the reflectors, leading dimension r). Outputs as jacobi_pinv_t.
-template <int kT, int kE>
-__device__ __forceinline__ void pinv_qr_lds(c128* A, c128* B, int r, int ld, double atol, double rtol,
-                                            c128* __restrict__ U,
-                                            int ldu, double* __restrict__ sigma, c128* __restrict__ Wg, int* s_flag,
+// (receives the reflectors, leading dimension r; untouched when vfree);
+// wload(idx): element idx (column-major, ld r) of the original W, used when
+// vfree. Outputs as jacobi_pinv_t.
+template <int kT, int kE, typename WLoad>
+__device__ __forceinline__ void pinv_qr_lds(c128* A, c128* B, int r, int ld, double atol, double rtol, bool vfree,
+                                            const WLoad& wload, c128* __restrict__ U, int ldu,
+                                            double* __restrict__ sigma, c128* __restrict__ Wg, int* s_flag,
                                             double* s_w, int* s_phys, double* s_norm, double* s_tau,
                                             c128* s_alpha, int* kept_out, double* smax_out, double label_eps,
                                             int* rank_eps_out, int* sweeps_out) {
-    const int cur = qrcp_lds<kT>(A, r, ld, s_phys, s_norm, s_tau, s_alpha, Wg);
+    const int cur = qrcp_lds<kT>(A, r, ld, s_phys, s_norm, s_tau, s_alpha, vfree ? nullptr : Wg);
     const int* phys = s_phys + cur * r;
     __syncthreads();
     // X = R^H into B (X[i][j] = conj(R[j][i]) for i >= j)
@@ -490,13 +509,33 @@ __device__ __forceinline__ void pinv_qr_lds(c128* A, c128* B, int r, int ld, dou
         B[(size_t)j * ld + i] = x;
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
-        const int i = idx % r, j = idx / r;
-        A[(size_t)j * ld + i] = c128{i == j ? 1.0 : 0.0, 0.0};
+    if (vfree) {
+        *sweeps_out = jacobi_sweeps<kT, kE>(B, nullptr, r, ld, s_flag);
+        jacobi_weights<kT>(B, r, ld, atol, rtol, label_eps, sigma, s_w, kept_out, smax_out, rank_eps_out, true);
+        // Y[k][j] = sum_l W[k][phys[l]] X_f[l][j] into A (free since X was formed)
+        for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
+            const int k = idx % r, j = idx / r;
+            const c128* xj = B + (size_t)j * ld;
+            double yr = 0, yi = 0;
+            if (s_w[j] != 0.0) {
+#pragma unroll 4
+                for (int l = 0; l < r; ++l) {
+                    const c128 w = wload((size_t)phys[l] * r + k), x = xj[l];
+                    yr += w.re * x.re - w.im * x.im;
+                    yi += w.re * x.im + w.im * x.re;
+                }
+            }
+            A[(size_t)j * ld + k] = c128{yr, yi};
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
+            const int i = idx % r, j = idx / r;
+            A[(size_t)j * ld + i] = c128{i == j ? 1.0 : 0.0, 0.0};
+        }
+        *sweeps_out = jacobi_sweeps<kT, kE>(B, A, r, ld, s_flag);
+        jacobi_weights<kT>(B, r, ld, atol, rtol, label_eps, sigma, s_w, kept_out, smax_out, rank_eps_out);
+        apply_q_left<kT>(A, r, ld, Wg, s_tau);
     }
-    *sweeps_out = jacobi_sweeps<kT, kE>(B, A, r, ld, s_flag);
-    jacobi_weights<kT>(B, r, ld, atol, rtol, label_eps, sigma, s_w, kept_out, smax_out, rank_eps_out);
-    apply_q_left<kT>(A, r, ld, Wg, s_tau);
     __syncthreads();
     pinv_form(B, A, s_w, r, ld, phys, U, ldu);
     __syncthreads();

@@ -340,8 +340,12 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
         for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) L.a[idx] = load_core(J, idx, f32);
         int kept, reps, sweeps;
         double smax;
-        pinv_qr_lds<kT, kE>(L.a, L.b, r, ld, atol, rtol, J.U, r, J.sigma, J.W, L.flag, L.w, L.phys, L.norm, L.tau,
-                            L.alpha, &kept, &smax, label_eps, &reps, &sweeps);
+        // Relative thresholds of 1e-8 and above keep the weak components'
+        // error of the V-free form (eps sigma_max / sigma_j) negligible.
+        const bool vfree = rtol >= 1e-8;
+        auto wload = [&](size_t idx) { return load_core(J, (int)idx, f32); };
+        pinv_qr_lds<kT, kE>(L.a, L.b, r, ld, atol, rtol, vfree, wload, J.U, r, J.sigma, J.W, L.flag, L.w, L.phys,
+                            L.norm, L.tau, L.alpha, &kept, &smax, label_eps, &reps, &sweeps);
         if (threadIdx.x == 0) {
             out[jb].kept = kept;
             out[jb].sigma_max = smax;
