@@ -481,6 +481,7 @@ struct RoundPlan {
     int pinv_threads[kPinvClasses] = {64, 128, 128, 256, 256, 256, 512, 512, 512, 512};
     int resid_threads = 256;
     double atol = 1e-10, rtol = 0, label_eps = 1e-4;
+    int one_buf = 0;  // every job gathers its core from C (one-buffer pinv allowed)
     uint64_t seed = 0;
     // device pointers
     const BlockJob* jobs = nullptr;
@@ -533,7 +534,7 @@ void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt, hipStream_
         hipStream_t sc = side && (k++ & 1) == 0 ? side : s;
         kt.begin(4 + c, sc, &a);
         launch_pinv(P.jobs, P.list[c], P.nlist[c], P.atol, P.rtol, P.label_eps, kPinvClassMax[c], P.pinv_threads[c],
-                    P.f32, P.out, sc);
+                    P.f32, P.one_buf, P.out, sc);
         kt.end(4 + c, sc, a);
     }
     if (side) {
@@ -553,6 +554,7 @@ struct PackedRound {
     size_t off_jobs = 0, off_pa = 0, off_pwr = 0, off_out = 0, size = 0;
     size_t off_list[kPinvClasses] = {};
     int total_all = 0, total_wr = 0;
+    int all_rpos = 1;  // every job gathers its core from C
     int nlist[kPinvClasses] = {};
 };
 
@@ -565,8 +567,10 @@ void pack_layout(const BlockJob* jobs, int nj, bool pivot, PackedRound& pk, Pack
     sc.pa.assign(nj + 1, 0);
     sc.pwr.assign(nj + 1, 0);
     for (auto& l : sc.lists) l.clear();
+    pk.all_rpos = 1;
     for (int j = 0; j < nj; ++j) {
         const BlockJob& J = jobs[j];
+        if (!J.rpos) pk.all_rpos = 0;
         const int pw = J.rpos ? 0 : kPanelW;  // with row positions W is gathered from C, not filled
         const int all = pivot ? fill_tiles(J.m, J.n, J.r, J.h, kPanelC)
                               : fill_tiles(J.m, J.n, J.r, J.h, kPanelC | pw | kPanelR | kPanelH);
@@ -631,6 +635,7 @@ RoundPlan plan_from(const PackedRound& pk, void* dev, int njobs) {
         P.nlist[c] = pk.nlist[c];
     }
     P.out = (JobOut*)(d + pk.off_out);
+    P.one_buf = pk.all_rpos;
     return P;
 }
 
@@ -910,7 +915,7 @@ int farcur_pinv(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, doubl
     Plan* dp = ctx->d_plan.as<Plan>();
     const int cls = pinv_class(r);
     const int threads = cls < kPinvClasses - 1 ? ctx->tune.pinv_threads[cls] : (r > 128 ? 1024 : 256);
-    launch_pinv(&dp->job, dp->list, 1, atol, rtol, 0.0, kPinvClassMax[cls], threads, 0, &dp->out, ctx->stream);
+    launch_pinv(&dp->job, dp->list, 1, atol, rtol, 0.0, kPinvClassMax[cls], threads, 0, 0, &dp->out, ctx->stream);
     HIPTRY(hipGetLastError());
     HIPTRY(hipMemcpyAsync(U, ctx->d_U.p, rr, hipMemcpyDeviceToHost, ctx->stream));
     std::vector<double> s(r);
@@ -2204,10 +2209,14 @@ namespace {
 
 // Times the pinv of nprob random r x r problems with the given launch shape
 // (min over reps, ms).
+// Times the pinv of nprob random r x r cores. The LDS variant (rmax > 0) runs
+// as in a batch round: each core gathered from a panel (here the pristine
+// copy W0 through identity row positions) with the FP32 pipeline's relative
+// threshold 1e-6, i.e. the one-buffer V-free form.
 int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* best_ms, double* sweeps = nullptr) {
     hipStream_t s = ctx->stream;
     const size_t rr = (size_t)r * r;
-    DBuf W0, W, V, U, sig, plan;
+    DBuf W0, W, V, U, sig, plan, ids;
     TRY(W0.ensure(nprob * rr * 16));
     TRY(W.ensure(nprob * rr * 16));
     TRY(V.ensure(nprob * pinv_v_elems(r) * 16));
@@ -2222,6 +2231,11 @@ int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* 
         z.im = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
     }
     HIPTRY(hipMemcpyAsync(W0.p, h.data(), h.size() * 16, hipMemcpyHostToDevice, s));
+    std::vector<int32_t> iden(r);
+    for (int i = 0; i < r; ++i) iden[i] = i;
+    TRY(ids.ensure((size_t)r * 4));
+    HIPTRY(hipMemcpyAsync(ids.p, iden.data(), (size_t)r * 4, hipMemcpyHostToDevice, s));
+    const bool gathered = rmax > 0;
     std::vector<BlockJob> jobs(nprob);
     std::vector<int32_t> list(nprob);
     for (int i = 0; i < nprob; ++i) {
@@ -2232,6 +2246,11 @@ int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* 
         J.V = V.as<c128>() + i * pinv_v_elems(r);
         J.U = U.as<c128>() + i * rr;
         J.sigma = sig.as<double>() + (size_t)i * r;
+        if (gathered) {
+            J.m = r;
+            J.C = W0.as<c128>() + i * rr;
+            J.rpos = ids.as<int32_t>();
+        }
         list[i] = i;
     }
     const size_t bj = nprob * sizeof(BlockJob), bl = (size_t)nprob * 4, bo = (size_t)nprob * sizeof(JobOut);
@@ -2243,8 +2262,8 @@ int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* 
     for (int rep = 0; rep < 3; ++rep) {
         HIPTRY(hipMemcpyAsync(W.p, W0.p, nprob * rr * 16, hipMemcpyDeviceToDevice, s));
         HIPTRY(hipEventRecord(ctx->ev0, s));
-        launch_pinv((const BlockJob*)d, (const int32_t*)(d + bj), nprob, 1e-10, 0.0, 0.0, rmax, threads, 0,
-                    (JobOut*)(d + bj + bl + 256), s);
+        launch_pinv((const BlockJob*)d, (const int32_t*)(d + bj), nprob, 1e-10, gathered ? 1e-6 : 0.0, 0.0, rmax,
+                    threads, 0, gathered ? 1 : 0, (JobOut*)(d + bj + bl + 256), s);
         HIPTRY(hipEventRecord(ctx->ev1, s));
         HIPTRY(hipGetLastError());
         HIPTRY(hipStreamSynchronize(s));

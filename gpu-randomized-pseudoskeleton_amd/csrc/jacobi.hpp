@@ -273,9 +273,9 @@ __device__ __forceinline__ void jacobi_weights(const c128* A, int r, int ld, dou
 }
 
 // U[k, map(i)] = sum_j L[i, j] w_j conj(M[k, j]) (L, M r x r col-major with
-// leading dimension ld; map the identity when rowmap is NULL). All threads
-// must call.
-__device__ __forceinline__ void pinv_form(const c128* L, const c128* M, const double* s_w, int r, int ld,
+// leading dimensions ld, ldm; map the identity when rowmap is NULL). All
+// threads must call.
+__device__ __forceinline__ void pinv_form(const c128* L, const c128* M, const double* s_w, int r, int ld, int ldm,
                                           const int* rowmap, c128* __restrict__ U, int ldu) {
     for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
         const int i = idx % r, k = idx / r;
@@ -284,7 +284,7 @@ __device__ __forceinline__ void pinv_form(const c128* L, const c128* M, const do
             const double w = s_w[j];
             if (w == 0.0) continue;
             const c128 v = L[(size_t)j * ld + i];
-            const c128 a = M[(size_t)j * ld + k];
+            const c128 a = M[(size_t)j * ldm + k];
             // v * conj(a) * w
             ur += w * (v.re * a.re + v.im * a.im);
             ui += w * (v.im * a.re - v.re * a.im);
@@ -309,7 +309,7 @@ __device__ __forceinline__ void jacobi_pinv_t(c128* A, c128* V, int r, double at
     }
     *sweeps_out = jacobi_sweeps<kT, kE>(A, V, r, r, s_flag);
     jacobi_weights<kT>(A, r, r, atol, rtol, label_eps, sigma, s_w, kept_out, smax_out, rank_eps_out);
-    pinv_form(V, A, s_w, r, r, nullptr, U, ldu);
+    pinv_form(V, A, s_w, r, r, r, nullptr, U, ldu);
     __syncthreads();
 }
 
@@ -485,7 +485,9 @@ __device__ __forceinline__ void apply_q_left(c128* Z, int r, int ld, const c128*
 // A, B: r x r LDS buffers with leading dimension ld; Wg: the global W panel
 // (receives the reflectors, leading dimension r; untouched when vfree);
 // wload(idx): element idx (column-major, ld r) of the original W, used when
-// vfree. Outputs as jacobi_pinv_t.
+// vfree. B = NULL (vfree only, W not read from Wg): one LDS buffer; X = R^H
+// is staged through U (global) and Y goes to Wg, so the launch needs half
+// the LDS and fits twice the workgroups per CU. Outputs as jacobi_pinv_t.
 template <int kT, int kE, typename WLoad>
 __device__ __forceinline__ void pinv_qr_lds(c128* A, c128* B, int r, int ld, double atol, double rtol, bool vfree,
                                             const WLoad& wload, c128* __restrict__ U, int ldu,
@@ -496,7 +498,11 @@ __device__ __forceinline__ void pinv_qr_lds(c128* A, c128* B, int r, int ld, dou
     const int cur = qrcp_lds<kT>(A, r, ld, s_phys, s_norm, s_tau, s_alpha, vfree ? nullptr : Wg);
     const int* phys = s_phys + cur * r;
     __syncthreads();
-    // X = R^H into B (X[i][j] = conj(R[j][i]) for i >= j)
+    // X = R^H into B (X[i][j] = conj(R[j][i]) for i >= j); with one buffer
+    // through U into A
+    const bool one_buf = B == nullptr;
+    c128* Xs = one_buf ? U : B;
+    const int ldx = one_buf ? ldu : ld;
     for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
         const int i = idx % r, j = idx / r;
         c128 x{0.0, 0.0};
@@ -506,13 +512,24 @@ __device__ __forceinline__ void pinv_qr_lds(c128* A, c128* B, int r, int ld, dou
             const c128 rji = A[(size_t)phys[i] * ld + j];
             x = c128{rji.re, -rji.im};
         }
-        B[(size_t)j * ld + i] = x;
+        Xs[(size_t)j * ldx + i] = x;
     }
     __syncthreads();
+    if (one_buf) {
+        for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
+            const int i = idx % r, j = idx / r;
+            A[(size_t)j * ld + i] = U[(size_t)j * ldu + i];
+        }
+        B = A;  // X lives in A from here on
+        __syncthreads();
+    }
     if (vfree) {
         *sweeps_out = jacobi_sweeps<kT, kE>(B, nullptr, r, ld, s_flag);
         jacobi_weights<kT>(B, r, ld, atol, rtol, label_eps, sigma, s_w, kept_out, smax_out, rank_eps_out, true);
-        // Y[k][j] = sum_l W[k][phys[l]] X_f[l][j] into A (free since X was formed)
+        // Y[k][j] = sum_l W[k][phys[l]] X_f[l][j] into A (free since X was
+        // formed) or, with one buffer, into Wg (ld r; W is not read from it)
+        c128* Y = one_buf ? Wg : A;
+        const int ldy = one_buf ? r : ld;
         for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
             const int k = idx % r, j = idx / r;
             const c128* xj = B + (size_t)j * ld;
@@ -525,8 +542,12 @@ __device__ __forceinline__ void pinv_qr_lds(c128* A, c128* B, int r, int ld, dou
                     yi += w.re * x.im + w.im * x.re;
                 }
             }
-            A[(size_t)j * ld + k] = c128{yr, yi};
+            Y[(size_t)j * ldy + k] = c128{yr, yi};
         }
+        __syncthreads();
+        pinv_form(B, Y, s_w, r, ld, ldy, phys, U, ldu);
+        __syncthreads();
+        return;
     } else {
         for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
             const int i = idx % r, j = idx / r;
@@ -537,7 +558,7 @@ __device__ __forceinline__ void pinv_qr_lds(c128* A, c128* B, int r, int ld, dou
         apply_q_left<kT>(A, r, ld, Wg, s_tau);
     }
     __syncthreads();
-    pinv_form(B, A, s_w, r, ld, phys, U, ldu);
+    pinv_form(B, A, s_w, r, ld, ld, phys, U, ldu);
     __syncthreads();
 }
 

@@ -311,10 +311,10 @@ struct PinvLds {
     c128 *a, *b, *alpha;
     double *w, *norm, *tau;
     int *phys, *flag;
-    __device__ PinvLds(char* smem, int rmax) {
+    __device__ PinvLds(char* smem, int rmax, bool one_buf) {
         a = (c128*)smem;
-        b = a + (size_t)rmax * rmax;
-        alpha = b + (size_t)rmax * rmax;
+        b = one_buf ? nullptr : a + (size_t)rmax * rmax;
+        alpha = a + (size_t)rmax * rmax * (one_buf ? 1 : 2);
         w = (double*)(alpha + rmax);
         norm = w + rmax;     // 2 x rmax
         tau = norm + 2 * rmax;
@@ -323,16 +323,18 @@ struct PinvLds {
     }
 };
 
-// Two r x r complex buffers and the small arrays.
-size_t pinv_lds_bytes(int rmax) {
-    return (size_t)32 * rmax * rmax + (size_t)16 * rmax + (size_t)8 * 4 * rmax + (size_t)4 * 2 * rmax + 16;
+// Two (one_buf: one) r x r complex buffers and the small arrays.
+size_t pinv_lds_bytes(int rmax, bool one_buf) {
+    return (size_t)(one_buf ? 16 : 32) * rmax * rmax + (size_t)16 * rmax + (size_t)8 * 4 * rmax +
+           (size_t)4 * 2 * rmax + 16;
 }
 
 template <int kT, int kE>
 __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist, int rmax,
-                           double atol, double rtol, double label_eps, int f32, JobOut* __restrict__ out) {
+                           double atol, double rtol, double label_eps, int f32, int one_buf,
+                           JobOut* __restrict__ out) {
     extern __shared__ __align__(16) char smem[];
-    const PinvLds L(smem, rmax);
+    const PinvLds L(smem, rmax, one_buf != 0);
     for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
         const int jb = list[li];
         const BlockJob& J = jobs[jb];
@@ -342,7 +344,9 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
         double smax;
         // Relative thresholds of 1e-8 and above keep the weak components'
         // error of the V-free form (eps sigma_max / sigma_j) negligible.
-        const bool vfree = rtol >= 1e-8;
+        // one_buf (set by the host only with rtol >= 1e-8 and every core
+        // gathered from C) implies the V-free form.
+        const bool vfree = rtol >= 1e-8 || one_buf;
         auto wload = [&](size_t idx) { return load_core(J, (int)idx, f32); };
         pinv_qr_lds<kT, kE>(L.a, L.b, r, ld, atol, rtol, vfree, wload, J.U, r, J.sigma, J.W, L.flag, L.w, L.phys,
                             L.norm, L.tau, L.alpha, &kept, &smax, label_eps, &reps, &sweeps);
@@ -410,15 +414,15 @@ namespace {
 
 template <int kT, int kE>
 void launch_lds(int grid, int threads, size_t lds, hipStream_t s, const BlockJob* jobs, const int32_t* list,
-                int nlist, int rmax, double atol, double rtol, double label_eps, int f32, JobOut* out) {
+                int nlist, int rmax, double atol, double rtol, double label_eps, int f32, int one_buf, JobOut* out) {
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute((const void*)k_pinv_lds<kT, kE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)pinv_lds_bytes(kPinvLdsMaxR));
+                                  (int)pinv_lds_bytes(kPinvLdsMaxR, false));
         attr_set = true;
     }
     hipLaunchKernelGGL((k_pinv_lds<kT, kE>), dim3(grid), dim3(threads), lds, s, jobs, list, nlist, rmax, atol, rtol,
-                       label_eps, f32, out);
+                       label_eps, f32, one_buf, out);
 }
 
 // E rounded up to a compiled register count; past 16 elements the team
@@ -426,32 +430,33 @@ void launch_lds(int grid, int threads, size_t lds, hipStream_t s, const BlockJob
 template <int kT>
 void launch_lds_t(int E, int grid, int threads, size_t lds, hipStream_t s, const BlockJob* jobs,
                   const int32_t* list, int nlist, int rmax, double atol, double rtol, double label_eps, int f32,
-                  JobOut* out) {
-    if (E <= 1) launch_lds<kT, 1>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out);
-    else if (E <= 2) launch_lds<kT, 2>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out);
-    else if (E <= 4) launch_lds<kT, 4>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out);
-    else if (E <= 8) launch_lds<kT, 8>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out);
-    else if (E <= 16) launch_lds<kT, 16>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out);
-    else launch_lds<kT, 0>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out);
+                  int one_buf, JobOut* out) {
+    if (E <= 1) launch_lds<kT, 1>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
+    else if (E <= 2) launch_lds<kT, 2>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
+    else if (E <= 4) launch_lds<kT, 4>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
+    else if (E <= 8) launch_lds<kT, 8>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
+    else if (E <= 16) launch_lds<kT, 16>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
+    else launch_lds<kT, 0>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
 }
 
 }  // namespace
 
 void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, double label_eps,
-                 int rmax, int threads, int f32, JobOut* out, hipStream_t s) {
+                 int rmax, int threads, int f32, int one_buf, JobOut* out, hipStream_t s) {
     if (nlist <= 0) return;
     const int grid = nlist < 16384 ? nlist : 16384;
     if (rmax > 0 && rmax <= kPinvLdsMaxR) {
         int T, E;
         pinv_lds_shape(rmax, threads, &T, &E);
-        const size_t lds = pinv_lds_bytes(rmax);
+        one_buf = one_buf && rtol >= 1e-8;
+        const size_t lds = pinv_lds_bytes(rmax, one_buf != 0);
         switch (T) {
             case 1:
-            case 2: launch_lds_t<2>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out); break;
-            case 4: launch_lds_t<4>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out); break;
-            case 8: launch_lds_t<8>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out); break;
-            case 16: launch_lds_t<16>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out); break;
-            default: launch_lds_t<32>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, out); break;
+            case 2: launch_lds_t<2>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out); break;
+            case 4: launch_lds_t<4>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out); break;
+            case 8: launch_lds_t<8>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out); break;
+            case 16: launch_lds_t<16>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out); break;
+            default: launch_lds_t<32>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out); break;
         }
     } else {
         hipLaunchKernelGGL(k_pinv_global, dim3(grid), dim3(threads), 0, s, jobs, list, nlist, atol, rtol, label_eps,

@@ -87,15 +87,17 @@ inline int pinv_class(int r) {
         if (r <= kPinvClassMax[c]) return c;
     return kPinvClasses - 1;
 }
-size_t pinv_lds_bytes(int rmax);
+size_t pinv_lds_bytes(int rmax, bool one_buf);
 // Complex elements of the Jacobi scratch V for sample count r (V plus, past
 // r = 1024, the r weights).
 inline size_t pinv_v_elems(int r) { return (size_t)r * r + (size_t)(r + 1) / 2; }
 // Largest sample count any entry point accepts.
 constexpr int kMaxSamples = 8192;
-// f32: panel precision of C (for jobs whose W is gathered from C).
+// f32: panel precision of C (for jobs whose W is gathered from C). one_buf:
+// every listed job gathers W from C (rpos set), so with rtol >= 1e-8 the LDS
+// variant may use one r x r buffer (W panel and U as global scratch).
 void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, double label_eps,
-                 int rmax, int threads, int f32, JobOut* out, hipStream_t s);
+                 int rmax, int threads, int f32, int one_buf, JobOut* out, hipStream_t s);
 void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, JobOut* out, hipStream_t s);
 
 // Plain panel fill Z[i + j*ld] = Z(rows[i], cols[j]) (standalone, one panel).
