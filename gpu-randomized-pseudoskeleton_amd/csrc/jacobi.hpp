@@ -484,8 +484,7 @@ __device__ __forceinline__ void apply_q_left(c128* Z, int r, int ld, const c128*
 //    caller uses this form only when tau >= 1e-8 sigma_max (rtol >= 1e-8).
 // A, B: r x r LDS buffers with leading dimension ld; Wg: the global W panel
 // (receives the reflectors, leading dimension r; untouched when vfree);
-// wload(idx): element idx (column-major, ld r) of the original W, used when
-// vfree. B = NULL (vfree only, W not read from Wg): one LDS buffer; X = R^H
+// wload(k, c): element (k, c) of the original W, used when vfree. B = NULL (vfree only, W not read from Wg): one LDS buffer; X = R^H
 // is staged through U (global) and Y goes to Wg, so the launch needs half
 // the LDS and fits twice the workgroups per CU. Outputs as jacobi_pinv_t.
 template <int kT, int kE, typename WLoad>
@@ -537,7 +536,7 @@ __device__ __forceinline__ void pinv_qr_lds(c128* A, c128* B, int r, int ld, dou
             if (s_w[j] != 0.0) {
 #pragma unroll 4
                 for (int l = 0; l < r; ++l) {
-                    const c128 w = wload((size_t)phys[l] * r + k), x = xj[l];
+                    const c128 w = wload(k, phys[l]), x = xj[l];
                     yr += w.re * x.re - w.im * x.im;
                     yi += w.re * x.im + w.im * x.re;
                 }

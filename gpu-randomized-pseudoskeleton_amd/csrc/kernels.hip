@@ -347,7 +347,16 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
         // one_buf (set by the host only with rtol >= 1e-8 and every core
         // gathered from C) implies the V-free form.
         const bool vfree = rtol >= 1e-8 || one_buf;
-        auto wload = [&](size_t idx) { return load_core(J, (int)idx, f32); };
+        // element (k, c) of W: row position rpos[k] of C's column c, or the W panel
+        const void* Cg = J.C;
+        const int32_t* rp = J.rpos;
+        const c128* Wp = J.W;
+        const int m = J.m;
+        auto wload = [=](int k, int c) -> c128 {
+            if (!rp) return Wp[(size_t)c * r + k];
+            const size_t at = (size_t)c * m + rp[k];
+            return f32 ? to_c128(((const c64*)Cg)[at]) : ((const c128*)Cg)[at];
+        };
         pinv_qr_lds<kT, kE>(L.a, L.b, r, ld, atol, rtol, vfree, wload, J.U, r, J.sigma, J.W, L.flag, L.w, L.phys,
                             L.norm, L.tau, L.alpha, &kept, &smax, label_eps, &reps, &sweeps);
         if (threadIdx.x == 0) {
