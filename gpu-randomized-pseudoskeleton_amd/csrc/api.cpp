@@ -1566,6 +1566,9 @@ struct farcur_batch {
     bool pinv_side_stream = true;
     std::vector<char> h_plan;
     std::vector<PackScratch> pack_sc;  // per chunk, reused across rounds
+    std::vector<BlockJob> jobs_buf;     // the round's job list, reused across rounds
+    std::vector<RoundBufs> bufs_buf;    // the round's buffers and results, reused across rounds
+    std::vector<JobOut> outs_buf;
     // Rounds captured as hipGraphs, keyed by plan content and launch
     // parameters (at most kMaxGraphs, least recently used evicted).
     static constexpr int kMaxGraphs = 8;
@@ -1580,45 +1583,37 @@ struct farcur_batch {
 
 namespace {
 
-int batch_alloc_round(farcur_batch* B, int r, int m, int n, RoundBufs& rb) {
+// A round's buffers of one block (and its h held-out rows), carved from one
+// arena allocation: sample ids and row positions, C, R, W (then the pinv's
+// scratch), V when r exceeds the LDS pinv, U, sigma, held-out ids, positions
+// and rows; each piece 256-byte aligned.
+int batch_alloc_round(farcur_batch* B, int r, int m, int n, int h, RoundBufs& rb) {
     const size_t ps = psize(B->f32);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t sz_ids = al((size_t)r * 4), sz_C = al((size_t)m * r * ps), sz_R = al((size_t)r * n * ps);
+    const size_t sz_rr = al((size_t)r * r * 16), sz_V = r > kPinvLdsMaxR ? al(pinv_v_elems(r) * 16) : 0;
+    const size_t sz_sig = al((size_t)r * 8), sz_h = h > 0 ? al((size_t)h * 4) : 0;
+    const size_t sz_H = h > 0 ? al((size_t)h * n * ps) : 0;
     void* p;
-    TRY(B->arena.alloc((size_t)r * 4, &p));
-    rb.cs = (int32_t*)p;
-    TRY(B->arena.alloc((size_t)r * 4, &p));
-    rb.rs = (int32_t*)p;
-    TRY(B->arena.alloc((size_t)r * 4, &p));
-    rb.rpos = (int32_t*)p;
-    TRY(B->arena.alloc((size_t)m * r * ps, &p));
-    rb.C = p;
-    TRY(B->arena.alloc((size_t)r * n * ps, &p));
-    rb.R = p;
-    TRY(B->arena.alloc((size_t)r * r * 16, &p));
-    rb.W = (c128*)p;
-    if (r > kPinvLdsMaxR) {
-        TRY(B->arena.alloc(pinv_v_elems(r) * 16, &p));
-        rb.V = (c128*)p;
-    } else {
-        rb.V = nullptr;
-    }
-    TRY(B->arena.alloc((size_t)r * r * 16, &p));
-    rb.U = (c128*)p;
-    TRY(B->arena.alloc((size_t)r * 8, &p));
-    rb.sigma = (double*)p;
-    rb.hs = rb.hpos = nullptr;
-    rb.H = nullptr;
-    return FARCUR_OK;
-}
-
-int batch_alloc_heldout(farcur_batch* B, int h, int n, RoundBufs& rb) {
-    if (h <= 0) return FARCUR_OK;
-    void* p;
-    TRY(B->arena.alloc((size_t)h * 4, &p));
-    rb.hs = (int32_t*)p;
-    TRY(B->arena.alloc((size_t)h * 4, &p));
-    rb.hpos = (int32_t*)p;
-    TRY(B->arena.alloc((size_t)h * n * psize(B->f32), &p));
-    rb.H = p;
+    TRY(B->arena.alloc(3 * sz_ids + sz_C + sz_R + 2 * sz_rr + sz_V + sz_sig + 2 * sz_h + sz_H, &p));
+    char* q = (char*)p;
+    auto take = [&](size_t bytes) {
+        char* at = q;
+        q += bytes;
+        return at;
+    };
+    rb.cs = (int32_t*)take(sz_ids);
+    rb.rs = (int32_t*)take(sz_ids);
+    rb.rpos = (int32_t*)take(sz_ids);
+    rb.C = take(sz_C);
+    rb.R = take(sz_R);
+    rb.W = (c128*)take(sz_rr);
+    rb.V = sz_V ? (c128*)take(sz_V) : nullptr;
+    rb.U = (c128*)take(sz_rr);
+    rb.sigma = (double*)take(sz_sig);
+    rb.hs = h > 0 ? (int32_t*)take(sz_h) : nullptr;
+    rb.hpos = h > 0 ? (int32_t*)take(sz_h) : nullptr;
+    rb.H = h > 0 ? (void*)take(sz_H) : nullptr;
     return FARCUR_OK;
 }
 
@@ -1744,14 +1739,16 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
                 std::vector<JobOut>& outs) {
     farcur_ctx* ctx = B->ctx;
     const auto t_plan0 = std::chrono::steady_clock::now();
-    std::vector<BlockJob> jobs;
+    // the job list keeps its capacity across rounds and calls (no fresh
+    // multi-megabyte allocation, and its page faults, per round)
+    std::vector<BlockJob>& jobs = B->jobs_buf;
+    jobs.clear();
     jobs.reserve(active.size());
     bufs.resize(active.size());
     double total_cost = 0;
     for (size_t i = 0; i < active.size(); ++i) {
         const int b = active[i];
-        TRY(batch_alloc_round(B, B->st[b].r, B->st[b].m, B->st[b].n, bufs[i]));
-        if (B->learned) TRY(batch_alloc_heldout(B, B->st[b].h, B->st[b].n, bufs[i]));
+        TRY(batch_alloc_round(B, B->st[b].r, B->st[b].m, B->st[b].n, B->learned ? B->st[b].h : 0, bufs[i]));
         jobs.push_back(make_job(B, b, bufs[i], o));
         total_cost += job_cost(jobs.back());
     }
@@ -1903,7 +1900,7 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
     }
     B->host_ms[2] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan1).count();
     B->kt.collect();
-    if (B->profile) B->rounds.push_back(std::move(jobs));
+    if (B->profile) B->rounds.push_back(jobs);
     return FARCUR_OK;
 }
 
@@ -2085,8 +2082,8 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
             s.h = heldout_rows(s.m, s.r, o.heldout);
             active[b] = b;
         }
-        std::vector<RoundBufs> bufs;
-        std::vector<JobOut> outs;
+        std::vector<RoundBufs>& bufs = B->bufs_buf;
+        std::vector<JobOut>& outs = B->outs_buf;
         bool first = true;
         while (!active.empty()) {
             TRY(batch_round(B, active, o, bufs, outs));
@@ -2106,8 +2103,8 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
     } else if (mode == FARCUR_MODE_BATCHED) {
         std::vector<int> active(B->nb);
         for (int b = 0; b < B->nb; ++b) active[b] = b;
-        std::vector<RoundBufs> bufs;
-        std::vector<JobOut> outs;
+        std::vector<RoundBufs>& bufs = B->bufs_buf;
+        std::vector<JobOut>& outs = B->outs_buf;
         while (!active.empty()) {
             TRY(batch_round(B, active, o, bufs, outs));
             std::vector<int> next;
