@@ -341,7 +341,7 @@ int farcur_batch_kernel_times(farcur_batch* batch, const char** names, double* m
 #define FARCUR_PINV_CLASSES 10
 typedef struct {
     int pinv_threads[FARCUR_PINV_CLASSES];
-    int fill_grid;
+    int fill_grid;     /* fill workgroups per launch (capped at the launch's tile count) */
     int resid_threads;
     double pinv_ms[FARCUR_PINV_CLASSES]; /* autotune: best time of the class's probe (ms) */
 } farcur_tuning;
