@@ -578,7 +578,7 @@ __global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, JobOut*
 
 void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, JobOut* out, hipStream_t s) {
     if (njobs <= 0) return;
-    const int grid = njobs < 8192 ? njobs : 8192;
+    const int grid = njobs < 65536 ? njobs : 65536;  // one workgroup per block up to 65536
     if (f32)
         hipLaunchKernelGGL(k_residual<c64>, dim3(grid), dim3(threads), 0, s, jobs, njobs, out);
     else
