@@ -68,21 +68,24 @@ __device__ __forceinline__ double team_sum_t(double v) {
     return v;
 }
 
-// 1 / sqrt(x) and 1 / x for x > 0 (finite): hardware estimate, then two
-// Newton-Raphson steps (y <- y + y (1 - x y^2) / 2, y <- y + y (1 - x y)).
+// 1 / sqrt(x) and 1 / x for x > 0 (finite): hardware estimate, then kSteps
+// Newton-Raphson steps (y <- y + y (1 - x y^2) / 2, y <- y + y (1 - x y));
+// each step doubles the correct bits, two give full double precision.
+template <int kSteps = 2>
 __device__ __forceinline__ double rsqrt_nr(double x) {
     double y = __builtin_amdgcn_rsq(x);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < kSteps; ++i) {
         const double e = fma(-x * y, y, 1.0);
         y = fma(0.5 * y, e, y);
     }
     return y;
 }
+template <int kSteps = 2>
 __device__ __forceinline__ double rcp_nr(double x) {
     double y = __builtin_amdgcn_rcp(x);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < kSteps; ++i) {
         const double e = fma(-x, y, 1.0);
         y = fma(y, e, y);
     }
@@ -97,11 +100,15 @@ __device__ __forceinline__ double rcp_nr(double x) {
 struct Rot {
     double c, s, er, ei;
 };
+// c and s come from t as c = 1 / sqrt(1 + t^2), s = c t, so the rotation is
+// orthogonal whatever t's rounding: t needs only enough accuracy to cancel
+// most of the coupling (the next sweep removes the rest), and its two
+// estimates take one Newton step each; e (|e| = 1) and c take two.
 __device__ __forceinline__ Rot rotation(double al, double be, double gr, double gi, double g2) {
     const double ig = rsqrt_nr(g2);
     const double zeta = 0.5 * (be - al) * ig;
     const double z2 = fma(zeta, zeta, 1.0);
-    const double t = (zeta >= 0 ? 1.0 : -1.0) * rcp_nr(fabs(zeta) + z2 * rsqrt_nr(z2));
+    const double t = (zeta >= 0 ? 1.0 : -1.0) * rcp_nr<1>(fabs(zeta) + z2 * rsqrt_nr<1>(z2));
     const double c = rsqrt_nr(fma(t, t, 1.0));
     return Rot{c, c * t, gr * ig, gi * ig};
 }

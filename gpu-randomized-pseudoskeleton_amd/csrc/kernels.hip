@@ -434,7 +434,7 @@ void launch_lds(int grid, int threads, size_t lds, hipStream_t s, const BlockJob
                        label_eps, f32, one_buf, out);
 }
 
-// E rounded up to a compiled register count; past 16 elements the team
+// E rounded up to a compiled register count (1, 2, 3, 4, 6, 8, 16); past 16 elements the team
 // streams its columns from LDS (kE = 0).
 template <int kT>
 void launch_lds_t(int E, int grid, int threads, size_t lds, hipStream_t s, const BlockJob* jobs,
@@ -442,7 +442,9 @@ void launch_lds_t(int E, int grid, int threads, size_t lds, hipStream_t s, const
                   int one_buf, JobOut* out) {
     if (E <= 1) launch_lds<kT, 1>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
     else if (E <= 2) launch_lds<kT, 2>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
+    else if (E <= 3) launch_lds<kT, 3>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
     else if (E <= 4) launch_lds<kT, 4>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
+    else if (E <= 6) launch_lds<kT, 6>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
     else if (E <= 8) launch_lds<kT, 8>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
     else if (E <= 16) launch_lds<kT, 16>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
     else launch_lds<kT, 0>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
