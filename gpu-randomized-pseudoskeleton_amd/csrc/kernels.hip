@@ -35,31 +35,38 @@ __device__ __forceinline__ U4 stream4(uint64_t seed, uint32_t block, uint32_t ro
 // other. Positions are recorded first and mapped to RWG ids afterwards, so the
 // scan carries no dependent global loads.
 __device__ void sample_rows_heldout(const BlockJob& J, uint64_t seed) {
-    const uint32_t m = J.m, h = J.h, r = J.r, N = m - h;
+    // the job's fields in registers: the scan's stores could otherwise alias
+    // the job record and force reloads every iteration
+    const uint32_t m = J.m, h = J.h, r = J.r, N = m - h, b = J.b, round = J.round;
+    int32_t* const hpos = J.hpos;
+    int32_t* const rpos = J.rpos;
     uint32_t goth = 0, got = 0, cand = 0;
     U4 oh{}, orow{};
     for (uint32_t t = 0; t < m && (goth < h || got < r); ++t) {
-        if ((t & 3u) == 0 && goth < h) oh = stream4(seed, J.b, J.round, kStreamHeldout, t);
+        if ((t & 3u) == 0 && goth < h) oh = stream4(seed, b, round, kStreamHeldout, t);
         if (goth < h) {
             const uint32_t l = t & 3u;
             const uint32_t u = l == 0 ? oh.x : (l == 1 ? oh.y : (l == 2 ? oh.z : oh.w));
             if (select_step(u, m - t, h - goth)) {
-                J.hpos[goth++] = t;
+                hpos[goth++] = t;
                 continue;
             }
         }
         if (got < r) {
-            if ((cand & 3u) == 0) orow = stream4(seed, J.b, J.round, kStreamRows, cand);
+            if ((cand & 3u) == 0) orow = stream4(seed, b, round, kStreamRows, cand);
             const uint32_t l = cand & 3u;
             const uint32_t u = l == 0 ? orow.x : (l == 1 ? orow.y : (l == 2 ? orow.z : orow.w));
-            if (select_step(u, N - cand, r - got)) J.rpos[got++] = t;
+            if (select_step(u, N - cand, r - got)) rpos[got++] = t;
         }
         ++cand;
     }
+    const int32_t* const rows = J.rows;
+    int32_t* const hs = J.hs;
+    int32_t* const rs = J.rs;
 #pragma unroll 8
-    for (uint32_t k = 0; k < goth; ++k) J.hs[k] = J.rows[J.hpos[k]];
+    for (uint32_t k = 0; k < goth; ++k) hs[k] = rows[hpos[k]];
 #pragma unroll 8
-    for (uint32_t k = 0; k < got; ++k) J.rs[k] = J.rows[J.rpos[k]];
+    for (uint32_t k = 0; k < got; ++k) rs[k] = rows[rpos[k]];
 }
 
 __global__ void k_sample(const BlockJob* __restrict__ jobs, int njobs, uint64_t seed) {
@@ -80,13 +87,13 @@ __global__ void k_sample(const BlockJob* __restrict__ jobs, int njobs, uint64_t 
     // the output), RWG ids after the scan
     int32_t* pos = kind == 1 && J.rpos ? J.rpos : out;
     const uint32_t purpose = kind ? kStreamRows : kStreamCols;
-    const uint32_t need = J.r;
+    const uint32_t need = J.r, b = J.b, round = J.round;
     uint32_t got = 0;
     for (uint32_t t4 = 0; t4 < N && got < need; t4 += 4) {
         U4 c;
         c.x = t4 >> 2;
-        c.y = (uint32_t)J.b;
-        c.z = ((uint32_t)J.round << 8) | purpose;
+        c.y = b;
+        c.z = (round << 8) | purpose;
         c.w = 0x46435552u;
         const U4 o = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
         const uint32_t w[4] = {o.x, o.y, o.z, o.w};
