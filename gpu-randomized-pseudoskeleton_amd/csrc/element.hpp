@@ -156,6 +156,17 @@ constexpr int kTileCols = 64;
 constexpr int kFillWaves = 4;
 constexpr int kFillThreads = kFillWaves * 64;
 
+// A second list appended to the LDS set of a wide-panel tile (lanes over
+// columns): the held-out rows after the sample rows, so that R and H, which
+// share the lane set, fill in one tile. Entry j >= n1 of the LDS set is RWG
+// ids[j - n1] and goes to out[i * ld + (j - n1)].
+struct LdsTail {
+    const int* ids;
+    int n;
+    void* out;
+    int ld;
+};
+
 // ---------------------------------------------------------------- FP64 tile
 struct FillTileLds {
     double col[kTileCols][kRecFields];
@@ -168,19 +179,26 @@ template <typename OutC, bool kLanesAreRows>
 __device__ __forceinline__ void fill_tile(OutC* __restrict__ out, int ld, const int* __restrict__ rows, int nr,
                                           const int* __restrict__ cols, int nc, int row0, int col0,
                                           const double* __restrict__ rec, double k, double c4k2, double scale,
-                                          FillTileLds& lds) {
+                                          FillTileLds& lds, LdsTail tail) {
     const int* lane_ids = kLanesAreRows ? rows : cols;
     const int* lds_ids = kLanesAreRows ? cols : rows;
     const int nlane = kLanesAreRows ? nr : nc;
-    const int nlds = kLanesAreRows ? nc : nr;
+    const int nl1 = kLanesAreRows ? nc : nr;
+    const int nlds = kLanesAreRows ? nl1 : nl1 + tail.n;
     const int lane0 = kLanesAreRows ? row0 : col0;
     const int lds0 = kLanesAreRows ? col0 : row0;
     const int tid = threadIdx.x;
     const int ncol = min(kTileCols, nlds - lds0);
+    auto lds_id = [&](int j) {
+        if constexpr (kLanesAreRows)
+            return lds_ids[j];
+        else
+            return j < nl1 ? lds_ids[j] : tail.ids[j - nl1];
+    };
     __syncthreads();  // the LDS tile may still be read by a previous tile
     for (int idx = tid; idx < ncol * kRecFields; idx += kFillThreads) {
         const int c = idx / kRecFields, f = idx - c * kRecFields;
-        lds.col[c][f] = rec[(size_t)lds_ids[lds0 + c] * kRecStride + f];
+        lds.col[c][f] = rec[(size_t)lds_id(lds0 + c) * kRecStride + f];
     }
     __syncthreads();
     const int lane = tid & (kWave - 1);
@@ -196,8 +214,10 @@ __device__ __forceinline__ void fill_tile(OutC* __restrict__ out, int ld, const 
         entry_f64(a, lds.col[c], k, c4k2, scale, zr, zi);
         if constexpr (kLanesAreRows)
             store_c(&out[(size_t)j * ld + i], zr, zi);
-        else
+        else if (j < nl1)
             store_c(&out[(size_t)i * ld + j], zr, zi);
+        else
+            store_c(&((OutC*)tail.out)[(size_t)i * tail.ld + (j - nl1)], zr, zi);
     }
 }
 
@@ -315,7 +335,7 @@ __device__ __forceinline__ void tri_item(FillTriLds& L, int a, int c, int it, co
 }
 
 // One tile of a panel, FP32 (panels out of OutC = c64). Same tile geometry
-// and arguments as fill_tile; lane_perm (or NULL for the list order) orders the
+// and arguments as fill_tile (including the LDS-set tail); lane_perm (or NULL for the list order) orders the
 // lane set so that a tile's 64 RWGs form a compact patch sharing triangles:
 // tile lane i takes position lane_perm[lane0 + i] of the lane set. Must be
 // called by all 256 threads.
@@ -324,11 +344,18 @@ __device__ __forceinline__ void fill_tile_tri(OutC* __restrict__ out, int ld, co
                                               const int* __restrict__ cols, int nc, int row0, int col0,
                                               const int32_t* __restrict__ lane_perm, const float* __restrict__ rec,
                                               const double* __restrict__ cen, const TriGeom& tg,
-                                              const F32Consts& fc, FillTriLds& L) {
+                                              const F32Consts& fc, FillTriLds& L, LdsTail tail) {
     const int* lane_ids = kLanesAreRows ? rows : cols;
     const int* lds_ids = kLanesAreRows ? cols : rows;
     const int nlane = kLanesAreRows ? nr : nc;
-    const int nlds = kLanesAreRows ? nc : nr;
+    const int nl1 = kLanesAreRows ? nc : nr;
+    const int nlds = kLanesAreRows ? nl1 : nl1 + tail.n;
+    auto lds_id = [&](int j) {
+        if constexpr (kLanesAreRows)
+            return lds_ids[j];
+        else
+            return j < nl1 ? lds_ids[j] : tail.ids[j - nl1];
+    };
     const int lane0 = kLanesAreRows ? row0 : col0;
     const int lds0 = kLanesAreRows ? col0 : row0;
     const int tid = threadIdx.x;
@@ -337,16 +364,16 @@ __device__ __forceinline__ void fill_tile_tri(OutC* __restrict__ out, int ld, co
     __syncthreads();  // the LDS of a previous tile may still be read
     for (int idx = tid; idx < ncol * kRecFields; idx += kFillThreads) {
         const int c = idx / kRecFields, f = idx - c * kRecFields;
-        L.src[c][f] = rec[(size_t)lds_ids[lds0 + c] * kRecStride + f];
+        L.src[c][f] = rec[(size_t)lds_id(lds0 + c) * kRecStride + f];
     }
     for (int idx = tid; idx < ncol * 6; idx += kFillThreads) {
         const int c = idx / 6, j = idx - 6 * c;
-        const float* u = rec + (size_t)lds_ids[lds0 + c] * kRecStride + j * 3;
+        const float* u = rec + (size_t)lds_id(lds0 + c) * kRecStride + j * 3;
         L.src[c][37 + j] = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     }
     for (int idx = tid; idx < ncol * 3; idx += kFillThreads) {
         const int c = idx / 3, x = idx - 3 * c;
-        L.srcc[c][x] = cen[(size_t)lds_ids[lds0 + c] * kCenStride + x];
+        L.srcc[c][x] = cen[(size_t)lds_id(lds0 + c) * kCenStride + x];
     }
     for (int idx = tid; idx < kTriHash; idx += kFillThreads) L.hkey[idx] = -1;
     if (tid == 0) L.ntri = 0;
@@ -435,8 +462,10 @@ __device__ __forceinline__ void fill_tile_tri(OutC* __restrict__ out, int ld, co
                 const int j = lds0 + c;
                 if constexpr (kLanesAreRows)
                     store_c(&out[(size_t)j * ld + i], kap * zi, -kap * zr);
-                else
+                else if (j < nl1)
                     store_c(&out[(size_t)i * ld + j], kap * zi, -kap * zr);
+                else
+                    store_c(&((OutC*)tail.out)[(size_t)i * tail.ld + (j - nl1)], kap * zi, -kap * zr);
             }
         }
         cdone = cnew;

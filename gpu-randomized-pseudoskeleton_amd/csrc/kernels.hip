@@ -164,13 +164,17 @@ void launch_pivot_rows(const BlockJob* jobs, int njobs, int f32, hipStream_t s) 
 // ---------------------------------------------------------------- fill
 __host__ __device__ static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-// C and W: lanes over rows; R and H (few rows, many columns): lanes over columns.
+// C and W: lanes over rows; R and H (few rows, many columns): lanes over
+// columns, R's r sample rows and H's h held-out rows in the same tiles.
 int fill_tiles(int m, int n, int r, int h, int panels) {
     int t = 0;
+    const int hh = (panels & kPanelH) && h > 0 ? h : 0;
     if (panels & kPanelC) t += cdiv(m, kTileRows) * cdiv(r, kTileCols);
     if (panels & kPanelW) t += cdiv(r, kTileRows) * cdiv(r, kTileCols);
-    if (panels & kPanelR) t += cdiv(n, kTileRows) * cdiv(r, kTileCols);
-    if ((panels & kPanelH) && h > 0) t += cdiv(n, kTileRows) * cdiv(h, kTileCols);
+    if (panels & kPanelR)
+        t += cdiv(n, kTileRows) * cdiv(r + hh, kTileCols);
+    else if (hh > 0)
+        t += cdiv(n, kTileRows) * cdiv(hh, kTileCols);
     return t;
 }
 
@@ -182,6 +186,8 @@ __device__ inline F32Consts fill_consts_f32(double k, double eta) {
     const double il = k / (2.0 * M_PI);
     return F32Consts{(float)il, (float)(4.0 / (k * k)), (float)(k * eta / (144.0 * M_PI)), il};
 }
+
+constexpr LdsTail kNoTail{nullptr, 0, nullptr, 0};
 
 // A tile of a panel in the fill's precision: FP64 per entry, FP32 through the
 // tile's distinct triangles.
@@ -200,13 +206,14 @@ template <typename T, typename OutC, bool kLanesAreRows>
 __device__ __forceinline__ void fill_any(OutC* out, int ld, const int* rows, int nr, const int* cols, int nc,
                                          int row0, int col0, const int32_t* lane_perm, const T* rec,
                                          const double* cen, const TriGeom& tg, double k, double eta,
-                                         typename FillLdsOf<T>::type& lds) {
+                                         typename FillLdsOf<T>::type& lds, LdsTail tail) {
     if constexpr (sizeof(T) == 8) {
         const FillConsts fc = fill_consts_f64(k, eta);
-        fill_tile<OutC, kLanesAreRows>(out, ld, rows, nr, cols, nc, row0, col0, rec, fc.k, fc.c4k2, fc.scale, lds);
+        fill_tile<OutC, kLanesAreRows>(out, ld, rows, nr, cols, nc, row0, col0, rec, fc.k, fc.c4k2, fc.scale, lds,
+                                       tail);
     } else {
         fill_tile_tri<OutC, kLanesAreRows>(out, ld, rows, nr, cols, nc, row0, col0, lane_perm, rec, cen, tg,
-                                           fill_consts_f32(k, eta), lds);
+                                           fill_consts_f32(k, eta), lds, tail);
     }
 }
 
@@ -232,7 +239,7 @@ __global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restric
             const int mt = cdiv(m, kTileRows), nC = mt * cdiv(r, kTileCols);
             if (local < nC) {
                 fill_any<T, PC, true>((PC*)J.C, m, J.rows, m, J.cs, r, (local % mt) * kTileRows,
-                                      (local / mt) * kTileCols, J.row_perm, rec, cen, tg, k, eta, lds);
+                                      (local / mt) * kTileCols, J.row_perm, rec, cen, tg, k, eta, lds, kNoTail);
                 continue;
             }
             local -= nC;
@@ -243,22 +250,26 @@ __global__ __launch_bounds__(kFillThreads) void k_fill(const BlockJob* __restric
             const int nW = rt * cdiv(r, kTileCols);
             if (local < nW) {
                 fill_any<T, c128, true>(J.W, r, J.rs, r, J.cs, r, (local % rt) * kTileRows, (local / rt) * kTileCols,
-                                        nullptr, rec, cen, tg, k, eta, lds);
+                                        nullptr, rec, cen, tg, k, eta, lds, kNoTail);
                 continue;
             }
             local -= nW;
         }
         if (panels_j & kPanelR) {
-            const int nR = nt * cdiv(r, kTileCols);
+            // R's sample rows and H's held-out rows share the lane set (the
+            // block columns): one tile range over the r + h rows
+            const int hh = (panels_j & kPanelH) && J.h > 0 ? J.h : 0;
+            const int nR = nt * cdiv(r + hh, kTileCols);
             if (local < nR) {
                 fill_any<T, PC, false>((PC*)J.R, r, J.rs, r, J.cols, n, (local / nt) * kTileCols,
-                                       (local % nt) * kTileRows, J.col_perm, rec, cen, tg, k, eta, lds);
+                                       (local % nt) * kTileRows, J.col_perm, rec, cen, tg, k, eta, lds,
+                                       LdsTail{J.hs, hh, J.H, J.h});
                 continue;
             }
-            local -= nR;
+            continue;  // no H-only tiles follow
         }
         fill_any<T, PC, false>((PC*)J.H, J.h, J.hs, J.h, J.cols, n, (local / nt) * kTileCols, (local % nt) * kTileRows,
-                               J.col_perm, rec, cen, tg, k, eta, lds);
+                               J.col_perm, rec, cen, tg, k, eta, lds, LdsTail{J.hs, 0, J.H, J.h});
     }
 }
 
@@ -283,7 +294,7 @@ __global__ __launch_bounds__(kFillThreads) void k_fill_panel(PC* __restrict__ Z,
                                                              double eta) {
     __shared__ typename FillLdsOf<T>::type lds;
     fill_any<T, PC, true>(Z, ld, rows, nr, cols, nc, blockIdx.x * kTileRows, blockIdx.y * kTileCols, nullptr, rec,
-                          cen, tg, k, eta, lds);
+                          cen, tg, k, eta, lds, kNoTail);
 }
 
 void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32_t* cols, int nc,
