@@ -537,7 +537,7 @@ void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt, hipStream_
         hipStream_t sc = side && (k++ & 1) == 0 ? side : s;
         kt.begin(4 + c, sc, &a);
         launch_pinv(P.jobs, P.list[c], P.nlist[c], P.atol, P.rtol, P.label_eps, kPinvClassMax[c], P.pinv_threads[c],
-                    P.f32, P.one_buf, P.out, sc);
+                    P.f32, P.one_buf, 1, P.out, sc);
         kt.end(4 + c, sc, a);
     }
     if (side) {
@@ -545,7 +545,8 @@ void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt, hipStream_
         (void)hipStreamWaitEvent(s, ctx->ev_pjoin, 0);
     }
     kt.begin(3, s, &a);
-    launch_residual(P.jobs, P.njobs, P.f32, P.resid_threads, P.out, s);
+    // held-out errors of the LDS-variant jobs came with their pinv launch
+    launch_residual(P.jobs, P.njobs, P.f32, P.resid_threads, 1, P.out, s);
     kt.end(3, s, a);
 }
 
@@ -918,7 +919,7 @@ int farcur_pinv(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, doubl
     Plan* dp = ctx->d_plan.as<Plan>();
     const int cls = pinv_class(r);
     const int threads = cls < kPinvClasses - 1 ? ctx->tune.pinv_threads[cls] : (r > 128 ? 1024 : 256);
-    launch_pinv(&dp->job, dp->list, 1, atol, rtol, 0.0, kPinvClassMax[cls], threads, 0, 0, &dp->out, ctx->stream);
+    launch_pinv(&dp->job, dp->list, 1, atol, rtol, 0.0, kPinvClassMax[cls], threads, 0, 0, 0, &dp->out, ctx->stream);
     HIPTRY(hipGetLastError());
     HIPTRY(hipMemcpyAsync(U, ctx->d_U.p, rr, hipMemcpyDeviceToHost, ctx->stream));
     std::vector<double> s(r);
@@ -967,7 +968,7 @@ int farcur_cur_residual(farcur_ctx* ctx, const farcur_c128* C, int m, int r, con
     TRY(ctx->d_plan.ensure(sizeof(Plan)));
     HIPTRY(hipMemcpyAsync(ctx->d_plan.p, &plan, sizeof(Plan), hipMemcpyHostToDevice, s));
     Plan* dp = ctx->d_plan.as<Plan>();
-    launch_residual(&dp->job, 1, 0, 1024, &dp->out, s);
+    launch_residual(&dp->job, 1, 0, 1024, 0, &dp->out, s);
     HIPTRY(hipGetLastError());
     HIPTRY(hipMemcpyAsync(y_new, ctx->d_y1.p, (size_t)m * 16, hipMemcpyDeviceToHost, s));
     HIPTRY(hipMemcpyAsync(&plan, ctx->d_plan.p, sizeof(Plan), hipMemcpyDeviceToHost, s));
@@ -2263,7 +2264,7 @@ int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* 
         HIPTRY(hipMemcpyAsync(W.p, W0.p, nprob * rr * 16, hipMemcpyDeviceToDevice, s));
         HIPTRY(hipEventRecord(ctx->ev0, s));
         launch_pinv((const BlockJob*)d, (const int32_t*)(d + bj), nprob, 1e-10, gathered ? 1e-6 : 0.0, 0.0, rmax,
-                    threads, 0, gathered ? 1 : 0, (JobOut*)(d + bj + bl + 256), s);
+                    threads, 0, gathered ? 1 : 0, 0, (JobOut*)(d + bj + bl + 256), s);
         HIPTRY(hipEventRecord(ctx->ev1, s));
         HIPTRY(hipGetLastError());
         HIPTRY(hipStreamSynchronize(s));

@@ -310,6 +310,9 @@ void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32
 }
 
 // ---------------------------------------------------------------- pinv
+template <typename PC>
+__device__ double heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, double* s_red);
+
 // Element idx (column-major, r x r) of the core W: the W panel, or the rows
 // rpos of C (the same entries, filled once).
 __device__ __forceinline__ c128 load_core(const BlockJob& J, int idx, int f32) {
@@ -348,8 +351,11 @@ size_t pinv_lds_bytes(int rmax, bool one_buf) {
 }
 
 template <int kT, int kE>
+// heldout: the workgroup also computes the held-out error of each job with
+// h > 0 (it has just written U; R, H and C are filled), reusing the core's
+// LDS buffer; the residual launch then skips those jobs.
 __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist, int rmax,
-                           double atol, double rtol, double label_eps, int f32, int one_buf,
+                           double atol, double rtol, double label_eps, int f32, int one_buf, int heldout,
                            JobOut* __restrict__ out) {
     extern __shared__ __align__(16) char smem[];
     const PinvLds L(smem, rmax, one_buf != 0);
@@ -382,6 +388,14 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
             out[jb].sigma_max = smax;
             out[jb].rank_eps = reps;
             out[jb].sweeps = sweeps;
+        }
+        if (heldout && J.h > 0) {
+            // group buffers: the two halves of the core buffer (rmax^2 complex,
+            // rmax >= 16, so each half holds >= 8 rows of r <= rmax)
+            const int cap = rmax * rmax / 2;
+            const double e = f32 ? heldout_error<c64>(J, L.a, L.a + cap, cap, L.norm)
+                                 : heldout_error<c128>(J, L.a, L.a + cap, cap, L.norm);
+            if (threadIdx.x == 0) out[jb].err = e;
         }
         __syncthreads();
     }
@@ -441,7 +455,8 @@ namespace {
 
 template <int kT, int kE>
 void launch_lds(int grid, int threads, size_t lds, hipStream_t s, const BlockJob* jobs, const int32_t* list,
-                int nlist, int rmax, double atol, double rtol, double label_eps, int f32, int one_buf, JobOut* out) {
+                int nlist, int rmax, double atol, double rtol, double label_eps, int f32, int one_buf, int heldout,
+                JobOut* out) {
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute((const void*)k_pinv_lds<kT, kE>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -449,7 +464,7 @@ void launch_lds(int grid, int threads, size_t lds, hipStream_t s, const BlockJob
         attr_set = true;
     }
     hipLaunchKernelGGL((k_pinv_lds<kT, kE>), dim3(grid), dim3(threads), lds, s, jobs, list, nlist, rmax, atol, rtol,
-                       label_eps, f32, one_buf, out);
+                       label_eps, f32, one_buf, heldout, out);
 }
 
 // E rounded up to a compiled register count (1, 2, 3, 4, 6, 8, 16); past 16 elements the team
@@ -457,21 +472,21 @@ void launch_lds(int grid, int threads, size_t lds, hipStream_t s, const BlockJob
 template <int kT>
 void launch_lds_t(int E, int grid, int threads, size_t lds, hipStream_t s, const BlockJob* jobs,
                   const int32_t* list, int nlist, int rmax, double atol, double rtol, double label_eps, int f32,
-                  int one_buf, JobOut* out) {
-    if (E <= 1) launch_lds<kT, 1>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
-    else if (E <= 2) launch_lds<kT, 2>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
-    else if (E <= 3) launch_lds<kT, 3>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
-    else if (E <= 4) launch_lds<kT, 4>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
-    else if (E <= 6) launch_lds<kT, 6>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
-    else if (E <= 8) launch_lds<kT, 8>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
-    else if (E <= 16) launch_lds<kT, 16>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
-    else launch_lds<kT, 0>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out);
+                  int one_buf, int heldout, JobOut* out) {
+    if (E <= 1) launch_lds<kT, 1>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
+    else if (E <= 2) launch_lds<kT, 2>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
+    else if (E <= 3) launch_lds<kT, 3>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
+    else if (E <= 4) launch_lds<kT, 4>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
+    else if (E <= 6) launch_lds<kT, 6>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
+    else if (E <= 8) launch_lds<kT, 8>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
+    else if (E <= 16) launch_lds<kT, 16>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
+    else launch_lds<kT, 0>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
 }
 
 }  // namespace
 
 void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, double label_eps,
-                 int rmax, int threads, int f32, int one_buf, JobOut* out, hipStream_t s) {
+                 int rmax, int threads, int f32, int one_buf, int heldout, JobOut* out, hipStream_t s) {
     if (nlist <= 0) return;
     const int grid = nlist < 16384 ? nlist : 16384;
     if (rmax > 0 && rmax <= kPinvLdsMaxR) {
@@ -481,11 +496,11 @@ void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double at
         const size_t lds = pinv_lds_bytes(rmax, one_buf != 0);
         switch (T) {
             case 1:
-            case 2: launch_lds_t<2>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out); break;
-            case 4: launch_lds_t<4>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out); break;
-            case 8: launch_lds_t<8>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out); break;
-            case 16: launch_lds_t<16>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out); break;
-            default: launch_lds_t<32>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, out); break;
+            case 2: launch_lds_t<2>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out); break;
+            case 4: launch_lds_t<4>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out); break;
+            case 8: launch_lds_t<8>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out); break;
+            case 16: launch_lds_t<16>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out); break;
+            default: launch_lds_t<32>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out); break;
         }
     } else {
         hipLaunchKernelGGL(k_pinv_global, dim3(grid), dim3(threads), 0, s, jobs, list, nlist, atol, rtol, label_eps,
@@ -566,14 +581,17 @@ __device__ double heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, 
     return den > 0 ? sqrt(num / den) : (num > 0 ? INFINITY : 0.0);
 }
 
+// skip_heldout: the pinv launches computed the held-out error of the jobs with
+// h > 0 and r <= kPinvLdsMaxR (their LDS variant); those are skipped here.
 template <typename PC>
-__global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, JobOut* __restrict__ out) {
+__global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, int skip_heldout, JobOut* __restrict__ out) {
     __shared__ c128 s_t1[kResidMaxR];
     __shared__ c128 s_t2[kResidMaxR];
     __shared__ c128 s_part[1024];
     __shared__ double s_red[16];
     for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
         const BlockJob& J = jobs[jb];
+        if (J.h > 0 && skip_heldout && J.r <= kPinvLdsMaxR) continue;
         if (J.h > 0) {
             // Beyond kResidMaxR the group buffers use the W panel (r*r >= 2r),
             // which the pinv has already consumed.
@@ -596,13 +614,14 @@ __global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, JobOut*
     }
 }
 
-void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, JobOut* out, hipStream_t s) {
+void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, int skip_heldout, JobOut* out,
+                     hipStream_t s) {
     if (njobs <= 0) return;
     const int grid = njobs < 65536 ? njobs : 65536;  // one workgroup per block up to 65536
     if (f32)
-        hipLaunchKernelGGL(k_residual<c64>, dim3(grid), dim3(threads), 0, s, jobs, njobs, out);
+        hipLaunchKernelGGL(k_residual<c64>, dim3(grid), dim3(threads), 0, s, jobs, njobs, skip_heldout, out);
     else
-        hipLaunchKernelGGL(k_residual<c128>, dim3(grid), dim3(threads), 0, s, jobs, njobs, out);
+        hipLaunchKernelGGL(k_residual<c128>, dim3(grid), dim3(threads), 0, s, jobs, njobs, skip_heldout, out);
 }
 
 // ---------------------------------------------------------------- test vectors

@@ -96,9 +96,12 @@ constexpr int kMaxSamples = 8192;
 // f32: panel precision of C (for jobs whose W is gathered from C). one_buf:
 // every listed job gathers W from C (rpos set), so with rtol >= 1e-8 the LDS
 // variant may use one r x r buffer (W panel and U as global scratch).
+// heldout: the LDS variant also computes the held-out error of jobs with h > 0
+// (launch_residual with skip_heldout then skips them).
 void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, double label_eps,
-                 int rmax, int threads, int f32, int one_buf, JobOut* out, hipStream_t s);
-void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, JobOut* out, hipStream_t s);
+                 int rmax, int threads, int f32, int one_buf, int heldout, JobOut* out, hipStream_t s);
+void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, int skip_heldout, JobOut* out,
+                     hipStream_t s);
 
 // Plain panel fill Z[i + j*ld] = Z(rows[i], cols[j]) (standalone, one panel).
 void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32_t* cols, int nc,
