@@ -12,9 +12,11 @@ no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import math
 import pathlib
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -45,7 +47,21 @@ def lib() -> ctypes.CDLL:
         except AttributeError as e:  # a stale build missing an entry point
             raise FarcurError(f"{LIB_PATH} does not export {e}; rebuild it (make -C {HERE})") from None
         _lib = L  # only a fully configured library is kept
+        # Python's exit hooks run before the C runtime's, so every live context
+        # (and the batches and models on it) is released while the HIP runtime
+        # is still up; objects left to interpreter teardown would otherwise
+        # find this module's globals cleared and keep their graphs, streams and
+        # pinned buffers until HIP's own teardown.
+        atexit.register(_close_all)
     return _lib
+
+
+_contexts = weakref.WeakSet()
+
+
+def _close_all():
+    for c in list(_contexts):
+        c.close()
 
 
 vp, ci, cd, cu64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint64
@@ -330,9 +346,15 @@ def _ivec(a) -> np.ndarray:
 class Context:
     def __init__(self, device=0):
         self._h = vp()
+        # batches and models created on this context: they hold its device
+        # and stream, so close() releases them first
+        self._children = weakref.WeakSet()
         check(lib().farcur_create(device, ctypes.byref(self._h)))
+        _contexts.add(self)
 
     def close(self):
+        for ch in list(getattr(self, "_children", ())):
+            ch.close()
         if getattr(self, "_h", None):
             lib().farcur_destroy(self._h)
             self._h = None
@@ -453,6 +475,7 @@ class Batch:
         self._h = vp()
         check(lib().farcur_batch_create(ctx._h, self.nblocks, _p(self.row_off), _p(self.row_ids), _p(self.col_off),
                                         _p(self.col_ids), ctypes.byref(self._h)))
+        ctx._children.add(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -552,6 +575,7 @@ class Model:
         self.ctx = ctx
         self._h = vp()
         check(lib().farcur_model_create(ctx._h, seed, ctypes.byref(self._h)))
+        ctx._children.add(self)
 
     def close(self):
         if getattr(self, "_h", None):

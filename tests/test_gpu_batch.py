@@ -167,3 +167,21 @@ def test_round_graphs_replay_matches_eager(farcur, sphere):
         ctx.fill(ri[ro[0]:ro[1]], cix[co[0]:co[1]])
     captures, replays = gb.graph_stats()
     assert captures == rounds and replays == 2 * rounds
+
+
+def test_context_close_releases_its_batches_first(farcur, sphere):
+    # A context of its own: closing it releases the batch and model created
+    # on it (which hold its device and stream) before the context itself, so
+    # a later use of the batch is an error rather than a use of freed state.
+    _, _, _, (ro, ri, co, cix) = sphere
+    ctx = farcur.Context(0)
+    ctx.set_mesh(farcur.Mesh.icosphere(1.0, 1.0, 1.0, 3, 0.01, 1).build_rwg())
+    bt = farcur.Batch(ctx, ro, ri, co, cix)
+    model = farcur.Model(ctx, seed=3)
+    bt.compress(farcur.MODE_BATCHED, tol=1e-4, seed=5, precision=farcur.F32, pinv_rtol=1e-6)
+    ctx.close()
+    assert not bt._h and not model._h
+    with pytest.raises(farcur.FarcurError):
+        bt.compress(farcur.MODE_BATCHED, tol=1e-4, seed=5)
+    bt.close()
+    model.close()
