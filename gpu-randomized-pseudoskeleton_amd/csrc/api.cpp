@@ -1521,8 +1521,12 @@ int distinct_triangles(const std::vector<int32_t>& tri_of_rwg, const int32_t* id
 // kernels and the JobOut download, replayed as one hipGraph launch. The entry
 // owns the device plan the kernels read and the pinned staging buffers the
 // memcpy nodes name, so nothing it references is freed or moved while cached.
+// The kernels read the plan from device memory at run time; what the graph
+// fixes are the values passed by value (counts, section offsets, thresholds,
+// launch shapes), which `key` holds byte for byte. A round whose key matches
+// replays the graph after writing its own plan into `h_plan`.
 struct RoundGraph {
-    uint64_t key = 0;
+    std::vector<char> key;
     hipGraphExec_t exec = nullptr;
     DBuf d_plan;
     char* h_plan = nullptr;
@@ -1569,6 +1573,7 @@ struct farcur_batch {
     // pinv r classes split over the context's two streams within a round
     bool pinv_side_stream = true;
     std::vector<char> h_plan;
+    std::vector<char> key_buf;  // graph key of the round being planned (kept for its capacity)
     std::vector<PackScratch> pack_sc;  // per chunk, reused across rounds
     std::vector<BlockJob> jobs_buf;     // the round's job list, reused across rounds
     std::vector<RoundBufs> bufs_buf;    // the round's buffers and results, reused across rounds
@@ -1778,13 +1783,15 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
     std::vector<size_t> off(nchunks + 1, 0);
     if ((int)B->pack_sc.size() < nchunks) B->pack_sc.resize(nchunks);
     for (int c = 0; c < nchunks; ++c) {
+        std::memset((void*)&pk[c], 0, sizeof(PackedRound));  // padding bytes go into the graph key
         pack_layout(jobs.data() + cut[c], cut[c + 1] - cut[c], o.row_mode == FARCUR_ROWS_PIVOT, pk[c], B->pack_sc[c]);
         off[c + 1] = off[c] + pk[c].size;  // sizes are multiples of 256
     }
     const size_t plan_bytes = off[nchunks];
-    if (B->h_plan.size() < plan_bytes) B->h_plan.resize(plan_bytes);
-    for (int c = 0; c < nchunks; ++c)
-        pack_write(jobs.data() + cut[c], cut[c + 1] - cut[c], pk[c], B->pack_sc[c], B->h_plan.data() + off[c]);
+    auto write_plan = [&](char* dst) {
+        for (int c = 0; c < nchunks; ++c)
+            pack_write(jobs.data() + cut[c], cut[c + 1] - cut[c], pk[c], B->pack_sc[c], dst + off[c]);
+    };
     hipStream_t s0 = ctx->stream, s1 = ctx->stream2;
     // Launch parameters that the kernels take by value.
     struct KeyParams {
@@ -1792,7 +1799,10 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
         double atol, rtol, label_eps, k, eta;
         uint64_t seed, geom;
         Tuning tune;
-    } kp{};
+        const void* geom_ptr[5];  // device geometry the fill takes by value
+        int side, unused;
+    } kp;
+    std::memset((void*)&kp, 0, sizeof kp);
     kp.f32 = B->f32;
     kp.pivot = o.row_mode == FARCUR_ROWS_PIVOT;
     kp.nchunks = nchunks;
@@ -1805,6 +1815,12 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
     kp.seed = o.seed;
     kp.geom = ctx->geom_hash;
     kp.tune = ctx->tune;
+    {
+        const FillParams fp = ctx->fill_params(B->f32);
+        const void* gp[5] = {fp.rec, fp.cen, fp.tri.tri, fp.tri.tcen, fp.tri.lrec};
+        std::memcpy(kp.geom_ptr, gp, sizeof gp);
+    }
+    kp.side = B->pinv_side_stream ? 1 : 0;
     // Enqueues the round: plan upload from `hplan` to `dplan`, kernels, JobOut
     // download to `hout`.
     std::vector<RoundPlan> plans(nchunks);
@@ -1843,19 +1859,28 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
     const auto t_plan1 = std::chrono::steady_clock::now();
     B->host_ms[1] += std::chrono::duration<double, std::milli>(t_plan1 - t_plan0).count();
     if (!B->use_graphs || B->profile) {
+        if (B->h_plan.size() < plan_bytes) B->h_plan.resize(plan_bytes);
+        write_plan(B->h_plan.data());
         TRY(B->d_plan.ensure(plan_bytes));
         TRY(enqueue(B->h_plan.data(), B->d_plan.p, outs.data()));
         HIPTRY(hipStreamSynchronize(s0));
     } else {
-        // The key covers the launch parameters and the plan size; the plan
-        // itself (megabytes for a full round) is compared exactly below.
-        const uint64_t key = fnv1a(0xcbf29ce484222325ull, &kp, sizeof kp) ^ (uint64_t)plan_bytes;
+        // Key: every value the captured launches take by value (the launch
+        // parameters, the chunk cuts and each chunk's section layout, counts
+        // and class sizes); the plan contents are read from device memory.
+        std::vector<char>& key = B->key_buf;
+        key.clear();
+        auto put_key = [&](const void* p, size_t n) { key.insert(key.end(), (const char*)p, (const char*)p + n); };
+        put_key(&kp, sizeof kp);
+        put_key(cut.data(), cut.size() * sizeof(int));
+        put_key(pk.data(), pk.size() * sizeof(PackedRound));
         RoundGraph* g = nullptr;
         for (RoundGraph& e : B->graphs)
-            if (e.exec && e.key == key && e.plan_bytes == plan_bytes && e.njobs == nj &&
-                std::memcmp(e.h_plan, B->h_plan.data(), plan_bytes) == 0)
-                g = &e;
+            if (e.exec && e.plan_bytes == plan_bytes && e.njobs == nj && e.key == key) g = &e;
         if (g) {
+            // the previous launch of this graph has completed (every round
+            // ends with a stream synchronisation), so its staging is free
+            write_plan(g->h_plan);
             ++B->graph_replays;
         } else {
             if ((int)B->graphs.size() < farcur_batch::kMaxGraphs) {
@@ -1874,7 +1899,7 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
             TRY(g->d_plan.ensure(plan_bytes));
             HIPTRY(hipHostMalloc((void**)&g->h_plan, plan_bytes, hipHostMallocDefault));
             HIPTRY(hipHostMalloc((void**)&g->h_out, std::max<size_t>(nj, 1) * sizeof(JobOut), hipHostMallocDefault));
-            std::memcpy(g->h_plan, B->h_plan.data(), plan_bytes);
+            write_plan(g->h_plan);
             hipGraph_t graph = nullptr;
             HIPTRY(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
             const int rc = enqueue(g->h_plan, g->d_plan.p, g->h_out);

@@ -185,3 +185,27 @@ def test_context_close_releases_its_batches_first(farcur, sphere):
         bt.compress(farcur.MODE_BATCHED, tol=1e-4, seed=5)
     bt.close()
     model.close()
+
+
+def test_round_graphs_across_changing_plans(farcur, sphere):
+    # Calls whose rounds share launch parameters but not plans (the same
+    # first round, different fallback rounds at another tolerance) replay
+    # cached graphs with their own plans: every call equals an eager run.
+    ctx, g, part, (ro, ri, co, cix) = sphere
+    base = dict(seed=41, precision=farcur.F32, pinv_rtol=1e-6, heldout=8)
+    eager = farcur.Batch(ctx, ro, ri, co, cix)
+    eager.set_graphs(False)
+    gb = farcur.Batch(ctx, ro, ri, co, cix)
+    pick = list(range(0, gb.nblocks, 11))
+    for tol in (1e-4, 3e-5, 1e-4, 3e-5):
+        eager.compress(farcur.MODE_LEARNED, tol=tol, **base)
+        gb.compress(farcur.MODE_LEARNED, tol=tol, **base)
+        ie, ig = eager.info_arrays(), gb.info_arrays()
+        for k in ("r", "rounds", "kept", "converged", "err", "heldout_err"):
+            np.testing.assert_array_equal(ig[k], ie[k])
+        for b in pick:
+            x, y = eager.block(b), gb.block(b)
+            for k in ("C", "U", "R", "rs", "cs"):
+                np.testing.assert_array_equal(x[k], y[k])
+    captures, replays = gb.graph_stats()
+    assert replays > 0
