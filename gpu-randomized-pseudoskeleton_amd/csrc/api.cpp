@@ -567,13 +567,21 @@ struct PackScratch {
     std::vector<int32_t> lists[kPinvClasses];
 };
 
-void pack_layout(const BlockJob* jobs, int nj, bool pivot, PackedRound& pk, PackScratch& sc) {
+// What the layout needs of a job: its sizes and whether it has row positions.
+struct JobDims {
+    int32_t m, n, r, h, rpos;
+};
+
+inline JobDims dims_of(const BlockJob& J) { return JobDims{J.m, J.n, J.r, J.h, J.rpos != nullptr}; }
+
+template <class Dims>
+void pack_layout(int nj, bool pivot, PackedRound& pk, PackScratch& sc, const Dims& dims) {
     sc.pa.assign(nj + 1, 0);
     sc.pwr.assign(nj + 1, 0);
     for (auto& l : sc.lists) l.clear();
     pk.all_rpos = 1;
     for (int j = 0; j < nj; ++j) {
-        const BlockJob& J = jobs[j];
+        const JobDims J = dims(j);
         if (!J.rpos) pk.all_rpos = 0;
         const int pw = J.rpos ? 0 : kPanelW;  // with row positions W is gathered from C, not filled
         const int all = pivot ? fill_tiles(J.m, J.n, J.r, J.h, kPanelC)
@@ -602,14 +610,15 @@ void pack_layout(const BlockJob* jobs, int nj, bool pivot, PackedRound& pk, Pack
     pk.total_wr = sc.pwr[nj];
 }
 
-void pack_write(const BlockJob* jobs, int nj, const PackedRound& pk, const PackScratch& sc, char* dst) {
-    size_t at = 0;
+// Every section after the job records (which the caller writes at
+// dst + pk.off_jobs = dst): prefixes, class lists, zeroed padding and JobOut slots.
+void pack_write_tail(int nj, const PackedRound& pk, const PackScratch& sc, char* dst) {
+    size_t at = pk.off_jobs + (size_t)nj * sizeof(BlockJob);
     auto put = [&](size_t off, const void* src, size_t bytes) {
         if (off > at) std::memset(dst + at, 0, off - at);  // padding
         if (bytes) std::memcpy(dst + off, src, bytes);
         at = off + bytes;
     };
-    put(pk.off_jobs, jobs, nj * sizeof(BlockJob));
     put(pk.off_pa, sc.pa.data(), sc.pa.size() * 4);
     put(pk.off_pwr, sc.pwr.data(), sc.pwr.size() * 4);
     for (int c = 0; c < kPinvClasses; ++c) put(pk.off_list[c], sc.lists[c].data(), sc.lists[c].size() * 4);
@@ -617,10 +626,15 @@ void pack_write(const BlockJob* jobs, int nj, const PackedRound& pk, const PackS
     std::memset(dst + pk.off_out, 0, pk.size - pk.off_out);  // JobOut slots (written by the kernels)
 }
 
+void pack_write(const BlockJob* jobs, int nj, const PackedRound& pk, const PackScratch& sc, char* dst) {
+    std::memcpy(dst + pk.off_jobs, jobs, (size_t)nj * sizeof(BlockJob));
+    pack_write_tail(nj, pk, sc, dst);
+}
+
 // Single-job rounds (per-block loop, drop-in, standalone calls).
 void pack_round(const std::vector<BlockJob>& jobs, bool pivot, PackedRound& pk, std::vector<char>& bytes) {
     PackScratch sc;
-    pack_layout(jobs.data(), (int)jobs.size(), pivot, pk, sc);
+    pack_layout((int)jobs.size(), pivot, pk, sc, [&](int j) { return dims_of(jobs[j]); });
     bytes.resize(pk.size);
     pack_write(jobs.data(), (int)jobs.size(), pk, sc, bytes.data());
 }
@@ -1575,7 +1589,7 @@ struct farcur_batch {
     std::vector<char> h_plan;
     std::vector<char> key_buf;  // graph key of the round being planned (kept for its capacity)
     std::vector<PackScratch> pack_sc;  // per chunk, reused across rounds
-    std::vector<BlockJob> jobs_buf;     // the round's job list, reused across rounds
+    std::vector<JobDims> dims_buf;      // the round's job sizes, reused across rounds
     std::vector<RoundBufs> bufs_buf;    // the round's buffers and results, reused across rounds
     std::vector<JobOut> outs_buf;
     // Rounds captured as hipGraphs, keyed by plan content and launch
@@ -1733,9 +1747,9 @@ void advance(BState& s, const JobOut& out, const RoundBufs& rb, double tol) {
 
 // Cost estimate of a job for balancing pipeline chunks: fill entries (36 Green
 // evaluations each) plus the pinv (~ r^3 x sweeps, in the same units).
-double job_cost(const BlockJob& J) {
-    const double entries = (double)(J.m + J.n + J.r) * J.r + (double)J.h * J.n;
-    return 36.0 * entries + 60.0 * (double)J.r * J.r * J.r;
+double job_cost(int m, int n, int r, int h) {
+    const double entries = (double)(m + n + r) * r + (double)h * n;
+    return 36.0 * entries + 60.0 * (double)r * r * r;
 }
 
 // One round over the active blocks. With pipelining, the jobs are cut into
@@ -1748,49 +1762,57 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
                 std::vector<JobOut>& outs) {
     farcur_ctx* ctx = B->ctx;
     const auto t_plan0 = std::chrono::steady_clock::now();
-    // the job list keeps its capacity across rounds and calls (no fresh
-    // multi-megabyte allocation, and its page faults, per round)
-    std::vector<BlockJob>& jobs = B->jobs_buf;
-    jobs.clear();
-    jobs.reserve(active.size());
-    bufs.resize(active.size());
-    double total_cost = 0;
-    for (size_t i = 0; i < active.size(); ++i) {
-        const int b = active[i];
-        TRY(batch_alloc_round(B, B->st[b].r, B->st[b].m, B->st[b].n, B->learned ? B->st[b].h : 0, bufs[i]));
-        jobs.push_back(make_job(B, b, bufs[i], o));
-        total_cost += job_cost(jobs.back());
+    // Sizes and buffers first; the job records (192 bytes each) are built
+    // once, straight into the plan staging the round uploads, after the
+    // layout and the graph are known. The size list keeps its capacity
+    // across rounds and calls.
+    const int nj = (int)active.size();
+    bufs.resize(nj);
+    std::vector<JobDims>& dims = B->dims_buf;
+    dims.resize(nj);
+    for (int i = 0; i < nj; ++i) {
+        const BState& st = B->st[active[i]];
+        const int h = B->learned ? st.h : 0;
+        TRY(batch_alloc_round(B, st.r, st.m, st.n, h, bufs[i]));
+        dims[i] = JobDims{st.m, st.n, st.r, h, 1};  // batch jobs always carry row positions
     }
-    const int nj = (int)jobs.size();
     int nchunks = 1;
     if (!B->profile && B->pipeline_chunks > 1 && nj >= 64 * B->pipeline_chunks) nchunks = B->pipeline_chunks;
     // chunk boundaries at equal cost
     std::vector<int> cut{0};
-    {
+    if (nchunks > 1) {
+        auto cost = [&](int j) { return job_cost(dims[j].m, dims[j].n, dims[j].r, dims[j].h); };
+        double total_cost = 0;
+        for (int j = 0; j < nj; ++j) total_cost += cost(j);
         double acc = 0;
         int c = 1;
         for (int j = 0; j < nj && c < nchunks; ++j) {
-            acc += job_cost(jobs[j]);
+            acc += cost(j);
             if (acc >= total_cost * c / nchunks) {
                 cut.push_back(j + 1);
                 ++c;
             }
         }
-        if (cut.back() != nj) cut.push_back(nj);
     }
+    if (cut.back() != nj) cut.push_back(nj);
     nchunks = (int)cut.size() - 1;
     std::vector<PackedRound> pk(nchunks);
     std::vector<size_t> off(nchunks + 1, 0);
     if ((int)B->pack_sc.size() < nchunks) B->pack_sc.resize(nchunks);
     for (int c = 0; c < nchunks; ++c) {
         std::memset((void*)&pk[c], 0, sizeof(PackedRound));  // padding bytes go into the graph key
-        pack_layout(jobs.data() + cut[c], cut[c + 1] - cut[c], o.row_mode == FARCUR_ROWS_PIVOT, pk[c], B->pack_sc[c]);
+        const JobDims* dc = dims.data() + cut[c];
+        pack_layout(cut[c + 1] - cut[c], o.row_mode == FARCUR_ROWS_PIVOT, pk[c], B->pack_sc[c],
+                    [dc](int j) { return dc[j]; });
         off[c + 1] = off[c] + pk[c].size;  // sizes are multiples of 256
     }
     const size_t plan_bytes = off[nchunks];
     auto write_plan = [&](char* dst) {
-        for (int c = 0; c < nchunks; ++c)
-            pack_write(jobs.data() + cut[c], cut[c + 1] - cut[c], pk[c], B->pack_sc[c], dst + off[c]);
+        for (int c = 0; c < nchunks; ++c) {
+            BlockJob* rec = (BlockJob*)(dst + off[c] + pk[c].off_jobs);
+            for (int j = cut[c]; j < cut[c + 1]; ++j) rec[j - cut[c]] = make_job(B, active[j], bufs[j], o);
+            pack_write_tail(cut[c + 1] - cut[c], pk[c], B->pack_sc[c], dst + off[c]);
+        }
     };
     hipStream_t s0 = ctx->stream, s1 = ctx->stream2;
     // Launch parameters that the kernels take by value.
@@ -1929,7 +1951,11 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
     }
     B->host_ms[2] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan1).count();
     B->kt.collect();
-    if (B->profile) B->rounds.push_back(jobs);
+    if (B->profile) {
+        std::vector<BlockJob> jobs(nj);
+        for (int j = 0; j < nj; ++j) jobs[j] = make_job(B, active[j], bufs[j], o);
+        B->rounds.push_back(std::move(jobs));
+    }
     return FARCUR_OK;
 }
 
@@ -2050,6 +2076,7 @@ extern "C" int farcur_batch_profile(farcur_batch* B, int enable) {
 }
 
 extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, int mode, double* wall_ms) {
+    const auto t0 = std::chrono::steady_clock::now();  // the whole call, state set-up included
     if (!B) return fail(FARCUR_EINVAL, "NULL batch");
     TRY(validate_opts(opts));
     if (mode != FARCUR_MODE_PLAIN && mode != FARCUR_MODE_BATCHED && mode != FARCUR_MODE_LEARNED)
@@ -2081,7 +2108,6 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
         s.r = std::min(o.r_start > 0 ? o.r_start : r_start_rule(D), s.cap);
     }
     B->arena.reset();
-    const auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = ctx->stream;
     launch_vtest(B->d_v.as<c128>(), B->d_vo.as<int32_t>(), B->nb, o.seed, st);
     launch_ones(B->d_ya.as<c128>(), (size_t)B->row_off[B->nb], st);
