@@ -1602,6 +1602,12 @@ struct farcur_batch {
     // Interpolation matrices P_b = C_b U_b for the product (arena memory).
     std::vector<void*> interp;
     bool interp_ready = false;
+    // SVD recompression of the last result: outputs (C', R', sigma per block at
+    // rc_off, k in rc_k), workspace and job records
+    DBuf d_rc, d_rc_ws, d_rc_jobs;
+    std::vector<int32_t> rc_k;
+    std::vector<size_t> rc_off;
+    bool rc_ready = false;
 };
 
 namespace {
@@ -2062,11 +2068,125 @@ extern "C" void farcur_batch_destroy(farcur_batch* B) {
     hipSetDevice(B->ctx->device);
     hipStreamSynchronize(B->ctx->stream);
     DBuf* bufs[] = {&B->d_rows, &B->d_cols, &B->d_rperm, &B->d_cperm, &B->d_v,    &B->d_ya,  &B->d_yb,
-                    &B->d_vo,   &B->d_plan, &B->d_feat, &B->d_cap, &B->d_rpred};
+                    &B->d_vo,   &B->d_plan, &B->d_feat, &B->d_cap, &B->d_rpred, &B->d_rc,  &B->d_rc_ws,
+                    &B->d_rc_jobs};
     for (DBuf* b : bufs) b->release();
     for (RoundGraph& g : B->graphs) g.release();
     B->arena.release();
     delete B;
+}
+
+// Byte offsets of one block's recompression outputs within its region:
+// C' (m x r), R' (r x n) in panel precision, sigma (r doubles), k.
+struct RcLayout {
+    size_t c, r, sigma, k, size;
+    RcLayout(int m, int n, int rr, size_t ps) {
+        auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        c = 0;
+        r = al((size_t)m * rr * ps);
+        sigma = r + al((size_t)rr * n * ps);
+        k = sigma + al((size_t)rr * 8);
+        size = k + 256;
+    }
+};
+
+// SVD recompression of every block of the last result (svd_compress.m:19-34
+// through postcompress_wrapper.m:1-5): C' (m x k), sigma (k), R' (k x n) with
+// C U R = C' diag(sigma) R' up to the dropped singular values.
+extern "C" int farcur_batch_recompress(farcur_batch* B, double thresh, int* ranks, double* ms) {
+    if (!B) return fail(FARCUR_EINVAL, "NULL batch");
+    if (!B->has_result) return fail(FARCUR_ESTATE, "no batch result");
+    if (!(thresh >= 0) || !std::isfinite(thresh)) return fail(FARCUR_EINVAL, "thresh must be finite and >= 0");
+    farcur_ctx* ctx = B->ctx;
+    TRY(ctx->select());
+    const size_t ps = psize(B->f32);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    // outputs: per block C' (m x r), R' (r x n) in panel precision, sigma (r), k
+    B->rc_off.assign(B->nb + 1, 0);
+    for (int b = 0; b < B->nb; ++b) {
+        const BState& s = B->st[b];
+        if (s.fin_r > kRecompressMaxR)
+            return fail(FARCUR_EINVAL, "block %d: r = %d exceeds the recompression bound %d", b, s.fin_r,
+                        kRecompressMaxR);
+        B->rc_off[b + 1] = B->rc_off[b] + RcLayout(s.m, s.n, s.fin_r, ps).size;
+    }
+    TRY(B->d_rc.ensure(B->rc_off[B->nb]));
+    // workspace reused by chunks of blocks of at most ~1 GiB together
+    const size_t ws_cap = (size_t)1 << 26;  // c128 elements
+    size_t ws_max = 0;
+    std::vector<RecompressJob> jobs(B->nb);
+    std::vector<size_t> wsoff(B->nb);
+    std::vector<int> cut{0};
+    size_t ws_acc = 0;
+    for (int b = 0; b < B->nb; ++b) {
+        const BState& s = B->st[b];
+        const size_t w = al(recompress_ws_elems(s.m, s.n, s.fin_r) * 16) / 16;
+        if (ws_acc > 0 && ws_acc + w > ws_cap) {
+            cut.push_back(b);
+            ws_acc = 0;
+        }
+        char* o = (char*)B->d_rc.p + B->rc_off[b];
+        RecompressJob& J = jobs[b];
+        J.m = s.m;
+        J.n = s.n;
+        J.r = s.fin_r;
+        J.pad = 0;
+        J.C = s.fin.C;
+        J.U = s.fin.U;
+        J.R = s.fin.R;
+        wsoff[b] = ws_acc;  // workspace offset, based below
+        const RcLayout L(s.m, s.n, s.fin_r, ps);
+        J.Cout = o + L.c;
+        J.Rout = o + L.r;
+        J.sigma = (double*)(o + L.sigma);
+        J.k_out = (int32_t*)(o + L.k);
+        ws_acc += w;
+        ws_max = std::max(ws_max, ws_acc);
+    }
+    cut.push_back(B->nb);
+    TRY(B->d_rc_ws.ensure(std::max<size_t>(ws_max, 1) * 16));
+    for (int b = 0; b < B->nb; ++b) jobs[b].ws = B->d_rc_ws.as<c128>() + wsoff[b];
+    TRY(B->d_rc_jobs.ensure(jobs.size() * sizeof(RecompressJob)));
+    hipStream_t st = ctx->stream;
+    HIPTRY(hipMemcpyAsync(B->d_rc_jobs.p, jobs.data(), jobs.size() * sizeof(RecompressJob), hipMemcpyHostToDevice,
+                          st));
+    HIPTRY(hipEventRecord(ctx->ev0, st));
+    for (size_t c = 0; c + 1 < cut.size(); ++c)
+        launch_recompress(B->d_rc_jobs.as<RecompressJob>() + cut[c], cut[c + 1] - cut[c], thresh, B->f32, st);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipEventRecord(ctx->ev1, st));
+    B->rc_k.assign(B->nb, 0);
+    for (int b = 0; b < B->nb; ++b)
+        HIPTRY(hipMemcpyAsync(&B->rc_k[b], jobs[b].k_out, 4, hipMemcpyDeviceToHost, st));
+    HIPTRY(hipStreamSynchronize(st));
+    float t = 0;
+    HIPTRY(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    if (ms) *ms = t;
+    if (ranks) std::copy(B->rc_k.begin(), B->rc_k.end(), ranks);
+    B->rc_ready = true;
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_batch_recompressed_block(farcur_batch* B, int b, farcur_c128* C, double* sigma, farcur_c128* R,
+                                               int* k) {
+    if (!B) return fail(FARCUR_EINVAL, "NULL batch");
+    if (!B->rc_ready) return fail(FARCUR_ESTATE, "no recompression of the current result");
+    if (b < 0 || b >= B->nb) return fail(FARCUR_EINVAL, "block %d out of range", b);
+    farcur_ctx* ctx = B->ctx;
+    TRY(ctx->select());
+    const BState& s = B->st[b];
+    const int kk = B->rc_k[b];
+    if (k) *k = kk;
+    const RcLayout L(s.m, s.n, s.fin_r, psize(B->f32));
+    const char* o = (const char*)B->d_rc.p + B->rc_off[b];
+    hipStream_t st = ctx->stream;
+    if (kk > 0) {
+        TRY(copy_panel(C, o + L.c, (size_t)s.m * kk, B->f32, st));
+        TRY(copy_panel(R, o + L.r, (size_t)kk * s.n, B->f32, st));
+        if (sigma) HIPTRY(hipMemcpyAsync(sigma, o + L.sigma, (size_t)kk * 8, hipMemcpyDeviceToHost, st));
+    }
+    HIPTRY(hipStreamSynchronize(st));
+    return FARCUR_OK;
 }
 
 extern "C" int farcur_batch_profile(farcur_batch* B, int enable) {
@@ -2090,6 +2210,7 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
     B->learned = mode == FARCUR_MODE_LEARNED;
     B->first_round_misses = 0;
     B->has_result = false;
+    B->rc_ready = false;
     B->interp_ready = false;
     B->rounds.clear();
     B->kt.clear();

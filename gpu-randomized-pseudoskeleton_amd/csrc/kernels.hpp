@@ -54,6 +54,24 @@ struct JobOut {
     int32_t pad;
 };
 
+// One block of an SVD recompression (recompress.hip).
+struct RecompressJob {
+    int32_t m, n, r, pad;
+    const void* C;  // m x r (ld m), panel precision
+    const c128* U;  // r x r (ld r)
+    const void* R;  // r x n (ld r), panel precision
+    c128* ws;       // recompress_ws_elems(m, n, r) elements
+    void* Cout;     // m x k (ld m), panel precision
+    void* Rout;     // k x n (ld k), panel precision
+    double* sigma;  // k kept singular values, descending
+    int32_t* k_out; // k
+};
+
+// Largest r the recompression takes, and its workspace in c128 elements.
+constexpr int kRecompressMaxR = 1024;
+size_t recompress_ws_elems(int m, int n, int r);
+void launch_recompress(const RecompressJob* jobs, int njobs, double thresh, int f32, hipStream_t s);
+
 struct FillParams {
     const void* rec;     // RWG records (double or float SoA, kRecFields x ne)
     const double* cen;   // RWG centres, 3 x ne SoA (used by the FP32 fill)

@@ -165,6 +165,8 @@ def _setup(L: ctypes.CDLL) -> None:
         L.farcur_batch_compress.argtypes = [vp, vp, ci, vp]
         L.farcur_batch_info.argtypes = [vp, vp]
         L.farcur_batch_block.argtypes = [vp, ci, vp, vp, vp, vp, vp]
+        L.farcur_batch_recompress.argtypes = [vp, cd, vp, vp]
+        L.farcur_batch_recompressed_block.argtypes = [vp, ci, vp, vp, vp, vp]
         L.farcur_batch_profile.argtypes = [vp, ci]
         L.farcur_batch_kernel_times.argtypes = [vp, vp, vp, vp, vp, ci]
         L.farcur_batch_features.argtypes = [vp, vp]
@@ -512,6 +514,29 @@ class Batch:
         cs = np.empty(r, dtype=np.int32)
         check(lib().farcur_batch_block(self._h, b, _p(C), _p(U), _p(R), _p(rs), _p(cs)))
         return {"C": C, "U": U, "R": R, "rs": rs, "cs": cs}
+
+    def recompress(self, thresh):
+        """SVD recompression of every block of the last compress (svd_compress.m:19-34):
+        returns (k per block, device ms)."""
+        ranks = np.empty(self.nblocks, dtype=np.int32)
+        ms = cd()
+        check(lib().farcur_batch_recompress(self._h, float(thresh), _p(ranks), ctypes.byref(ms)))
+        return ranks, ms.value
+
+    def recompressed_block(self, b):
+        """Block b of the last recompression: C' (m x k), s (k), R' (k x n), with
+        C U R ~ C' diag(s) R'."""
+        m = int(self.row_off[b + 1] - self.row_off[b])
+        n = int(self.col_off[b + 1] - self.col_off[b])
+        k = ci()
+        check(lib().farcur_batch_recompressed_block(self._h, b, None, None, None, ctypes.byref(k)))
+        k = k.value
+        C = np.empty((m, k), dtype=np.complex128, order="F")
+        s = np.empty(k)
+        R = np.empty((k, n), dtype=np.complex128, order="F")
+        if k:
+            check(lib().farcur_batch_recompressed_block(self._h, b, _p(C), _p(s), _p(R), None))
+        return {"C": C, "s": s, "R": R}
 
     def features(self) -> np.ndarray:
         X = np.empty((self.nblocks, 8), dtype=np.float32)

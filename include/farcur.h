@@ -244,6 +244,22 @@ int farcur_batch_info(farcur_batch* batch, farcur_block_info* infos);
 int farcur_batch_block(farcur_batch* batch, int b, farcur_c128* C, farcur_c128* U, farcur_c128* R, int* row_samples,
                        int* col_samples);
 
+/* SVD recompression of every block of the last compress to its relative rank
+ * (svd_compress.m:19-34, reached through postcompress_wrapper.m:1-5; called as
+ * matlab_postcompress(C, U, R, 0.1 * err) at
+ * compres_single_block_spheres_for_lambda_sweep.jl:568). Per block:
+ * C = Q_c R_c and R^H = Q_r R_r (Householder QR), R_c U R_r^H = u diag(s) v^H,
+ * the k values with s_j / s_1 > thresh kept (strictly greater, as
+ * find(drs > thresh)), C' = Q_c u_k (m x k), U' = diag(s_k), R' = (Q_r v_k)^H
+ * (k x n). FP64 arithmetic; C' and R' are stored in the panel precision of the
+ * compress. ranks (nblocks, may be NULL) receives k per block, *ms (may be
+ * NULL) the device time. Blocks with r above 1024 are rejected. */
+int farcur_batch_recompress(farcur_batch* batch, double thresh, int* ranks, double* ms);
+/* Block b of the last recompression: C' (m x k), sigma (k, descending),
+ * R' (k x n), *k; any pointer may be NULL. U' is diag(sigma). */
+int farcur_batch_recompressed_block(farcur_batch* batch, int b, farcur_c128* C, double* sigma, farcur_c128* R,
+                                    int* k);
+
 /* Eight features per block (nblocks x 8, row-major) for the rank model:
  * diameter of the row group / lambda, diameter of the column group / lambda,
  * centre distance / lambda, cosines between each group's mean normal and the

@@ -252,3 +252,25 @@ def num_threads() -> int:
 
 def set_threads(n: int) -> None:
     lib().orc_set_threads(int(n))
+
+
+def svd_compress(C, U, R, thresh):
+    """SVD recompression of C U R (codes/ACA_vs_CUR/seed/svd_compress.m:19-34,
+    called through postcompress_wrapper.m:1-5): QR of C and of R^H (economy),
+    SVD of S = R_c U R_r^H, kept s_j / s_1 > thresh (strictly), returns
+    (C' = Q_c u_k, s_k, R' = (Q_r v_k)^H). The reference's qr_cbd (absent from
+    the tree) is taken as an economy QR: any orthonormal factorisation gives the
+    same C' diag(s) R' and s. Pins: the singular values of the m x n product,
+    orthonormal C' and R'^H, the Eckart-Young residual, exact reconstruction at
+    thresh 0 (tests/test_oracle.py)."""
+    C = np.asarray(C, dtype=np.complex128)
+    U = np.asarray(U, dtype=np.complex128)
+    R = np.asarray(R, dtype=np.complex128)
+    qc, rc = np.linalg.qr(C)
+    qr_, rr = np.linalg.qr(R.conj().T)
+    u, s, vh = np.linalg.svd(rc @ U @ rr.conj().T)
+    if s.size == 0 or s[0] == 0:
+        k = 0
+    else:
+        k = int(np.sum(s / s.max() > thresh))
+    return qc @ u[:, :k], s[:k], (qr_ @ vh.conj().T[:, :k]).conj().T

@@ -320,3 +320,53 @@ def test_element_dipole_limit(farcur, oracle, offset):
     want = _dipole_field_dot(pm, pn, rm, rn, k, eta)
     got = g.element(0, 1)
     assert abs(got - want) <= 1e-4 * abs(want)
+
+
+def _cur_like(rng, m, n, r, decay):
+    # C U R with a graded spectrum, like a far-field CUR (columns and rows of
+    # a numerically low-rank block, U their core's inverse)
+    Qa, _ = np.linalg.qr(rng.standard_normal((m, r)) + 1j * rng.standard_normal((m, r)))
+    Qb, _ = np.linalg.qr(rng.standard_normal((n, r)) + 1j * rng.standard_normal((n, r)))
+    s = decay ** np.arange(r)
+    Z = (Qa * s) @ Qb.conj().T
+    cs, rs = rng.choice(n, r, replace=False), rng.choice(m, r, replace=False)
+    C, R = Z[:, cs], Z[rs, :]
+    U = np.linalg.pinv(Z[np.ix_(rs, cs)], rcond=1e-13)
+    return C, U, R
+
+
+@pytest.mark.parametrize("thresh", [0.0, 1e-6, 1e-3])
+def test_svd_compress_pins(oracle, thresh):
+    # svd_compress.m:19-34 pinned by what the mathematics fixes: the kept
+    # singular values are those of the m x n product (computed another way),
+    # C' and R'^H have orthonormal columns, the residual is the Eckart-Young
+    # one, and nothing is dropped at thresh 0.
+    rng = np.random.default_rng(3)
+    C, U, R = _cur_like(rng, 90, 70, 24, 0.55)
+    A = C @ U @ R
+    Cp, s, Rp = oracle.svd_compress(C, U, R, thresh)
+    sa = np.linalg.svd(A, compute_uv=False)
+    k = int(np.sum(sa / sa[0] > thresh)) if thresh > 0 else min(24, int(np.sum(sa > 0)))
+    assert s.size == k
+    np.testing.assert_allclose(s, sa[:k], rtol=1e-8, atol=1e-13 * sa[0])
+    np.testing.assert_allclose(Cp.conj().T @ Cp, np.eye(k), atol=1e-10)
+    np.testing.assert_allclose(Rp @ Rp.conj().T, np.eye(k), atol=1e-10)
+    resid = np.linalg.norm(Cp @ np.diag(s) @ Rp - A)
+    assert abs(resid - np.linalg.norm(sa[k:24])) <= 1e-9 * sa[0]
+    # a transposed or unconjugated R' misses the product by O(1)
+    assert np.linalg.norm(Cp @ np.diag(s) @ Rp.conj() - A) > 0.1 * np.linalg.norm(A)
+
+
+def test_svd_compress_planted_rank(oracle):
+    rng = np.random.default_rng(4)
+    C = rng.standard_normal((40, 5)) + 1j * rng.standard_normal((40, 5))
+    Rf = rng.standard_normal((5, 30)) + 1j * rng.standard_normal((5, 30))
+    # rank 5 product through an r = 12 CUR-shaped factorisation
+    P = rng.standard_normal((5, 12)) + 1j * rng.standard_normal((5, 12))
+    Cb = C @ P
+    Rb = P.conj().T @ Rf
+    Ub = np.linalg.pinv(P) @ np.linalg.pinv(P.conj().T)
+    A = Cb @ Ub @ Rb
+    Cp, s, Rp = oracle.svd_compress(Cb, Ub, Rb, 1e-8)
+    assert s.size == 5
+    assert np.linalg.norm(Cp @ np.diag(s) @ Rp - A) <= 1e-10 * np.linalg.norm(A)
