@@ -123,6 +123,32 @@ def accuracy(fc, ctx, batch, blocks, nsample, seed=0):
     return float(np.max(errs)), float(np.median(errs))
 
 
+def recompress_bench(fc, ctx, batch, blocks, thresh, nsample, log):
+    """SVD recompression of every block of the last compress at `thresh`
+    (the reference's matlab_postcompress(C, U, R, 0.1 * err)): device time,
+    ranks and stored entries before and after, accuracy on sampled blocks."""
+    ro, ri, co, cix = blocks
+    nb = len(ro) - 1
+    r = batch.info_arrays()["r"]
+    batch.recompress(thresh)
+    runs = [batch.recompress(thresh) for _ in range(3)]
+    ranks, ms = runs[-1][0], float(np.mean([x[1] for x in runs]))
+    m, n = np.diff(ro), np.diff(co)
+    before = float(np.sum((m + n) * r + r * r))
+    after = float(np.sum((m + n) * ranks + ranks))
+    rng = np.random.default_rng(1)
+    errs = []
+    for b in rng.choice(nb, min(nsample, nb), replace=False):
+        rc = batch.recompressed_block(int(b))
+        Z = ctx.fill(ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]], fc.F64)
+        errs.append(np.linalg.norm((rc["C"] * rc["s"]) @ rc["R"] - Z) / np.linalg.norm(Z))
+    log(f"# recompression at {thresh:g}: {ms:.2f} ms, rank mean {r.mean():.1f} -> {ranks.mean():.1f}, "
+        f"stored entries {before:.3e} -> {after:.3e}, accuracy max {max(errs):.2e}")
+    return {"thresh": thresh, "ms": ms, "rank_mean_before": float(r.mean()), "rank_mean_after": float(ranks.mean()),
+            "stored_entries_before": before, "stored_entries_after": after,
+            "max_block_relerr_sampled": float(max(errs)), "median_block_relerr_sampled": float(np.median(errs))}
+
+
 def rank_model(fc, ctx, log):
     import importlib.util
 
@@ -323,6 +349,8 @@ def main():
     prod = None
     if not args.no_product:
         prod = product_bench(fc, learned, mesh, blocks, np.arange(256) * np.pi / 255, np.zeros(256), log)
+    # SVD recompression of the learned result at 0.1 x tol
+    recomp = recompress_bench(fc, ctx, learned, blocks, 0.1 * opts["tol"], args.check_blocks, log)
 
     # the same without the network: batched reference loop
     batched_res = None
@@ -372,6 +400,7 @@ def main():
         "speedup_vs_batched_plain": value / batched_res["blocks_per_s"] if batched_res else None,
         "profiled_step_ms": prof_wall,
         "product_256": prod,
+        "recompression": recomp,
         "roofline": roofline(kt),
         "kernels": kt,
         "tuning": tuning,

@@ -101,15 +101,19 @@ __device__ void apply_q(const c128* A, int mm, int r, int lda, const double* s_s
     }
 }
 
-template <typename PC>
+// kMaxR: the LDS bound of the launch; jobs with r above it are left to the
+// launch with the larger bound (kLo: the smaller bound below which jobs belong
+// to the other launch). kE: rows per lane cached by the Jacobi (0: streamed).
+template <typename PC, int kMaxR, int kLo, int kE>
 __global__ __launch_bounds__(kRcThreads) void k_recompress(const RecompressJob* __restrict__ jobs, double thresh) {
-    __shared__ c128 s_beta_c[kRcMaxR], s_beta_r[kRcMaxR];
-    __shared__ double s_scale_c[kRcMaxR], s_scale_r[kRcMaxR], s_sig[kRcMaxR];
-    __shared__ int s_ord[kRcMaxR];
+    __shared__ c128 s_beta_c[kMaxR], s_beta_r[kMaxR];
+    __shared__ double s_scale_c[kMaxR], s_scale_r[kMaxR], s_sig[kMaxR];
+    __shared__ int s_ord[kMaxR];
     __shared__ double s_red[kRcWaves];
     __shared__ int s_flag;
     const RecompressJob J = jobs[blockIdx.x];
     const int m = J.m, n = J.n, r = J.r, tid = threadIdx.x, nt = blockDim.x;
+    if (r > kMaxR || r <= kLo) return;  // the other launch's job
     if (r <= 0) {
         if (tid == 0) *J.k_out = 0;
         return;
@@ -161,8 +165,8 @@ __global__ __launch_bounds__(kRcThreads) void k_recompress(const RecompressJob* 
     // with couplings ~1e-10, enough for a pinv but not for singular values of
     // nearly equal pairs; a second call adds the sweep that takes them to
     // rounding (its first sweep meets the stop rule).
-    jacobi_sweeps<16, 0>(S, V, r, r, &s_flag);
-    jacobi_sweeps<16, 0>(S, V, r, r, &s_flag);
+    jacobi_sweeps<16, kE>(S, V, r, r, &s_flag);
+    jacobi_sweeps<16, kE>(S, V, r, r, &s_flag);
     __syncthreads();
     for (int j = tid; j < r; j += nt) {
         double nn = 0;
@@ -220,10 +224,16 @@ size_t recompress_ws_elems(int m, int n, int r) {
 
 void launch_recompress(const RecompressJob* jobs, int njobs, double thresh, int f32, hipStream_t s) {
     if (njobs <= 0) return;
-    if (f32)
-        hipLaunchKernelGGL(k_recompress<c64>, dim3(njobs), dim3(kRcThreads), 0, s, jobs, thresh);
-    else
-        hipLaunchKernelGGL(k_recompress<c128>, dim3(njobs), dim3(kRcThreads), 0, s, jobs, thresh);
+    // r <= 64 (the usual far-field core): small LDS arrays, Jacobi rows in
+    // registers; larger r: the second launch (its workgroups for small-r jobs
+    // return at once)
+    if (f32) {
+        hipLaunchKernelGGL((k_recompress<c64, 64, -1, 4>), dim3(njobs), dim3(kRcThreads), 0, s, jobs, thresh);
+        hipLaunchKernelGGL((k_recompress<c64, kRcMaxR, 64, 0>), dim3(njobs), dim3(kRcThreads), 0, s, jobs, thresh);
+    } else {
+        hipLaunchKernelGGL((k_recompress<c128, 64, -1, 4>), dim3(njobs), dim3(kRcThreads), 0, s, jobs, thresh);
+        hipLaunchKernelGGL((k_recompress<c128, kRcMaxR, 64, 0>), dim3(njobs), dim3(kRcThreads), 0, s, jobs, thresh);
+    }
 }
 
 }  // namespace farcur
