@@ -1608,6 +1608,11 @@ struct farcur_batch {
     std::vector<int32_t> rc_k;
     std::vector<size_t> rc_off;
     bool rc_ready = false;
+    // ACA of every block (k_aca): per-block regions of d_aca at aca_off
+    DBuf d_aca, d_aca_jobs;
+    std::vector<size_t> aca_off;
+    std::vector<int32_t> aca_kmax, aca_k, aca_status;
+    bool aca_ready = false;
 };
 
 namespace {
@@ -2069,7 +2074,7 @@ extern "C" void farcur_batch_destroy(farcur_batch* B) {
     hipStreamSynchronize(B->ctx->stream);
     DBuf* bufs[] = {&B->d_rows, &B->d_cols, &B->d_rperm, &B->d_cperm, &B->d_v,    &B->d_ya,  &B->d_yb,
                     &B->d_vo,   &B->d_plan, &B->d_feat, &B->d_cap, &B->d_rpred, &B->d_rc,  &B->d_rc_ws,
-                    &B->d_rc_jobs};
+                    &B->d_rc_jobs, &B->d_aca, &B->d_aca_jobs};
     for (DBuf* b : bufs) b->release();
     for (RoundGraph& g : B->graphs) g.release();
     B->arena.release();
@@ -2186,6 +2191,117 @@ extern "C" int farcur_batch_recompressed_block(farcur_batch* B, int b, farcur_c1
         if (sigma) HIPTRY(hipMemcpyAsync(sigma, o + L.sigma, (size_t)kk * 8, hipMemcpyDeviceToHost, st));
     }
     HIPTRY(hipStreamSynchronize(st));
+    return FARCUR_OK;
+}
+
+// Byte offsets of one block's ACA region: U (m x kmax), Vt (n x kmax), the
+// residual row (n), used flags (m + n), pivots I, J (kmax), k, status.
+struct AcaLayout {
+    size_t U, Vt, row, used, I, J, k, status, size;
+    AcaLayout(int m, int n, int kmax) {
+        auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        U = 0;
+        Vt = U + al((size_t)m * kmax * 16);
+        row = Vt + al((size_t)n * kmax * 16);
+        used = row + al((size_t)n * 16);
+        I = used + al((size_t)(m + n) * 4);
+        J = I + al((size_t)kmax * 4);
+        k = J + al((size_t)kmax * 4);
+        status = k + 256;
+        size = status + 256;
+    }
+};
+
+// Partially pivoted ACA of every block of the batch (ACA.m:40-128) with the
+// FP64 element: U (m x k), V (k x n) with Z ~ U V.
+extern "C" int farcur_batch_aca(farcur_batch* B, double tol, int max_rank, int* ranks, double* ms) {
+    if (!B) return fail(FARCUR_EINVAL, "NULL batch");
+    if (!(tol >= 0) || !std::isfinite(tol)) return fail(FARCUR_EINVAL, "tol must be finite and >= 0");
+    if (max_rank < 0) return fail(FARCUR_EINVAL, "max_rank must be >= 0");
+    farcur_ctx* ctx = B->ctx;
+    if (ctx->ne == 0) return fail(FARCUR_ESTATE, "no geometry");
+    TRY(ctx->select());
+    B->aca_ready = false;
+    B->aca_off.assign(B->nb + 1, 0);
+    B->aca_kmax.assign(B->nb, 0);
+    for (int b = 0; b < B->nb; ++b) {
+        const int m = B->row_off[b + 1] - B->row_off[b], n = B->col_off[b + 1] - B->col_off[b];
+        const int D = std::min(m, n);
+        B->aca_kmax[b] = std::max(1, std::min(D, max_rank > 0 ? max_rank : 128));
+        B->aca_off[b + 1] = B->aca_off[b] + AcaLayout(m, n, B->aca_kmax[b]).size;
+    }
+    TRY(B->d_aca.ensure(B->aca_off[B->nb]));
+    std::vector<AcaJob> jobs(B->nb);
+    for (int b = 0; b < B->nb; ++b) {
+        const int m = B->row_off[b + 1] - B->row_off[b], n = B->col_off[b + 1] - B->col_off[b];
+        const AcaLayout L(m, n, B->aca_kmax[b]);
+        char* o = (char*)B->d_aca.p + B->aca_off[b];
+        AcaJob& J = jobs[b];
+        J.m = m;
+        J.n = n;
+        J.kmax = B->aca_kmax[b];
+        J.pad = 0;
+        J.rows = B->d_rows.as<int32_t>() + B->row_off[b];
+        J.cols = B->d_cols.as<int32_t>() + B->col_off[b];
+        J.U = (c128*)(o + L.U);
+        J.Vt = (c128*)(o + L.Vt);
+        J.row = (c128*)(o + L.row);
+        J.used = (int32_t*)(o + L.used);
+        J.I = (int32_t*)(o + L.I);
+        J.J = (int32_t*)(o + L.J);
+        J.k_out = (int32_t*)(o + L.k);
+        J.status = (int32_t*)(o + L.status);
+    }
+    TRY(B->d_aca_jobs.ensure(jobs.size() * sizeof(AcaJob)));
+    hipStream_t st = ctx->stream;
+    HIPTRY(hipMemcpyAsync(B->d_aca_jobs.p, jobs.data(), jobs.size() * sizeof(AcaJob), hipMemcpyHostToDevice, st));
+    HIPTRY(hipEventRecord(ctx->ev0, st));
+    launch_aca(B->d_aca_jobs.as<AcaJob>(), B->nb, tol, ctx->fill_params(0), st);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipEventRecord(ctx->ev1, st));
+    B->aca_k.assign(B->nb, 0);
+    B->aca_status.assign(B->nb, 0);
+    for (int b = 0; b < B->nb; ++b) {
+        HIPTRY(hipMemcpyAsync(&B->aca_k[b], jobs[b].k_out, 4, hipMemcpyDeviceToHost, st));
+        HIPTRY(hipMemcpyAsync(&B->aca_status[b], jobs[b].status, 4, hipMemcpyDeviceToHost, st));
+    }
+    HIPTRY(hipStreamSynchronize(st));
+    float t = 0;
+    HIPTRY(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    if (ms) *ms = t;
+    if (ranks) std::copy(B->aca_k.begin(), B->aca_k.end(), ranks);
+    B->aca_ready = true;
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_batch_aca_block(farcur_batch* B, int b, farcur_c128* U, farcur_c128* V, int* row_pivots,
+                                      int* col_pivots, int* k, int* status) {
+    if (!B) return fail(FARCUR_EINVAL, "NULL batch");
+    if (!B->aca_ready) return fail(FARCUR_ESTATE, "no ACA result");
+    if (b < 0 || b >= B->nb) return fail(FARCUR_EINVAL, "block %d out of range", b);
+    farcur_ctx* ctx = B->ctx;
+    TRY(ctx->select());
+    const int m = B->row_off[b + 1] - B->row_off[b], n = B->col_off[b + 1] - B->col_off[b];
+    const int kk = B->aca_k[b];
+    if (k) *k = kk;
+    if (status) *status = B->aca_status[b];
+    const AcaLayout L(m, n, B->aca_kmax[b]);
+    const char* o = (const char*)B->d_aca.p + B->aca_off[b];
+    hipStream_t st = ctx->stream;
+    if (kk > 0) {
+        if (U) HIPTRY(hipMemcpyAsync(U, o + L.U, (size_t)m * kk * 16, hipMemcpyDeviceToHost, st));
+        std::vector<c128> vt;
+        if (V) {
+            vt.resize((size_t)n * kk);
+            HIPTRY(hipMemcpyAsync(vt.data(), o + L.Vt, vt.size() * 16, hipMemcpyDeviceToHost, st));
+        }
+        if (row_pivots) HIPTRY(hipMemcpyAsync(row_pivots, o + L.I, (size_t)kk * 4, hipMemcpyDeviceToHost, st));
+        if (col_pivots) HIPTRY(hipMemcpyAsync(col_pivots, o + L.J, (size_t)kk * 4, hipMemcpyDeviceToHost, st));
+        HIPTRY(hipStreamSynchronize(st));
+        if (V)  // V (k x n, ld k) from its rows stored as columns of Vt
+            for (int l = 0; l < kk; ++l)
+                for (int j = 0; j < n; ++j) V[l + (size_t)j * kk] = farcur_c128{vt[j + (size_t)l * n].re, vt[j + (size_t)l * n].im};
+    }
     return FARCUR_OK;
 }
 

@@ -624,6 +624,205 @@ void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, int 
         hipLaunchKernelGGL(k_residual<c128>, dim3(grid), dim3(threads), 0, s, jobs, njobs, skip_heldout, out);
 }
 
+// ---------------------------------------------------------------- ACA
+// Partially pivoted adaptive cross approximation of one block per workgroup
+// (ACA.m:40-128, the reference's comparator), with the FP64 element filling
+// one row or one column of the block per step.
+
+// Workgroup argmax of v over (value, index) pairs: largest value, smallest
+// index among ties (ACA.m's find(abs(R) == max(abs(R))) then the first).
+__device__ void block_argmax(double v, int i, double* s_v, int* s_i, double& bv, int& bi) {
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(v, o);
+        const int oi = __shfl_xor(i, o);
+        if (ov > v || (ov == v && oi < i)) {
+            v = ov;
+            i = oi;
+        }
+    }
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) {
+        s_v[w] = v;
+        s_i[w] = i;
+    }
+    __syncthreads();
+    bv = s_v[0];
+    bi = s_i[0];
+    for (int k = 1; k < (int)blockDim.x / kWave; ++k)
+        if (s_v[k] > bv || (s_v[k] == bv && s_i[k] < bi)) {
+            bv = s_v[k];
+            bi = s_i[k];
+        }
+}
+
+__global__ __launch_bounds__(kFillThreads) void k_aca(const AcaJob* __restrict__ jobs, double tol,
+                                                      const double* __restrict__ rec, double k, double eta) {
+    __shared__ FillTileLds lds;
+    __shared__ double s_v[kFillThreads / kWave], s_red[kFillThreads / kWave];
+    __shared__ int s_i[kFillThreads / kWave];
+    const AcaJob J = jobs[blockIdx.x];
+    const int M = J.m, N = J.n, tid = threadIdx.x, nt = blockDim.x;
+    const int lane = tid & (kWave - 1), w = tid / kWave, nw = nt / kWave;
+    const TriGeom tg{};
+    int32_t* used_r = J.used;
+    int32_t* used_c = J.used + M;
+    for (int i = tid; i < M + N; i += nt) J.used[i] = 0;
+    __syncthreads();
+    auto fill_row = [&](int i, c128* out) {  // out[j] = Z[i, j]
+        for (int c0 = 0; c0 < N; c0 += kTileCols)
+            fill_any<double, c128, false>(out, 1, J.rows + i, 1, J.cols, N, 0, c0, nullptr, rec, nullptr, tg, k, eta,
+                                          lds, kNoTail);
+    };
+    auto fill_col = [&](int j, c128* out) {  // out[i] = Z[i, j]
+        for (int r0 = 0; r0 < M; r0 += kTileRows)
+            fill_any<double, c128, true>(out, M, J.rows, M, J.cols + j, 1, r0, 0, nullptr, rec, nullptr, tg, k, eta,
+                                         lds, kNoTail);
+    };
+    if (M == 1 || N == 1) {  // a vector: nothing to compress (ACA.m:47)
+        if (N == 1) {
+            fill_col(0, J.U);
+            if (tid == 0) J.Vt[0] = c128{1.0, 0.0};
+        } else {
+            fill_row(0, J.Vt);
+            if (tid == 0) J.U[0] = c128{1.0, 0.0};
+        }
+        if (tid == 0) {
+            J.I[0] = 0;
+            J.J[0] = 0;
+            *J.k_out = 1;
+            *J.status = 1;
+        }
+        return;
+    }
+    const int kend = min(min(M, N), J.kmax);
+    int Ik = 0, kk = 0, status = 0;
+    if (tid == 0) used_r[0] = 1;
+    double normZ = 0;
+    while (true) {
+        // residual row Ik
+        fill_row(Ik, J.row);
+        __syncthreads();
+        for (int j = tid; j < N; j += nt) {
+            c128 x = J.row[j];
+            for (int l = 0; l < kk; ++l) {
+                const c128 u = J.U[Ik + (size_t)l * M], v = J.Vt[j + (size_t)l * N];
+                x.re -= u.re * v.re - u.im * v.im;
+                x.im -= u.re * v.im + u.im * v.re;
+            }
+            J.row[j] = x;
+        }
+        __syncthreads();
+        double bv = -1.0;
+        int Jk = 0x7fffffff;
+        for (int j = tid; j < N; j += nt)
+            if (!used_c[j]) {
+                const double a = J.row[j].re * J.row[j].re + J.row[j].im * J.row[j].im;
+                if (a > bv) {
+                    bv = a;
+                    Jk = j;
+                }
+            }
+        block_argmax(bv, Jk, s_v, s_i, bv, Jk);
+        const c128 piv = J.row[Jk];
+        if (piv.re == 0.0 && piv.im == 0.0) {  // R(Ik, Jk) == 0 (ACA.m:91)
+            status = 1;
+            break;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            used_c[Jk] = 1;
+            J.I[kk] = Ik;
+            J.J[kk] = Jk;
+        }
+        const double ip = 1.0 / (piv.re * piv.re + piv.im * piv.im);
+        for (int j = tid; j < N; j += nt) {  // V_k = R_row / R(Ik, Jk)
+            const c128 x = J.row[j];
+            J.Vt[j + (size_t)kk * N] = c128{(x.re * piv.re + x.im * piv.im) * ip, (x.im * piv.re - x.re * piv.im) * ip};
+        }
+        // residual column Jk into U_k
+        c128* Uk = J.U + (size_t)kk * M;
+        fill_col(Jk, Uk);
+        __syncthreads();
+        for (int i = tid; i < M; i += nt) {
+            c128 x = Uk[i];
+            for (int l = 0; l < kk; ++l) {
+                const c128 u = J.U[i + (size_t)l * M], v = J.Vt[Jk + (size_t)l * N];
+                x.re -= u.re * v.re - u.im * v.im;
+                x.im -= u.re * v.im + u.im * v.re;
+            }
+            Uk[i] = x;
+        }
+        __syncthreads();
+        // |U_k|^2 |V_k|^2 and the cross terms sum_l |(U_l^H U_k)(V_k V_l^H)|
+        double nu = 0, nv = 0;
+        for (int i = tid; i < M; i += nt) nu += Uk[i].re * Uk[i].re + Uk[i].im * Uk[i].im;
+        const c128* Vk = J.Vt + (size_t)kk * N;
+        for (int j = tid; j < N; j += nt) nv += Vk[j].re * Vk[j].re + Vk[j].im * Vk[j].im;
+        nu = block_sum(nu, s_red);
+        nv = block_sum(nv, s_red);
+        double cross = 0;
+        for (int l = w; l < kk; l += nw) {
+            const c128* Ul = J.U + (size_t)l * M;
+            const c128* Vl = J.Vt + (size_t)l * N;
+            double ar = 0, ai = 0, br = 0, bi = 0;
+            for (int i = lane; i < M; i += kWave) {  // U_l^H U_k
+                ar += Ul[i].re * Uk[i].re + Ul[i].im * Uk[i].im;
+                ai += Ul[i].re * Uk[i].im - Ul[i].im * Uk[i].re;
+            }
+            for (int j = lane; j < N; j += kWave) {  // V_k V_l^H
+                br += Vk[j].re * Vl[j].re + Vk[j].im * Vl[j].im;
+                bi += Vk[j].im * Vl[j].re - Vk[j].re * Vl[j].im;
+            }
+            ar = team_sum(ar, kWave);
+            ai = team_sum(ai, kWave);
+            br = team_sum(br, kWave);
+            bi = team_sum(bi, kWave);
+            if (lane == 0) cross += sqrt(ar * ar + ai * ai) * sqrt(br * br + bi * bi);
+        }
+        cross = block_sum(cross, s_red);
+        normZ += 2.0 * cross + nu * nv;
+        ++kk;
+        if (kk > 1 && sqrt(nu * nv) <= tol * sqrt(normZ)) {  // ACA.m:114
+            status = 1;
+            break;
+        }
+        if (kk == min(M, N)) {  // ACA.m:118
+            status = 1;
+            break;
+        }
+        if (kk == kend) {  // storage bound reached before convergence
+            status = 2;
+            break;
+        }
+        // next row: largest |U_k| among the rows not yet taken (ACA.m:123)
+        bv = -1.0;
+        int In = 0x7fffffff;
+        for (int i = tid; i < M; i += nt)
+            if (!used_r[i]) {
+                const double a = Uk[i].re * Uk[i].re + Uk[i].im * Uk[i].im;
+                if (a > bv) {
+                    bv = a;
+                    In = i;
+                }
+            }
+        block_argmax(bv, In, s_v, s_i, bv, In);
+        __syncthreads();
+        Ik = In;
+        if (tid == 0) used_r[Ik] = 1;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        *J.k_out = kk;
+        *J.status = status;
+    }
+}
+
+void launch_aca(const AcaJob* jobs, int njobs, double tol, const FillParams& fp, hipStream_t s) {
+    if (njobs <= 0) return;
+    hipLaunchKernelGGL(k_aca, dim3(njobs), dim3(kFillThreads), 0, s, jobs, tol, (const double*)fp.rec, fp.k, fp.eta);
+}
+
 // ---------------------------------------------------------------- test vectors
 __global__ void k_vtest(c128* __restrict__ v, const int32_t* __restrict__ offsets, int nblocks, uint64_t seed) {
     for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {

@@ -54,6 +54,21 @@ struct JobOut {
     int32_t pad;
 };
 
+// One block of an ACA (k_aca): U (m x kmax, ld m), V rows as the columns of
+// Vt (n x kmax, ld n), `row` (n) and `used` (m + n) scratch, pivots I, J (kmax).
+struct AcaJob {
+    int32_t m, n, kmax, pad;
+    const int32_t* rows;
+    const int32_t* cols;
+    c128* U;
+    c128* Vt;
+    c128* row;
+    int32_t* used;
+    int32_t* I;
+    int32_t* J;
+    int32_t* k_out;
+    int32_t* status;  // 1 converged (or exhausted min(m, n)), 2 stopped at kmax
+};
 // One block of an SVD recompression (recompress.hip).
 struct RecompressJob {
     int32_t m, n, r, pad;
@@ -80,6 +95,8 @@ struct FillParams {
     int32_t f32;      // 1: FP32 fill, panels c64; 0: FP64 fill, panels c128
     double k, eta;
 };
+
+void launch_aca(const AcaJob* jobs, int njobs, double tol, const FillParams& fp, hipStream_t s);
 
 // Tiles of the fill for a job of size (m, n, r, h held-out rows) restricted to `panels`.
 int fill_tiles(int m, int n, int r, int h, int panels);

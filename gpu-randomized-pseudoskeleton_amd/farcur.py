@@ -167,6 +167,8 @@ def _setup(L: ctypes.CDLL) -> None:
         L.farcur_batch_block.argtypes = [vp, ci, vp, vp, vp, vp, vp]
         L.farcur_batch_recompress.argtypes = [vp, cd, vp, vp]
         L.farcur_batch_recompressed_block.argtypes = [vp, ci, vp, vp, vp, vp]
+        L.farcur_batch_aca.argtypes = [vp, cd, ci, vp, vp]
+        L.farcur_batch_aca_block.argtypes = [vp, ci, vp, vp, vp, vp, vp, vp]
         L.farcur_batch_profile.argtypes = [vp, ci]
         L.farcur_batch_kernel_times.argtypes = [vp, vp, vp, vp, vp, ci]
         L.farcur_batch_features.argtypes = [vp, vp]
@@ -537,6 +539,28 @@ class Batch:
         if k:
             check(lib().farcur_batch_recompressed_block(self._h, b, _p(C), _p(s), _p(R), None))
         return {"C": C, "s": s, "R": R}
+
+    def aca(self, tol, max_rank=0):
+        """Partially pivoted ACA of every block (ACA.m:40-128): (k per block, device ms)."""
+        ranks = np.empty(self.nblocks, dtype=np.int32)
+        ms = cd()
+        check(lib().farcur_batch_aca(self._h, float(tol), int(max_rank), _p(ranks), ctypes.byref(ms)))
+        return ranks, ms.value
+
+    def aca_block(self, b):
+        """Block b of the last ACA: U (m x k), V (k x n), pivots I, J, status."""
+        m = int(self.row_off[b + 1] - self.row_off[b])
+        n = int(self.col_off[b + 1] - self.col_off[b])
+        k, st = ci(), ci()
+        check(lib().farcur_batch_aca_block(self._h, b, None, None, None, None, ctypes.byref(k), ctypes.byref(st)))
+        k = k.value
+        U = np.empty((m, k), dtype=np.complex128, order="F")
+        V = np.empty((k, n), dtype=np.complex128, order="F")
+        I = np.empty(k, dtype=np.int32)
+        J = np.empty(k, dtype=np.int32)
+        if k:
+            check(lib().farcur_batch_aca_block(self._h, b, _p(U), _p(V), _p(I), _p(J), None, None))
+        return {"U": U, "V": V, "I": I, "J": J, "status": st.value}
 
     def features(self) -> np.ndarray:
         X = np.empty((self.nblocks, 8), dtype=np.float32)

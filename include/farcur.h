@@ -260,6 +260,22 @@ int farcur_batch_recompress(farcur_batch* batch, double thresh, int* ranks, doub
 int farcur_batch_recompressed_block(farcur_batch* batch, int b, farcur_c128* C, double* sigma, farcur_c128* R,
                                     int* k);
 
+/* Partially pivoted adaptive cross approximation of every block of the batch
+ * with the FP64 element (ACA.m:40-128, the reference's comparator; its C twin
+ * c_ACA.c:327-466): first row 0; each step takes the residual row, its largest
+ * unused column (the first among ties), V_k = row / pivot, U_k = the residual
+ * column; normZ += 2 sum_l |(U_l^H U_k)(V_k V_l^H)| + |U_k|^2 |V_k|^2; stops
+ * after step k >= 2 when |U_k| |V_k| <= tol sqrt(normZ), on a zero pivot, at
+ * min(m, n) or at max_rank crosses (0: 128); the next row is the largest unused
+ * entry of U_k. Blocks with one row or column are returned as they are (U V =
+ * the block, k = 1). ranks (nblocks, may be NULL) receives k, *ms the device time. */
+int farcur_batch_aca(farcur_batch* batch, double tol, int max_rank, int* ranks, double* ms);
+/* Block b of the last ACA: U (m x k), V (k x n), the row and column pivots
+ * (block positions, k each), k and status (1 converged, 2 stopped at max_rank);
+ * any pointer may be NULL. */
+int farcur_batch_aca_block(farcur_batch* batch, int b, farcur_c128* U, farcur_c128* V, int* row_pivots,
+                           int* col_pivots, int* k, int* status);
+
 /* Eight features per block (nblocks x 8, row-major) for the rank model:
  * diameter of the row group / lambda, diameter of the column group / lambda,
  * centre distance / lambda, cosines between each group's mean normal and the

@@ -274,3 +274,69 @@ def svd_compress(C, U, R, thresh):
     else:
         k = int(np.sum(s / s.max() > thresh))
     return qc @ u[:, :k], s[:k], (qr_ @ vh.conj().T[:, :k]).conj().T
+
+
+def aca(geom: Geometry, rows, cols, tol, max_rank=128):
+    """Partially pivoted ACA of Z[rows, cols] (codes/ACA_vs_CUR/seed/ACA.m:40-128),
+    rows and columns of the residual filled with the FP64 element. Returns
+    (U m x k, V k x n, I, J, status); status 2 when max_rank crosses were taken
+    without convergence. Blocks with one row or column are returned whole
+    (ACA.m:47). Pins: exact recovery of a block of rank k in k steps, the
+    interpolation property (U V matches Z on every pivot row and column), the
+    normZ recursion equals |U V|_F^2 in exact arithmetic bounds (tests/test_oracle.py)."""
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    return aca_dense(lambda i: geom.fill(rows[i:i + 1], cols)[0],
+                     lambda j: geom.fill(rows, cols[j:j + 1])[:, 0], rows.size, cols.size, tol, max_rank)
+
+
+def aca_dense(row, col, M, N, tol, max_rank=128):
+    """ACA.m:40-128 over callables row(i) -> Z[i, :] and col(j) -> Z[:, j]."""
+    if M == 1 or N == 1:
+        if N == 1:
+            return col(0)[:, None].copy(), np.ones((1, 1), complex), np.zeros(1, int), np.zeros(1, int), 1
+        return np.ones((1, 1), complex), row(0)[None, :].copy(), np.zeros(1, int), np.zeros(1, int), 1
+    kend = min(M, N, max_rank)
+    used_r = np.zeros(M, bool)
+    used_c = np.zeros(N, bool)
+    Us, Vs, I, J = [], [], [], []
+    Ik, normZ, status = 0, 0.0, 0
+    used_r[0] = True
+    while True:
+        R = row(Ik).astype(complex)
+        for u, v in zip(Us, Vs):
+            R -= u[Ik] * v
+        a = np.where(used_c, -1.0, np.abs(R) ** 2)
+        Jk = int(np.argmax(a))  # first index among equal maxima
+        piv = R[Jk]
+        if piv == 0:
+            status = 1
+            break
+        used_c[Jk] = True
+        I.append(Ik)
+        J.append(Jk)
+        Vk = R / piv
+        Uk = col(Jk).astype(complex)
+        for u, v in zip(Us, Vs):
+            Uk -= u * v[Jk]
+        nu, nv = np.vdot(Uk, Uk).real, np.vdot(Vk, Vk).real
+        cross = sum(abs(np.vdot(u, Uk)) * abs(np.dot(Vk, v.conj())) for u, v in zip(Us, Vs))
+        normZ += 2 * cross + nu * nv
+        Us.append(Uk)
+        Vs.append(Vk)
+        k = len(Us)
+        if k > 1 and np.sqrt(nu * nv) <= tol * np.sqrt(normZ):
+            status = 1
+            break
+        if k == min(M, N):
+            status = 1
+            break
+        if k == kend:
+            status = 2
+            break
+        a = np.where(used_r, -1.0, np.abs(Uk) ** 2)
+        Ik = int(np.argmax(a))
+        used_r[Ik] = True
+    U = np.array(Us).T if Us else np.zeros((M, 0), complex)
+    V = np.array(Vs) if Vs else np.zeros((0, N), complex)
+    return U, V, np.array(I), np.array(J), status

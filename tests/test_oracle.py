@@ -370,3 +370,38 @@ def test_svd_compress_planted_rank(oracle):
     Cp, s, Rp = oracle.svd_compress(Cb, Ub, Rb, 1e-8)
     assert s.size == 5
     assert np.linalg.norm(Cp @ np.diag(s) @ Rp - A) <= 1e-10 * np.linalg.norm(A)
+
+
+def _smooth_block(rng, M, N):
+    # a numerically low-rank kernel block between two separated point clouds
+    x = rng.standard_normal((M, 3))
+    y = rng.standard_normal((N, 3)) + np.array([8.0, 0, 0])
+    d = np.linalg.norm(x[:, None, :] - y[None, :, :], axis=2)
+    return np.exp(-1j * 2.0 * d) / d
+
+
+def test_aca_pins(oracle):
+    # ACA.m:40-128 pinned by properties the method fixes: the cross
+    # approximation reproduces every pivot row and column, the first column
+    # pivot is the largest entry of row 0, the error falls with tol, and a
+    # block of exact rank 5 is recovered in 5 steps.
+    rng = np.random.default_rng(5)
+    Z = _smooth_block(rng, 70, 50)
+    row, col = (lambda i: Z[i].copy()), (lambda j: Z[:, j].copy())
+    U, V, I, J, st = oracle.aca_dense(row, col, 70, 50, 1e-6)
+    A = U @ V
+    assert st == 1 and I[0] == 0 and J[0] == int(np.argmax(np.abs(Z[0])))
+    np.testing.assert_allclose(A[I, :], Z[I, :], atol=1e-10 * np.abs(Z).max())
+    np.testing.assert_allclose(A[:, J], Z[:, J], atol=1e-10 * np.abs(Z).max())
+    e6 = np.linalg.norm(A - Z) / np.linalg.norm(Z)
+    U3, V3, *_ = oracle.aca_dense(row, col, 70, 50, 1e-3)
+    e3 = np.linalg.norm(U3 @ V3 - Z) / np.linalg.norm(Z)
+    assert e6 < 1e-5 and e3 > e6 and U3.shape[1] < U.shape[1]
+    P = (rng.standard_normal((40, 5)) + 1j * rng.standard_normal((40, 5))) @ (
+        rng.standard_normal((5, 30)) + 1j * rng.standard_normal((5, 30)))
+    U5, V5, I5, J5, _ = oracle.aca_dense(lambda i: P[i].copy(), lambda j: P[:, j].copy(), 40, 30, 1e-12)
+    assert U5.shape[1] in (5, 6)
+    assert np.linalg.norm(U5 @ V5 - P) <= 1e-10 * np.linalg.norm(P)
+    # vectors are returned whole (ACA.m:47)
+    Uv, Vv, *_ = oracle.aca_dense(row, col, 1, 50, 1e-6)
+    np.testing.assert_array_equal((Uv @ Vv)[0], Z[0])
