@@ -149,6 +149,27 @@ def recompress_bench(fc, ctx, batch, blocks, thresh, nsample, log):
             "max_block_relerr_sampled": float(max(errs)), "median_block_relerr_sampled": float(np.median(errs))}
 
 
+def aca_bench(fc, ctx, batch, blocks, tol, max_rank, nsample, log):
+    """The reference's comparator: partially pivoted ACA of every block with
+    the FP64 element at `tol`; device time, ranks and sampled accuracy."""
+    ro, ri, co, cix = blocks
+    nb = len(ro) - 1
+    batch.aca(tol, max_rank)
+    runs = [batch.aca(tol, max_rank) for _ in range(2)]
+    ranks, ms = runs[-1][0], float(np.mean([x[1] for x in runs]))
+    rng = np.random.default_rng(2)
+    errs = []
+    for b in rng.choice(nb, min(nsample, nb), replace=False):
+        out = batch.aca_block(int(b))
+        Z = ctx.fill(ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]], fc.F64)
+        errs.append(np.linalg.norm(out["U"] @ out["V"] - Z) / np.linalg.norm(Z))
+    log(f"# ACA comparator at {tol:g}: {ms:.2f} ms ({nb / (ms * 1e-3):.0f} blocks/s), rank mean {ranks.mean():.1f}, "
+        f"accuracy max {max(errs):.2e} median {np.median(errs):.2e}")
+    return {"tol": tol, "max_rank": max_rank, "ms": ms, "blocks_per_s": nb / (ms * 1e-3),
+            "rank_mean": float(ranks.mean()), "max_block_relerr_sampled": float(max(errs)),
+            "median_block_relerr_sampled": float(np.median(errs))}
+
+
 def rank_model(fc, ctx, log):
     import importlib.util
 
@@ -351,6 +372,8 @@ def main():
         prod = product_bench(fc, learned, mesh, blocks, np.arange(256) * np.pi / 255, np.zeros(256), log)
     # SVD recompression of the learned result at 0.1 x tol
     recomp = recompress_bench(fc, ctx, learned, blocks, 0.1 * opts["tol"], args.check_blocks, log)
+    # the reference's comparator on the same blocks: GPU ACA (FP64 element)
+    aca_res = aca_bench(fc, ctx, learned, blocks, opts["tol"], 64, args.check_blocks, log)
 
     # the same without the network: batched reference loop
     batched_res = None
@@ -401,6 +424,7 @@ def main():
         "profiled_step_ms": prof_wall,
         "product_256": prod,
         "recompression": recomp,
+        "aca_comparator": aca_res,
         "roofline": roofline(kt),
         "kernels": kt,
         "tuning": tuning,
