@@ -309,7 +309,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(far-field blocks shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+              "(far-field blocks are independent: a multi-GPU run would split the block list by predicted cost)", file=sys.stderr)
         sys.exit(2)
     log = lambda s: print(s, flush=True)
 
