@@ -125,8 +125,15 @@ __device__ __forceinline__ void rotate_pair(const Rot& R, c128& a, c128& b) {
 // layout) unless V is NULL. All threads of the workgroup must call; teams of kT lanes; with
 // kE > 0 every r <= kE * kT (elements cached in registers), with kE = 0 any r
 // (elements streamed). s_flag: one int of LDS. Returns the number of sweeps.
-template <int kT, int kE>
-__device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int ld, int* s_flag) {
+// kWeak: pairs whose columns both have squared norms below weak2 are not
+// rotated nor counted for convergence. Columns at rounding level (the null
+// space of a rank-deficient A) otherwise keep random couplings that no
+// rotation reduces, and the sweeps run to the limit. With weak2 = tau^2 / r
+// the columns kept by a threshold tau are unchanged: a rotation leaves the
+// smaller of its two norms no larger than either input, so the skipped
+// columns hold together less than tau^2.
+template <int kT, int kE, bool kWeak = false>
+__device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int ld, int* s_flag, double weak2 = 0.0) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = r + (r & 1);
     const int npairs = n / 2;
@@ -177,6 +184,8 @@ __device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int ld, in
                     be = team_sum_t<kT>(be);
                     gr = team_sum_t<kT>(gr);
                     gi = team_sum_t<kT>(gi);
+                    if constexpr (kWeak)
+                        if (al < weak2 && be < weak2) continue;
                     const double g2 = gr * gr + gi * gi;
                     if (!(g2 > tol2 * (al * be))) continue;
                     const Rot R = rotation(al, be, gr, gi, g2);
@@ -208,6 +217,8 @@ __device__ __forceinline__ int jacobi_sweeps(c128* A, c128* V, int r, int ld, in
                     be = team_sum_t<kT>(be);
                     gr = team_sum_t<kT>(gr);
                     gi = team_sum_t<kT>(gi);
+                    if constexpr (kWeak)
+                        if (al < weak2 && be < weak2) continue;
                     const double g2 = gr * gr + gi * gi;
                     if (!(g2 > tol2 * (al * be))) continue;
                     const Rot R = rotation(al, be, gr, gi, g2);

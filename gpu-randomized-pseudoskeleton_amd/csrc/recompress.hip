@@ -161,12 +161,27 @@ __global__ __launch_bounds__(kRcThreads) void k_recompress(const RecompressJob* 
         V[idx] = c128{i == c ? 1.0 : 0.0, 0.0};
     }
     __syncthreads();
-    // S V = X with orthogonal columns: S = X V^H, s_j = |x_j|. The sweeps stop
-    // with couplings ~1e-10, enough for a pinv but not for singular values of
-    // nearly equal pairs; a second call adds the sweep that takes them to
-    // rounding (its first sweep meets the stop rule).
-    jacobi_sweeps<16, kE>(S, V, r, r, &s_flag);
-    jacobi_sweeps<16, kE>(S, V, r, r, &s_flag);
+    // S V = X with orthogonal columns: S = X V^H, s_j = |x_j|. S has the rank
+    // of C U R: pairs of columns below (tau / sqrt r)^2, tau = thresh x the
+    // largest column norm (<= s_1), are not rotated; they are dropped by the
+    // threshold whatever rotations among them do, and at rounding level they
+    // would keep the sweeps going to the limit. thresh = 0 keeps every nonzero
+    // value, so there every pair is rotated. The sweeps stop with couplings ~1e-10, enough for a pinv but not
+    // for singular values of nearly equal pairs; a second call adds the sweep
+    // that takes them to rounding (its first sweep meets the stop rule).
+    double cmax2 = 0;
+    for (int j = tid; j < r; j += nt) {
+        double nn = 0;
+        for (int i = 0; i < r; ++i) nn += abs2(S[i + (size_t)j * r]);
+        cmax2 = fmax(cmax2, nn);
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) cmax2 = fmax(cmax2, __shfl_xor(cmax2, o));
+    if ((tid & (kWave - 1)) == 0) s_red[tid / kWave] = cmax2;
+    __syncthreads();
+    for (int w = 0; w < kRcWaves; ++w) cmax2 = fmax(cmax2, s_red[w]);
+    const double weak2 = thresh * thresh * cmax2 / r;
+    jacobi_sweeps<16, kE, true>(S, V, r, r, &s_flag, weak2);
+    jacobi_sweeps<16, kE, true>(S, V, r, r, &s_flag, weak2);
     __syncthreads();
     for (int j = tid; j < r; j += nt) {
         double nn = 0;
