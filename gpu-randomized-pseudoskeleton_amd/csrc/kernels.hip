@@ -658,26 +658,36 @@ __device__ void block_argmax(double v, int i, double* s_v, int* s_i, double& bv,
 
 __global__ __launch_bounds__(kFillThreads) void k_aca(const AcaJob* __restrict__ jobs, double tol,
                                                       const double* __restrict__ rec, double k, double eta) {
-    __shared__ FillTileLds lds;
     __shared__ double s_v[kFillThreads / kWave], s_red[kFillThreads / kWave];
     __shared__ int s_i[kFillThreads / kWave];
     const AcaJob J = jobs[blockIdx.x];
     const int M = J.m, N = J.n, tid = threadIdx.x, nt = blockDim.x;
     const int lane = tid & (kWave - 1), w = tid / kWave, nw = nt / kWave;
-    const TriGeom tg{};
+    const FillConsts fc = fill_consts_f64(k, eta);
     int32_t* used_r = J.used;
     int32_t* used_c = J.used + M;
     for (int i = tid; i < M + N; i += nt) J.used[i] = 0;
     __syncthreads();
+    // one entry per thread, the row RWG as the test function (as the tiles'
+    // lane records): a vector keeps every thread busy without LDS staging
     auto fill_row = [&](int i, c128* out) {  // out[j] = Z[i, j]
-        for (int c0 = 0; c0 < N; c0 += kTileCols)
-            fill_any<double, c128, false>(out, 1, J.rows + i, 1, J.cols, N, 0, c0, nullptr, rec, nullptr, tg, k, eta,
-                                          lds, kNoTail);
+        RwgRec<double> a;
+        load_rec(rec, J.rows[i], a);
+        for (int j = tid; j < N; j += nt) {
+            double zr, zi;
+            entry_f64(a, rec + (size_t)J.cols[j] * kRecStride, fc.k, fc.c4k2, fc.scale, zr, zi);
+            out[j] = c128{zr, zi};
+        }
     };
     auto fill_col = [&](int j, c128* out) {  // out[i] = Z[i, j]
-        for (int r0 = 0; r0 < M; r0 += kTileRows)
-            fill_any<double, c128, true>(out, M, J.rows, M, J.cols + j, 1, r0, 0, nullptr, rec, nullptr, tg, k, eta,
-                                         lds, kNoTail);
+        const double* b = rec + (size_t)J.cols[j] * kRecStride;
+        for (int i = tid; i < M; i += nt) {
+            RwgRec<double> a;
+            load_rec(rec, J.rows[i], a);
+            double zr, zi;
+            entry_f64(a, b, fc.k, fc.c4k2, fc.scale, zr, zi);
+            out[i] = c128{zr, zi};
+        }
     };
     if (M == 1 || N == 1) {  // a vector: nothing to compress (ACA.m:47)
         if (N == 1) {

@@ -88,7 +88,10 @@ def test_aca_ragged_and_rank_cap(farcur, oracle):
             A = out["U"] @ out["V"]
             np.testing.assert_allclose(A[out["I"], :], Z[out["I"], :], atol=1e-8 * np.abs(Z).max())
             np.testing.assert_allclose(A[:, out["J"]], Z[:, out["J"]], atol=1e-8 * np.abs(Z).max())
-            assert _relfro(out["U"] @ out["V"], Z) <= 2 * _relfro(Uo @ Vo, Z) + 1e-12, (m, n, max_rank)
+            # different tie-broken pivot paths stop at different crosses; the
+            # stopping rule estimates the error at the tol level (ACA.m:114)
+            e, eo = _relfro(out["U"] @ out["V"], Z), _relfro(Uo @ Vo, Z)
+            assert e <= max(2 * eo, 5e-6) + 1e-12, (m, n, max_rank, e, eo)
             if max_rank:
                 assert ranks[b] <= max_rank and out["status"] == so
     bt.close()
