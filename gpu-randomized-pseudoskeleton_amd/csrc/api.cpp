@@ -1604,12 +1604,12 @@ struct farcur_batch {
     bool interp_ready = false;
     // SVD recompression of the last result: outputs (C', R', sigma per block at
     // rc_off, k in rc_k), workspace and job records
-    DBuf d_rc, d_rc_ws, d_rc_jobs;
+    DBuf d_rc, d_rc_ws, d_rc_jobs, d_rc_k;
     std::vector<int32_t> rc_k;
     std::vector<size_t> rc_off;
     bool rc_ready = false;
     // ACA of every block (k_aca): per-block regions of d_aca at aca_off
-    DBuf d_aca, d_aca_jobs;
+    DBuf d_aca, d_aca_jobs, d_aca_ks;  // d_aca_ks: k then status, nblocks each
     std::vector<size_t> aca_off;
     std::vector<int32_t> aca_kmax, aca_k, aca_status;
     bool aca_ready = false;
@@ -2074,7 +2074,7 @@ extern "C" void farcur_batch_destroy(farcur_batch* B) {
     hipStreamSynchronize(B->ctx->stream);
     DBuf* bufs[] = {&B->d_rows, &B->d_cols, &B->d_rperm, &B->d_cperm, &B->d_v,    &B->d_ya,  &B->d_yb,
                     &B->d_vo,   &B->d_plan, &B->d_feat, &B->d_cap, &B->d_rpred, &B->d_rc,  &B->d_rc_ws,
-                    &B->d_rc_jobs, &B->d_aca, &B->d_aca_jobs};
+                    &B->d_rc_jobs, &B->d_rc_k, &B->d_aca, &B->d_aca_jobs, &B->d_aca_ks};
     for (DBuf* b : bufs) b->release();
     for (RoundGraph& g : B->graphs) g.release();
     B->arena.release();
@@ -2082,16 +2082,15 @@ extern "C" void farcur_batch_destroy(farcur_batch* B) {
 }
 
 // Byte offsets of one block's recompression outputs within its region:
-// C' (m x r), R' (r x n) in panel precision, sigma (r doubles), k.
+// C' (m x r), R' (r x n) in panel precision, sigma (r doubles).
 struct RcLayout {
-    size_t c, r, sigma, k, size;
+    size_t c, r, sigma, size;
     RcLayout(int m, int n, int rr, size_t ps) {
         auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
         c = 0;
         r = al((size_t)m * rr * ps);
         sigma = r + al((size_t)rr * n * ps);
-        k = sigma + al((size_t)rr * 8);
-        size = k + 256;
+        size = sigma + al((size_t)rr * 8);
     }
 };
 
@@ -2116,6 +2115,7 @@ extern "C" int farcur_batch_recompress(farcur_batch* B, double thresh, int* rank
         B->rc_off[b + 1] = B->rc_off[b] + RcLayout(s.m, s.n, s.fin_r, ps).size;
     }
     TRY(B->d_rc.ensure(B->rc_off[B->nb]));
+    TRY(B->d_rc_k.ensure((size_t)B->nb * 4));
     // workspace reused by chunks of blocks of at most ~1 GiB together
     const size_t ws_cap = (size_t)1 << 26;  // c128 elements
     size_t ws_max = 0;
@@ -2144,7 +2144,7 @@ extern "C" int farcur_batch_recompress(farcur_batch* B, double thresh, int* rank
         J.Cout = o + L.c;
         J.Rout = o + L.r;
         J.sigma = (double*)(o + L.sigma);
-        J.k_out = (int32_t*)(o + L.k);
+        J.k_out = B->d_rc_k.as<int32_t>() + b;
         ws_acc += w;
         ws_max = std::max(ws_max, ws_acc);
     }
@@ -2161,8 +2161,7 @@ extern "C" int farcur_batch_recompress(farcur_batch* B, double thresh, int* rank
     HIPTRY(hipGetLastError());
     HIPTRY(hipEventRecord(ctx->ev1, st));
     B->rc_k.assign(B->nb, 0);
-    for (int b = 0; b < B->nb; ++b)
-        HIPTRY(hipMemcpyAsync(&B->rc_k[b], jobs[b].k_out, 4, hipMemcpyDeviceToHost, st));
+    HIPTRY(hipMemcpyAsync(B->rc_k.data(), B->d_rc_k.p, (size_t)B->nb * 4, hipMemcpyDeviceToHost, st));
     HIPTRY(hipStreamSynchronize(st));
     float t = 0;
     HIPTRY(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
@@ -2195,9 +2194,9 @@ extern "C" int farcur_batch_recompressed_block(farcur_batch* B, int b, farcur_c1
 }
 
 // Byte offsets of one block's ACA region: U (m x kmax), Vt (n x kmax), the
-// residual row (n), used flags (m + n), pivots I, J (kmax), k, status.
+// residual row (n), used flags (m + n), pivots I, J (kmax).
 struct AcaLayout {
-    size_t U, Vt, row, used, I, J, k, status, size;
+    size_t U, Vt, row, used, I, J, size;
     AcaLayout(int m, int n, int kmax) {
         auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
         U = 0;
@@ -2206,9 +2205,7 @@ struct AcaLayout {
         used = row + al((size_t)n * 16);
         I = used + al((size_t)(m + n) * 4);
         J = I + al((size_t)kmax * 4);
-        k = J + al((size_t)kmax * 4);
-        status = k + 256;
-        size = status + 256;
+        size = J + al((size_t)kmax * 4);
     }
 };
 
@@ -2231,6 +2228,7 @@ extern "C" int farcur_batch_aca(farcur_batch* B, double tol, int max_rank, int* 
         B->aca_off[b + 1] = B->aca_off[b] + AcaLayout(m, n, B->aca_kmax[b]).size;
     }
     TRY(B->d_aca.ensure(B->aca_off[B->nb]));
+    TRY(B->d_aca_ks.ensure((size_t)B->nb * 8));
     std::vector<AcaJob> jobs(B->nb);
     for (int b = 0; b < B->nb; ++b) {
         const int m = B->row_off[b + 1] - B->row_off[b], n = B->col_off[b + 1] - B->col_off[b];
@@ -2249,8 +2247,8 @@ extern "C" int farcur_batch_aca(farcur_batch* B, double tol, int max_rank, int* 
         J.used = (int32_t*)(o + L.used);
         J.I = (int32_t*)(o + L.I);
         J.J = (int32_t*)(o + L.J);
-        J.k_out = (int32_t*)(o + L.k);
-        J.status = (int32_t*)(o + L.status);
+        J.k_out = B->d_aca_ks.as<int32_t>() + b;
+        J.status = B->d_aca_ks.as<int32_t>() + B->nb + b;
     }
     TRY(B->d_aca_jobs.ensure(jobs.size() * sizeof(AcaJob)));
     hipStream_t st = ctx->stream;
@@ -2259,13 +2257,11 @@ extern "C" int farcur_batch_aca(farcur_batch* B, double tol, int max_rank, int* 
     launch_aca(B->d_aca_jobs.as<AcaJob>(), B->nb, tol, ctx->fill_params(0), st);
     HIPTRY(hipGetLastError());
     HIPTRY(hipEventRecord(ctx->ev1, st));
-    B->aca_k.assign(B->nb, 0);
-    B->aca_status.assign(B->nb, 0);
-    for (int b = 0; b < B->nb; ++b) {
-        HIPTRY(hipMemcpyAsync(&B->aca_k[b], jobs[b].k_out, 4, hipMemcpyDeviceToHost, st));
-        HIPTRY(hipMemcpyAsync(&B->aca_status[b], jobs[b].status, 4, hipMemcpyDeviceToHost, st));
-    }
+    std::vector<int32_t> ks((size_t)B->nb * 2);
+    HIPTRY(hipMemcpyAsync(ks.data(), B->d_aca_ks.p, ks.size() * 4, hipMemcpyDeviceToHost, st));
     HIPTRY(hipStreamSynchronize(st));
+    B->aca_k.assign(ks.begin(), ks.begin() + B->nb);
+    B->aca_status.assign(ks.begin() + B->nb, ks.end());
     float t = 0;
     HIPTRY(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
     if (ms) *ms = t;
