@@ -22,6 +22,7 @@
 #include "kernels.hpp"
 #include "mlp.hpp"
 #include "mrhs.hpp"
+#include "noalloc.hpp"
 #include "philox.hpp"
 
 using namespace farcur;
@@ -817,6 +818,9 @@ int farcur_create(int device, farcur_ctx** out) {
     auto ctx = std::make_unique<farcur_ctx>();
     ctx->device = device;
     HIPTRY(hipSetDevice(device));
+    // device address of the allocation-free pinv's static scratch, once per device
+    if (const int e = noalloc_init(device))
+        return fail(FARCUR_EHIP, "hipGetSymbolAddress of the no-allocation scratch failed (%d)", e);
     HIPTRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     HIPTRY(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
     HIPTRY(hipEventCreate(&ctx->ev0));
@@ -943,6 +947,25 @@ int farcur_pinv(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, doubl
     std::sort(s.begin(), s.end(), [](double a, double b) { return a > b; });
     if (sigma) std::copy(s.begin(), s.end(), sigma);
     if (kept) *kept = plan.out.kept;
+    return FARCUR_OK;
+}
+
+int farcur_pinv_noalloc(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, double rtol, farcur_c128* U,
+                        double* sigma, int* kept, void* stream) {
+    if (!ctx || !W || !U) return fail(FARCUR_EINVAL, "NULL argument");
+    if (r <= 0 || r > kNoAllocMaxR)
+        return fail(FARCUR_EINVAL, "r = %d outside [1, %d], the largest core the no-allocation scratch holds", r,
+                    kNoAllocMaxR);
+    if (!(atol >= 0) || !(rtol >= 0)) return fail(FARCUR_EINVAL, "thresholds must be >= 0");
+    unsigned char* base = noalloc_scratch(ctx->device);
+    if (!base) return fail(FARCUR_ESTATE, "no-allocation scratch not set up on device %d", ctx->device);
+    TRY(ctx->select());
+    // the same class, launch shape and threshold semantics as farcur_pinv
+    const int cls = pinv_class(r);
+    const int threads = cls < kPinvClasses - 1 ? ctx->tune.pinv_threads[cls] : (r > 128 ? 1024 : 256);
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    launch_pinv_noalloc(base, (const c128*)W, r, atol, rtol, kPinvClassMax[cls], threads, (c128*)U, sigma, kept, s);
+    HIPTRY(hipGetLastError());
     return FARCUR_OK;
 }
 

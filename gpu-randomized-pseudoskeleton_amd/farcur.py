@@ -143,6 +143,7 @@ def _setup(L: ctypes.CDLL) -> None:
     L.farcur_set_mesh.argtypes = [vp, vp]
     L.farcur_fill.argtypes = [vp, vp, ci, vp, ci, ci, vp]
     L.farcur_pinv.argtypes = [vp, vp, ci, cd, cd, vp, vp, vp]
+    L.farcur_pinv_noalloc.argtypes = [vp, vp, ci, cd, cd, vp, vp, vp, vp]
     L.farcur_cur_residual.argtypes = [vp, vp, ci, ci, vp, vp, ci, vp, vp, vp, vp]
     L.farcur_sample.argtypes = [vp, cu64, ci, ci, ci, ci, ci, vp]
     L.farcur_vtest.argtypes = [cu64, ci, ci, vp]
@@ -394,6 +395,13 @@ class Context:
         kept = ci()
         check(lib().farcur_pinv(self._h, _p(W), r, atol, rtol, _p(U), _p(s), ctypes.byref(kept)))
         return U, s, kept.value
+
+    def pinv_noalloc(self, W_dev: int, r: int, atol: float, rtol: float, U_dev: int, sigma_dev: int = 0,
+                     kept_dev: int = 0, stream: int = 0) -> None:
+        """farcur_pinv_noalloc on device addresses (ints): enqueues only, on
+        `stream` (0: the context's stream); the caller synchronises."""
+        check(lib().farcur_pinv_noalloc(self._h, W_dev, r, atol, rtol, U_dev, sigma_dev or None, kept_dev or None,
+                                        stream or None))
 
     def cur_residual(self, C, U, R, v, y_old):
         C, U, R, v, y_old = _cmat(C), _cmat(U), _cmat(R), _cvec(v), _cvec(y_old)

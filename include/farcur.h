@@ -150,6 +150,22 @@ int farcur_fill(farcur_ctx* ctx, const int* rows, int nr, const int* cols, int n
 int farcur_pinv(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, double rtol, farcur_c128* U,
                 double* sigma, int* kept);
 
+/* farcur_pinv without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid hipMalloc).
+ * No counterpart in the reference. W, U (r x r), sigma (r, descending; may be
+ * NULL) and kept (one int; may be NULL) are DEVICE pointers; W is not
+ * modified. The call only enqueues its kernels on `stream` (a hipStream_t;
+ * NULL: the context's stream) and returns: the results are ready when that
+ * stream reaches them. Same kernels, launch shapes and threshold semantics
+ * as farcur_pinv, so the outputs match it. Its scratch (W's working copy, the
+ * Jacobi workspace, the job record) is a static device array of about 2 MiB,
+ * set up once per device by farcur_create and held by every GPU the library
+ * runs on until the process ends: r above 256 is rejected (FARCUR_EINVAL).
+ * Every call on one GPU shares that array, so calls that overlap in time on
+ * one device (other streams or contexts) race; serialise them. */
+int farcur_pinv_noalloc(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, double rtol, farcur_c128* U,
+                        double* sigma, int* kept, void* stream);
+
 /* y_new = C (U (R v)), returns err = ||y_old - y_new|| / ||y_new|| in *err
  * (CU:1928-1966). C m x r, U r x r, R r x n. */
 int farcur_cur_residual(farcur_ctx* ctx, const farcur_c128* C, int m, int r, const farcur_c128* U,
