@@ -138,6 +138,255 @@ void orc_block_matvec(const orc_geom* g, const int32_t* rows, int nr, const int3
     }
 }
 
+/* ------------------------------------------------------------------ near field
+ * Near-field element: the reference integrates triangle pairs closer than
+ * its Rinteg_s / Ranal_s radii with singular rules inside its binary-only
+ * element (options at compres_single_block_spheres_for_sweep2.jl:1095-1098,
+ * declaration user_impedance_for_julia_cuda.cu:178-214). Restated here with
+ * the textbook singularity extraction: G = 1/(4 pi R) + (e^{-jkR} - 1)/(4 pi R),
+ * the 1/R part integrated over the source triangle in closed form (Wilton,
+ * Rao, Glisson, Schaubert, Al-Bundak, Butler, "Potential integrals for
+ * uniform and linear source distributions on polygonal and polyhedral
+ * domains", IEEE Trans. AP-32(3), 1984), the bounded remainder and the test
+ * triangle by the 7-point degree-5 rule (Dunavant 1985). A pair is near when
+ * the centroid distance is below near_factor x the longest edge of the two
+ * triangles; other pairs keep the 3 x 3 rule of orc_element.
+ * Pins (tests/test_oracle.py): the closed forms against polar quadrature
+ * around the projected point (in plane, above, on a vertex's extension) and
+ * against brute force away from the triangle; the element against the far
+ * rule for separated pairs; reciprocity; the Mie series of a PEC sphere.
+ */
+static const double kD7[7][4] = {
+    /* barycentric a, b, c and weight (sum 1) */
+    {1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 0.225},
+    {0.059715871789770, 0.470142064105115, 0.470142064105115, 0.132394152788506},
+    {0.470142064105115, 0.059715871789770, 0.470142064105115, 0.132394152788506},
+    {0.470142064105115, 0.470142064105115, 0.059715871789770, 0.132394152788506},
+    {0.797426985353087, 0.101286507323456, 0.101286507323456, 0.125939180544827},
+    {0.101286507323456, 0.797426985353087, 0.101286507323456, 0.125939180544827},
+    {0.101286507323456, 0.101286507323456, 0.797426985353087, 0.125939180544827},
+};
+
+static double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static void cross3(const double* a, const double* b, double* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* Is = int_T 1/|r - r'| dS', Iv = int_T (r' - rho)/|r - r'| dS' with rho the
+ * projection of r on the plane of T (v0, v1, v2 counterclockwise about the
+ * normal n = (v1 - v0) x (v2 - v0) / |...|). Wilton et al. 1984, eqs. for
+ * uniform and linear sources: per edge i (a -> b), unit tangent l, outward
+ * in-plane normal u = l x n, l+- = (b|a - rho).l, P0 = (a - rho).u,
+ * R0^2 = P0^2 + h^2, R+- = |r - b|a|, f = ln((R+ + l+)/(R- + l-)):
+ *   Is = sum P0 f - |h| sum [atan(P0 l+ / (R0^2 + |h| R+)) - atan(P0 l- / (R0^2 + |h| R-))]
+ *   Iv = 1/2 sum u [R0^2 f + l+ R+ - l- R-]. */
+void orc_potentials(const double* v0, const double* v1, const double* v2, const double* r, double* Is, double* Iv) {
+    const double* V[3] = {v0, v1, v2};
+    double e1[3], e2[3], n[3];
+    for (int c = 0; c < 3; ++c) {
+        e1[c] = v1[c] - v0[c];
+        e2[c] = v2[c] - v0[c];
+    }
+    cross3(e1, e2, n);
+    const double nn = sqrt(dot3(n, n));
+    for (int c = 0; c < 3; ++c) n[c] /= nn;
+    double d0[3];
+    for (int c = 0; c < 3; ++c) d0[c] = r[c] - v0[c];
+    const double h = dot3(d0, n), ah = fabs(h);
+    double rho[3];
+    for (int c = 0; c < 3; ++c) rho[c] = r[c] - h * n[c];
+    double is = 0, iv[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i) {
+        const double* a = V[i];
+        const double* b = V[(i + 1) % 3];
+        double l[3], u[3], am[3], bm[3];
+        for (int c = 0; c < 3; ++c) {
+            l[c] = b[c] - a[c];
+            am[c] = a[c] - rho[c];
+            bm[c] = b[c] - rho[c];
+        }
+        const double len = sqrt(dot3(l, l));
+        for (int c = 0; c < 3; ++c) l[c] /= len;
+        cross3(l, n, u);
+        const double lp = dot3(bm, l), lm = dot3(am, l), p0 = dot3(am, u);
+        const double r02 = p0 * p0 + h * h;
+        double ra[3], rb[3];
+        for (int c = 0; c < 3; ++c) {
+            ra[c] = r[c] - a[c];
+            rb[c] = r[c] - b[c];
+        }
+        const double rp = sqrt(dot3(rb, rb)), rm = sqrt(dot3(ra, ra));
+        double f = 0;
+        if (r02 > 1e-28 * len * len) {
+            /* the form without cancellation: (R + l)(R - l) = R0^2 */
+            f = (lp + lm >= 0) ? log((rp + lp) / (rm + lm)) : log((rm - lm) / (rp - lp));
+        } else if (lm > 0 || lp < 0) {
+            /* on the edge's line, outside the segment: f finite, its weights P0 and R0^2 vanish */
+            f = 0;
+        }
+        is += p0 * f;
+        if (ah > 0) is -= ah * (atan2(p0 * lp, r02 + ah * rp) - atan2(p0 * lm, r02 + ah * rm));
+        const double vv = 0.5 * (r02 * f + lp * rp - lm * rm);
+        for (int c = 0; c < 3; ++c) iv[c] += vv * u[c];
+    }
+    *Is = is;
+    for (int c = 0; c < 3; ++c) Iv[c] = iv[c];
+}
+
+static void tri_vertices(const orc_geom* g, int t, const double* v[3]) {
+    for (int a = 0; a < 3; ++a) v[a] = &g->vertex[3 * g->topol[3 * t + a]];
+}
+
+static double tri_longest_edge(const double* v[3]) {
+    double m = 0;
+    for (int i = 0; i < 3; ++i) {
+        double d[3];
+        for (int c = 0; c < 3; ++c) d[c] = v[(i + 1) % 3][c] - v[i][c];
+        m = fmax(m, sqrt(dot3(d, d)));
+    }
+    return m;
+}
+
+/* Singular-extraction integral of one triangle pair (test ta with free
+ * vertex va, source tb with free vertex vb):
+ *   J = int_Ta int_Tb [(r - va).(r' - vb) - 4/k^2] G dS' dS  (complex). */
+static void near_pair(const orc_geom* g, int ta, const double* va, int tb, const double* vb, double* jr, double* ji) {
+    const double k = g->k;
+    const double *A[3], *B[3];
+    tri_vertices(g, ta, A);
+    tri_vertices(g, tb, B);
+    double aa, ab, qa[3][3], qb[3][3];
+    tri_points(g, ta, qa, &aa);
+    tri_points(g, tb, qb, &ab);
+    (void)qa;
+    (void)qb;
+    double nb[3], e1[3], e2[3];
+    for (int c = 0; c < 3; ++c) {
+        e1[c] = B[1][c] - B[0][c];
+        e2[c] = B[2][c] - B[0][c];
+    }
+    cross3(e1, e2, nb);
+    const double nn = sqrt(dot3(nb, nb));
+    for (int c = 0; c < 3; ++c) nb[c] /= nn;
+    double sr = 0, si = 0;
+    for (int p = 0; p < 7; ++p) {
+        double r[3];
+        for (int c = 0; c < 3; ++c) r[c] = kD7[p][0] * A[0][c] + kD7[p][1] * A[1][c] + kD7[p][2] * A[2][c];
+        const double wp = kD7[p][3] * aa;
+        /* 1/R part in closed form */
+        double is, iv[3];
+        orc_potentials(B[0], B[1], B[2], r, &is, iv);
+        const double h = (r[0] - B[0][0]) * nb[0] + (r[1] - B[0][1]) * nb[1] + (r[2] - B[0][2]) * nb[2];
+        double i1s[3];  /* int (r' - vb)/R = Iv + (rho - vb) Is */
+        for (int c = 0; c < 3; ++c) i1s[c] = iv[c] + (r[c] - h * nb[c] - vb[c]) * is;
+        /* bounded remainder (e^{-jkR} - 1)/R by the 7-point rule */
+        double i0r = 0, i0i = 0, i1r[3] = {0, 0, 0}, i1i[3] = {0, 0, 0};
+        for (int q = 0; q < 7; ++q) {
+            double rs[3], d[3];
+            for (int c = 0; c < 3; ++c) {
+                rs[c] = kD7[q][0] * B[0][c] + kD7[q][1] * B[1][c] + kD7[q][2] * B[2][c];
+                d[c] = r[c] - rs[c];
+            }
+            const double R = sqrt(dot3(d, d));
+            double gr, gi;  /* (e^{-jkR} - 1)/R, its limit -jk at R = 0 */
+            if (R > 1e-12 * sqrt(ab)) {
+                gr = (cos(k * R) - 1.0) / R;
+                gi = -sin(k * R) / R;
+            } else {
+                gr = 0;
+                gi = -k;
+            }
+            const double w = kD7[q][3] * ab;
+            i0r += w * gr;
+            i0i += w * gi;
+            for (int c = 0; c < 3; ++c) {
+                i1r[c] += w * gr * (rs[c] - vb[c]);
+                i1i[c] += w * gi * (rs[c] - vb[c]);
+            }
+        }
+        /* integrand (r - va).I1 - 4/k^2 I0, I0 = is + i0, I1 = i1s + i1 (times 1/4pi below) */
+        double tr = 0, ti = 0;
+        for (int c = 0; c < 3; ++c) {
+            const double x = r[c] - va[c];
+            tr += x * (i1s[c] + i1r[c]);
+            ti += x * i1i[c];
+        }
+        const double c4 = 4.0 / (k * k);
+        tr -= c4 * (is + i0r);
+        ti -= c4 * i0i;
+        sr += wp * tr;
+        si += wp * ti;
+    }
+    *jr = sr / (4 * M_PI);
+    *ji = si / (4 * M_PI);
+}
+
+/* Element with the near-pair rule (see above). */
+void orc_element_near(const orc_geom* g, int m, int n, double near_factor, double* zre, double* zim) {
+    const double k = g->k;
+    double sr = 0, si = 0;  /* sum of s_a s_b l_m l_n / (4 A_a A_b) J */
+    for (int hm = 0; hm < 2; ++hm) {
+        const int tm = g->edges[4 * m + hm];
+        const double* vm = &g->vertex[3 * g->edges[4 * m + 2 + hm]];
+        const double sm = hm == 0 ? 1.0 : -1.0;
+        double qm[3][3], am;
+        tri_points(g, tm, qm, &am);
+        const double *Vm[3];
+        tri_vertices(g, tm, Vm);
+        double cm[3];
+        for (int c = 0; c < 3; ++c) cm[c] = (Vm[0][c] + Vm[1][c] + Vm[2][c]) / 3.0;
+        for (int hn = 0; hn < 2; ++hn) {
+            const int tn = g->edges[4 * n + hn];
+            const double* vn = &g->vertex[3 * g->edges[4 * n + 2 + hn]];
+            const double sn = hn == 0 ? 1.0 : -1.0;
+            double qn[3][3], an;
+            tri_points(g, tn, qn, &an);
+            const double *Vn[3];
+            tri_vertices(g, tn, Vn);
+            double d[3];
+            for (int c = 0; c < 3; ++c) d[c] = cm[c] - (Vn[0][c] + Vn[1][c] + Vn[2][c]) / 3.0;
+            const double lim = near_factor * fmax(tri_longest_edge(Vm), tri_longest_edge(Vn));
+            const double coef = sm * sn * g->ln[m] * g->ln[n] / (4 * am * an);
+            if (dot3(d, d) < lim * lim) {
+                double jr, ji;
+                near_pair(g, tm, vm, tn, vn, &jr, &ji);
+                sr += coef * jr;
+                si += coef * ji;
+            } else {
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) {
+                        double pp = 0, r2 = 0;
+                        for (int c = 0; c < 3; ++c) {
+                            pp += (qm[i][c] - vm[c]) * (qn[j][c] - vn[c]);
+                            const double e = qm[i][c] - qn[j][c];
+                            r2 += e * e;
+                        }
+                        const double R = sqrt(r2);
+                        const double br = (pp - 4.0 / (k * k)) * am * an / 9.0;
+                        sr += coef * br * cos(k * R) / (4 * M_PI * R);
+                        si -= coef * br * sin(k * R) / (4 * M_PI * R);
+                    }
+            }
+        }
+    }
+    *zre = k * g->eta * si;
+    *zim = -k * g->eta * sr;
+}
+
+void orc_fill_near(const orc_geom* g, const int32_t* rows, int nr, const int32_t* cols, int nc, double near_factor,
+                   double* Z) {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int j = 0; j < nc; ++j)
+        for (int i = 0; i < nr; ++i) {
+            double re, im;
+            orc_element_near(g, rows[i], cols[j], near_factor, &re, &im);
+            Z[2 * ((size_t)j * nr + i)] = re;
+            Z[2 * ((size_t)j * nr + i) + 1] = im;
+        }
+}
+
 /* ------------------------------------------------------------------ pinv */
 /*
  * One-sided Jacobi SVD of the r x r complex matrix W (column-major,

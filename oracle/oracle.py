@@ -99,6 +99,9 @@ def _setup(L: ctypes.CDLL) -> None:
     L.orc_pivot_rows.argtypes = [vp, i, i, vp]
     L.orc_compress_block.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.orc_compress_block.restype = i
+    L.orc_potentials.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.orc_element_near.argtypes = [vp, i, i, d, vp, vp]
+    L.orc_fill_near.argtypes = [vp, vp, i, vp, i, d, vp]
     L.orc_num_threads.restype = i
     L.orc_set_threads.argtypes = [i]
 
@@ -134,6 +137,15 @@ class Geometry:
         cols = np.ascontiguousarray(cols, dtype=np.int32)
         Z = np.empty((cols.size, rows.size), dtype=np.complex128)  # column-major storage
         lib().orc_fill(self.ptr, _ptr(rows), rows.size, _ptr(cols), cols.size, _ptr(Z))
+        return Z.T
+
+    def fill_near(self, rows, cols, near_factor=2.0) -> np.ndarray:
+        """Z[rows, cols] with triangle pairs closer than near_factor x the longer
+        longest edge integrated by singularity extraction (orc_element_near)."""
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        Z = np.empty((cols.size, rows.size), dtype=np.complex128)
+        lib().orc_fill_near(self.ptr, _ptr(rows), rows.size, _ptr(cols), cols.size, float(near_factor), _ptr(Z))
         return Z.T
 
     def block_matvec(self, rows, cols, v) -> np.ndarray:
@@ -246,6 +258,16 @@ def compress_block(geom: Geometry | None, rows=None, cols=None, dense=None, *, t
     }
 
 
+def potentials(v0, v1, v2, r):
+    """(int_T 1/R dS', int_T (r' - rho)/R dS') over the triangle (v0, v1, v2)
+    for the point r (Wilton et al. 1984 closed forms, orc_potentials)."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (v0, v1, v2, r)]
+    Is = ctypes.c_double()
+    Iv = np.zeros(3)
+    lib().orc_potentials(*[_ptr(x) for x in a], ctypes.byref(Is), _ptr(Iv))
+    return Is.value, Iv
+
+
 def num_threads() -> int:
     return lib().orc_num_threads()
 
@@ -280,7 +302,9 @@ def aca(geom: Geometry, rows, cols, tol, max_rank=128):
     """Partially pivoted ACA of Z[rows, cols] (codes/ACA_vs_CUR/seed/ACA.m:40-128),
     rows and columns of the residual filled with the FP64 element. Returns
     (U m x k, V k x n, I, J, status); status 2 when max_rank crosses were taken
-    without convergence. Blocks with one row or column are returned whole
+    without convergence, 3 when the residual row or column has no finite entry
+    left to pivot on or the cross norms are not finite (the element is
+    non-finite for coincident triangles). Blocks with one row or column are returned whole
     (ACA.m:47). Pins: exact recovery of a block of rank k in k steps, the
     interpolation property (U V matches Z on every pivot row and column), the
     normZ recursion equals |U V|_F^2 in exact arithmetic bounds (tests/test_oracle.py)."""
@@ -306,8 +330,12 @@ def aca_dense(row, col, M, N, tol, max_rank=128):
         R = row(Ik).astype(complex)
         for u, v in zip(Us, Vs):
             R -= u[Ik] * v
-        a = np.where(used_c, -1.0, np.abs(R) ** 2)
-        Jk = int(np.argmax(a))  # first index among equal maxima
+        a = np.abs(R) ** 2
+        cand = ~used_c & np.isfinite(a)  # a non-finite entry is never a pivot
+        if not cand.any():  # no finite residual entry left: status 3
+            status = 3
+            break
+        Jk = int(np.argmax(np.where(cand, a, -1.0)))  # first index among equal maxima
         piv = R[Jk]
         if piv == 0:
             status = 1
@@ -325,6 +353,9 @@ def aca_dense(row, col, M, N, tol, max_rank=128):
         Us.append(Uk)
         Vs.append(Vk)
         k = len(Us)
+        if not (np.isfinite(nu * nv) and np.isfinite(normZ)):  # NaN or Inf in the cross: status 3
+            status = 3
+            break
         if k > 1 and np.sqrt(nu * nv) <= tol * np.sqrt(normZ):
             status = 1
             break
@@ -334,8 +365,12 @@ def aca_dense(row, col, M, N, tol, max_rank=128):
         if k == kend:
             status = 2
             break
-        a = np.where(used_r, -1.0, np.abs(Uk) ** 2)
-        Ik = int(np.argmax(a))
+        a = np.abs(Uk) ** 2
+        cand = ~used_r & np.isfinite(a)
+        if not cand.any():
+            status = 3
+            break
+        Ik = int(np.argmax(np.where(cand, a, -1.0)))
         used_r[Ik] = True
     U = np.array(Us).T if Us else np.zeros((M, 0), complex)
     V = np.array(Vs) if Vs else np.zeros((0, N), complex)

@@ -405,3 +405,148 @@ def test_aca_pins(oracle):
     # vectors are returned whole (ACA.m:47)
     Uv, Vv, *_ = oracle.aca_dense(row, col, 1, 50, 1e-6)
     np.testing.assert_array_equal((Uv @ Vv)[0], Z[0])
+
+
+def test_aca_non_finite_entries_stop_with_status_3(oracle):
+    # Coincident row and column triangles make the 3-point element non-finite
+    # (R = 0). The cross iteration must then stop with status 3 instead of
+    # pivoting on a NaN: a row with no finite entry stops before any cross, a
+    # NaN column stops after the first cross. The GPU kernel has the same
+    # guards; the library also refuses such blocks when the batch is created.
+    rng = np.random.default_rng(6)
+    Z = _smooth_block(rng, 30, 20)
+    Zn = Z.copy()
+    Zn[0, :] = np.nan
+    U, V, I, J, st = oracle.aca_dense(lambda i: Zn[i].copy(), lambda j: Zn[:, j].copy(), 30, 20, 1e-6)
+    assert st == 3 and U.shape == (30, 0) and V.shape == (0, 20) and I.size == 0
+    Zc = Z.copy()
+    j0 = int(np.argmax(np.abs(Z[0])))
+    Zc[5:, j0] = np.nan  # the first pivot column carries NaN below row 0
+    U, V, I, J, st = oracle.aca_dense(lambda i: Zc[i].copy(), lambda j: Zc[:, j].copy(), 30, 20, 1e-6)
+    assert st == 3 and U.shape[1] == 1 and J[0] == j0
+    # finite entries next to a NaN are still pivots: the NaN is skipped, not chosen
+    Zp = Z.copy()
+    Zp[0, 3] = np.nan
+    U, V, I, J, st = oracle.aca_dense(lambda i: Zp[i].copy(), lambda j: Zp[:, j].copy(), 30, 20, 1e-6)
+    assert J[0] == int(np.nanargmax(np.abs(Zp[0])))
+
+
+# ---------------------------------------------------------------- near field
+
+
+def _polar_potentials(v, r, n=256):
+    """int_T 1/R and int_T (r' - rho)/R by Gauss-Legendre quadrature in polar
+    coordinates about the projection rho of r: T splits into the signed
+    triangles (rho, v_i, v_i+1), each integrated along rays from rho to the
+    edge (v_i, v_i+1) with the Jacobian t, which removes the 1/R singularity.
+    Independent of the closed forms under test."""
+    v = [np.asarray(x, float) for x in v]
+    r = np.asarray(r, float)
+    nrm = np.cross(v[1] - v[0], v[2] - v[0])
+    nrm /= np.linalg.norm(nrm)
+    h = np.dot(r - v[0], nrm)
+    rho = r - h * nrm
+    x, w = np.polynomial.legendre.leggauss(n)
+    Is, Iv = 0.0, np.zeros(3)
+    for i in range(3):
+        A, B = v[i] - rho, v[(i + 1) % 3] - rho
+        sgn = np.sign(np.dot(np.cross(A, B), nrm))
+        if sgn == 0:
+            continue
+        s = 0.5 * (x + 1)  # position along the edge
+        P = A[None, :] + s[:, None] * (B - A)[None, :]  # edge points relative to rho
+        L = np.linalg.norm(P, axis=1)
+        d = P / L[:, None]
+        jac = np.linalg.norm(np.cross(A, B)) / L ** 2 * 0.5 * w  # d(angle) = |A x B| / L^2 ds
+        t = 0.5 * L[:, None] * (x[None, :] + 1)  # radius along each ray
+        f = t / np.sqrt(t * t + h * h) * 0.5 * L[:, None] * w[None, :] * (jac * sgn)[:, None]
+        Is += f.sum()
+        Iv += ((f * t).sum(axis=1)[:, None] * d).sum(axis=0)
+    return Is, Iv
+
+
+@pytest.mark.parametrize("r", [[0.2, 0.25, 0.0], [0.2, 0.25, 0.3], [0.2, 0.25, -0.05], [1.5, -0.5, 0.0],
+                               [2.0, 0.0, 0.0], [-1.0, 0.0, 0.0], [2.0, 0.0, 0.4], [0.0, 0.0, 0.5],
+                               [0.5, 0.5, 1e-3], [3.0, 3.0, 1.0]])
+def test_potentials_closed_form_vs_polar_quadrature(oracle, r):
+    # in-plane inside and outside, above and below, on an edge's extension
+    # (R0 = 0 outside the segment), above a vertex, near the hypotenuse, far
+    v = ([0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [0.1, 0.9, 0.0])
+    Is, Iv = oracle.potentials(*v, r)
+    Js, Jv = _polar_potentials(v, r)
+    assert Is == pytest.approx(Js, rel=1e-9, abs=1e-12)
+    np.testing.assert_allclose(Iv, Jv, rtol=1e-9, atol=1e-12)
+
+
+def _plate_geometry(farcur, oracle, nsq):
+    mesh = farcur.Mesh.plate(1.0, 1.0, nsq, nsq, 0, 0, 0).build_rwg()
+    t = mesh.tables()
+    return t, oracle.Geometry(t["vertex"], t["topol"], t["edges"], t["ln"])
+
+
+def test_near_element_matches_far_rule_on_separated_pairs(farcur, oracle):
+    # The singular-extraction path (forced with a huge near_factor) on pairs
+    # several triangles apart integrates the same smooth kernel as the 3 x 3
+    # rule, with the 7-point rule: both agree to the lower rule's error.
+    t, g = _plate_geometry(farcur, oracle, 12)
+    rows = np.array([0, 3, 10])
+    cols = np.array([int(t["ln"].size) - 1, int(t["ln"].size) - 20])
+    Zf = g.fill(rows, cols)
+    Zn = g.fill_near(rows, cols, 1e9)
+    assert np.abs(Zn - Zf).max() <= 2e-3 * np.abs(Zf).max()
+    # and with the default factor these pairs are far: the same rule, equal to rounding
+    np.testing.assert_allclose(g.fill_near(rows, cols, 2.0), Zf, rtol=1e-12, atol=1e-14 * np.abs(Zf).max())
+
+
+def test_near_element_reciprocity_and_finite_self_terms(farcur, oracle):
+    t, g = _plate_geometry(farcur, oracle, 6)
+    ne = t["ln"].size
+    Z = g.fill_near(np.arange(ne), np.arange(ne))
+    assert np.all(np.isfinite(Z))
+    # the Galerkin form is symmetric; the closed-form inner and 7-point outer
+    # integrals break it only at the quadrature error
+    assert np.linalg.norm(Z - Z.T) <= 2e-3 * np.linalg.norm(Z)
+    # the far rule puts quadrature points on top of each other for coincident triangles
+    assert not np.all(np.isfinite(g.fill(np.arange(ne), np.arange(ne))))
+
+
+def test_mie_series_limits():
+    import mie
+
+    k = 2 * np.pi
+    a = 1e-3
+    # Rayleigh limit of the PEC sphere's backscatter: 9 pi a^2 (ka)^4
+    assert mie.rcs(k, a, np.array([np.pi]))[0] == pytest.approx(9 * np.pi * a * a * (k * a) ** 4, rel=1e-4)
+    # optical limit: pi a^2
+    assert mie.rcs(k, 20.0, np.array([np.pi]))[0] == pytest.approx(np.pi * 400.0, rel=0.05)
+    # optical theorem: extinction = (4 pi / k^2) Re S(0), for both amplitudes
+    S1, S2 = mie.amplitudes(k * 0.3, np.array([0.0]))
+    assert mie.extinction(k, 0.3) == pytest.approx(4 * np.pi / k ** 2 * S1[0].real, rel=1e-10)
+    assert S1[0] == pytest.approx(S2[0], rel=1e-12)
+
+
+@pytest.mark.parametrize("a", [0.25, 0.35])
+def test_pec_sphere_rcs_matches_mie(farcur, oracle, a):
+    # End to end: the dense near-field matrix of a PEC sphere (1920 RWGs),
+    # J = -Z^-1 V for an x-polarised plane wave travelling along -z (the
+    # reference's solve, compres_single_block_spheres.jl:3773, and
+    # user_compute_Ei.m's excitation), then the bistatic RCS in the E-plane
+    # against the Mie series. A wrong sign or weight of the charge term, a
+    # lost 1/4 pi or wrong self terms move it by decibels.
+    import excitation
+    import mie
+
+    mesh = farcur.Mesh.icosphere(a, a, a, 3, 0.0, 1).build_rwg()
+    t = mesh.tables()
+    k, eta = 2 * np.pi, 120 * np.pi
+    g = oracle.Geometry(t["vertex"], t["topol"], t["edges"], t["ln"], k, eta)
+    N = t["ln"].size
+    Z = g.fill_near(np.arange(N), np.arange(N), 2.0)
+    V = excitation.plane_wave(t["vertex"], t["edges"], t["ln"], t["cent"], k, 0.0, 0.0)
+    J = np.linalg.solve(Z, -V)
+    th = np.linspace(0, np.pi, 37)
+    rhat = np.vstack([np.sin(th), np.zeros_like(th), np.cos(th)])
+    s_mom = mie.mom_rcs(t["vertex"], t["topol"], t["edges"], t["ln"], J, k, eta, rhat)
+    s_mie = mie.rcs(k, a, np.pi - th, "E")
+    db = 10 * np.log10(s_mom / s_mie)
+    assert np.abs(db).max() <= 0.3, db
