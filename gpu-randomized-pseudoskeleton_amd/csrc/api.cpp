@@ -23,6 +23,9 @@
 #include "mlp.hpp"
 #include "mrhs.hpp"
 #include "noalloc.hpp"
+#include "verify.hpp"
+#include "near.hpp"
+#include <complex>
 #include "philox.hpp"
 
 using namespace farcur;
@@ -473,6 +476,106 @@ int check_ids(const int* ids, int n, int ne, const char* what) {
     return FARCUR_OK;
 }
 
+// Far-field precondition of a block rows x cols. The fill integrates every
+// triangle pair with the 3-point rule, which is only valid when the test and
+// source triangles are apart: coincident triangles put quadrature points on
+// top of each other (R = 0, a non-finite entry) and triangles that share a
+// vertex or an edge get a wrong value with no sign of it. The reference
+// switches such pairs, and every pair of triangle centroids closer than its
+// Rinteg_s / Ranal_s radii, to singular integration (options passed at
+// codes/GPU_implementation/seed/compres_single_block_spheres_for_sweep2.jl:1095-1098
+// into the element, user_impedance_for_julia_cuda.cu:178-214), which a
+// compressed far block never needs. So a block whose row and column triangles
+// share a vertex, or (rmin > 0) have two centroids closer than rmin, is
+// refused with FARCUR_EINVAL. `stamp` holds one int per vertex (all below
+// `tag` on entry); the caller passes a fresh tag per block.
+struct FarCheck {
+    std::vector<int32_t> stamp;
+    std::vector<int32_t> tri_a, tri_b;
+    int32_t tag = 0;
+};
+
+int check_far_block(const farcur_ctx* ctx, const int32_t* rows, int m, const int32_t* cols, int n, double rmin,
+                    FarCheck& fc, const char* what, int block) {
+    const int nv = (int)(ctx->h_vertex.size() / 3);
+    if ((int)fc.stamp.size() != nv) {
+        fc.stamp.assign(nv, 0);
+        fc.tag = 0;
+    }
+    if (fc.tag == INT32_MAX) {
+        std::fill(fc.stamp.begin(), fc.stamp.end(), 0);
+        fc.tag = 0;
+    }
+    const int32_t tag = ++fc.tag;
+    for (int i = 0; i < m; ++i)
+        for (int h = 0; h < 2; ++h) {
+            const int t = ctx->tri_of_rwg[2 * (size_t)rows[i] + h];
+            for (int k = 0; k < 3; ++k) fc.stamp[ctx->h_topol[3 * t + k]] = tag;
+        }
+    for (int j = 0; j < n; ++j)
+        for (int h = 0; h < 2; ++h) {
+            const int t = ctx->tri_of_rwg[2 * (size_t)cols[j] + h];
+            for (int k = 0; k < 3; ++k)
+                if (fc.stamp[ctx->h_topol[3 * t + k]] == tag)
+                    return fail(FARCUR_EINVAL,
+                                "%s %d: column RWG %d touches a row triangle (shared vertex %d): a near-field pair, "
+                                "which the far-field quadrature cannot integrate",
+                                what, block, cols[j], ctx->h_topol[3 * t + k]);
+        }
+    if (!(rmin > 0)) return FARCUR_OK;
+    // triangle centroids of both sides; a box test first, pairs only if the boxes come closer than rmin
+    auto gather = [&](const int32_t* ids, int cnt, std::vector<int32_t>& out) {
+        out.clear();
+        for (int i = 0; i < cnt; ++i)
+            for (int h = 0; h < 2; ++h) out.push_back(ctx->tri_of_rwg[2 * (size_t)ids[i] + h]);
+        std::sort(out.begin(), out.end());
+        out.erase(std::unique(out.begin(), out.end()), out.end());
+    };
+    gather(rows, m, fc.tri_a);
+    gather(cols, n, fc.tri_b);
+    auto centroid = [&](int t, double* c) {
+        for (int x = 0; x < 3; ++x)
+            c[x] = (ctx->h_vertex[3 * ctx->h_topol[3 * t] + x] + ctx->h_vertex[3 * ctx->h_topol[3 * t + 1] + x] +
+                    ctx->h_vertex[3 * ctx->h_topol[3 * t + 2] + x]) / 3.0;
+    };
+    double lo[2][3], hi[2][3];
+    for (int s = 0; s < 2; ++s)
+        for (int x = 0; x < 3; ++x) {
+            lo[s][x] = 1e300;
+            hi[s][x] = -1e300;
+        }
+    for (int s = 0; s < 2; ++s)
+        for (int t : (s == 0 ? fc.tri_a : fc.tri_b)) {
+            double c[3];
+            centroid(t, c);
+            for (int x = 0; x < 3; ++x) {
+                lo[s][x] = std::min(lo[s][x], c[x]);
+                hi[s][x] = std::max(hi[s][x], c[x]);
+            }
+        }
+    double gap2 = 0;
+    for (int x = 0; x < 3; ++x) {
+        const double g = std::max({0.0, lo[1][x] - hi[0][x], lo[0][x] - hi[1][x]});
+        gap2 += g * g;
+    }
+    if (gap2 >= rmin * rmin) return FARCUR_OK;
+    std::vector<double> cb(3 * fc.tri_b.size());
+    for (size_t q = 0; q < fc.tri_b.size(); ++q) centroid(fc.tri_b[q], &cb[3 * q]);
+    for (int ta : fc.tri_a) {
+        double ca[3];
+        centroid(ta, ca);
+        for (size_t q = 0; q < fc.tri_b.size(); ++q) {
+            const double dx = ca[0] - cb[3 * q], dy = ca[1] - cb[3 * q + 1], dz = ca[2] - cb[3 * q + 2];
+            if (dx * dx + dy * dy + dz * dz < rmin * rmin)
+                return fail(FARCUR_EINVAL,
+                            "%s %d: triangles %d and %d are %.3g apart, closer than the integration radius %g: "
+                            "a near-field pair, which the far-field quadrature cannot integrate",
+                            what, block, ta, fc.tri_b[q], std::sqrt(dx * dx + dy * dy + dz * dz), rmin);
+        }
+    }
+    return FARCUR_OK;
+}
+
 // Panel element size.
 inline size_t psize(int f32) { return f32 ? sizeof(c64) : sizeof(c128); }
 
@@ -706,7 +809,7 @@ void farcur_opts_default(farcur_opts* o) {
     o->heldout = 8;
     o->oversample = 14;
     o->label_eps = 1e-4;
-    o->heldout_safety = 0.5;
+    o->heldout_safety = 0.25;
     o->growth = 1.5;
 }
 
@@ -803,6 +906,47 @@ int farcur_partition_tables(const farcur_partition* p, int* group_offsets, int* 
     if (diam) std::copy(P.group_diam.begin(), P.group_diam.end(), diam);
     if (far_pairs) std::copy(P.far_pairs.begin(), P.far_pairs.end(), far_pairs);
     if (near_pairs) std::copy(P.near_pairs.begin(), P.near_pairs.end(), near_pairs);
+    return FARCUR_OK;
+}
+
+int farcur_partition_blocks(const farcur_partition* p, int* a, int* b, int* comp, int* self) {
+    if (!p) return fail(FARCUR_EINVAL, "NULL partition");
+    const BlockPartition& P = p->p;
+    const size_t nf = P.far_pairs.size() / 2, nn = P.near_pairs.size() / 2;
+    for (size_t i = 0; i < nf + nn; ++i) {
+        const bool far = i < nf;
+        const int32_t ga = far ? P.far_pairs[2 * i] : P.near_pairs[2 * (i - nf)];
+        const int32_t gb = far ? P.far_pairs[2 * i + 1] : P.near_pairs[2 * (i - nf) + 1];
+        if (a) a[i] = ga;
+        if (b) b[i] = gb;
+        if (comp) comp[i] = far ? 1 : 0;
+        if (self) self[i] = ga == gb ? 1 : 0;
+    }
+    return FARCUR_OK;
+}
+
+int farcur_partition_save(const farcur_partition* p, const char* path) {
+    if (!p || !path) return fail(FARCUR_EINVAL, "NULL argument");
+    std::string err;
+    if (!save_partition(p->p, path, err)) return fail(FARCUR_EINVAL, "%s", err.c_str());
+    return FARCUR_OK;
+}
+
+int farcur_partition_load(const char* path, farcur_partition** out) {
+    if (!path || !out) return fail(FARCUR_EINVAL, "NULL argument");
+    *out = nullptr;
+    auto p = std::make_unique<farcur_partition>();
+    std::string err;
+    if (!load_partition(path, p->p, err)) return fail(FARCUR_EINVAL, "%s", err.c_str());
+    *out = p.release();
+    return FARCUR_OK;
+}
+
+int farcur_partition_params(const farcur_partition* p, double* leaf_size, double* eta_adm, int* nrwg_mesh) {
+    if (!p) return fail(FARCUR_EINVAL, "NULL partition");
+    if (leaf_size) *leaf_size = p->p.leaf_size;
+    if (eta_adm) *eta_adm = p->p.eta_adm;
+    if (nrwg_mesh) *nrwg_mesh = p->p.nrwg_mesh;
     return FARCUR_OK;
 }
 
@@ -1223,6 +1367,10 @@ extern "C" int farcur_compress_block(farcur_ctx* ctx, const int* rows, int m, co
     if (m <= 0 || n <= 0) return fail(FARCUR_EINVAL, "empty block");
     TRY(check_ids(rows, m, ctx->ne, "rows"));
     TRY(check_ids(cols, n, ctx->ne, "cols"));
+    {
+        FarCheck fc;
+        TRY(check_far_block(ctx, rows, m, cols, n, 0.0, fc, "block", opts->block_id));
+    }
     TRY(ctx->select());
     ctx->has_result = false;
     const int f32 = opts->precision == FARCUR_F32;
@@ -1335,7 +1483,7 @@ int to_ids(const double* src, int n, int ne, std::vector<int32_t>& out, const ch
 
 int compat_impl(farcur_c128* Cc, farcur_c128* Uc, farcur_c128* Rc, int r, double* vec_time, double pinv_tol,
                 farcur_c128* y_old, farcur_c128* y_new, farcur_c128* vtest, double* col_s, double* row_s,
-                int field, double k, double eta, double* r1, double* r2, double* vertex, double* topol,
+                int field, double k, double eta, double rnear, double* r1, double* r2, double* vertex, double* topol,
                 double* edges, double* ln, int n_r1, int n_r2, int rows_vertex, int cols_vertex, int rows_topol,
                 int cols_topol, int rows_edges, int cols_edges, int n_ln, double* err_out) {
     if (!Cc || !Uc || !Rc || !vec_time || !y_old || !y_new || !vtest || !col_s || !row_s || !r1 || !r2 || !vertex ||
@@ -1367,6 +1515,10 @@ int compat_impl(farcur_c128* Cc, farcur_c128* Uc, farcur_c128* Rc, int r, double
     TRY(to_ids(r2, n_r2, ctx->ne, cols, "r2"));
     TRY(to_ids(col_s, r, ctx->ne, cs, "col_samples"));
     TRY(to_ids(row_s, r, ctx->ne, rs, "row_samples"));
+    {
+        FarCheck fc;
+        TRY(check_far_block(ctx, rows.data(), n_r1, cols.data(), n_r2, rnear, fc, "block", 0));
+    }
     hipStream_t s = ctx->stream;
     const int m = n_r1, n = n_r2;
     TRY(ctx->d_rows.ensure(m * 4));
@@ -1442,7 +1594,7 @@ extern "C" double cuda_complete_compression(
     farcur_c128* Ccompressed, farcur_c128* Ucompressed, farcur_c128* Rcompressed, int cur_order, double* vec_time_cuda,
     double pinv_tol, farcur_c128* Avtest_old, farcur_c128* Avtest_new, farcur_c128* vtest, double* col_samples,
     double* row_samples, int, int, int, int, int, int, int, int, int, int, int, int, farcur_c128* /*Ztotal*/,
-    int field_arg, double k_arg, double eta_arg, double /*Rinteg_s*/, double /*Ranal_s*/, double /*Rinteg_f*/,
+    int field_arg, double k_arg, double eta_arg, double Rinteg_s_arg, double Ranal_s_arg, double Rinteg_f_arg,
     int /*cor_solid*/, int /*flag*/, double* r1_arg, double* r2_arg, double* vertex_arg, double* topol_arg,
     double* /*trian*/, double* edges_arg, double* /*un*/, double* /*ds*/, double* ln_arg, double* /*cent*/,
     int /*N_arg*/, int n_r1_arg, int n_r2_arg, int rows_vertex_arg, int cols_vertex_arg, int rows_topol_arg,
@@ -1450,8 +1602,11 @@ extern "C" double cuda_complete_compression(
     int /*rows_un*/, int /*cols_un*/, int /*rows_ds*/, int /*cols_ds*/, int n_ln_arg, int /*rows_cent*/,
     int /*cols_cent*/, int, int, int, int, int, int) {
     double err = NAN;
+    // the reference integrates triangle pairs closer than these radii with its singular rules
+    const double rnear = std::max({0.0, Rinteg_s_arg, Ranal_s_arg, Rinteg_f_arg});
     const int rc = compat_impl(Ccompressed, Ucompressed, Rcompressed, cur_order, vec_time_cuda, pinv_tol, Avtest_old,
-                               Avtest_new, vtest, col_samples, row_samples, field_arg, k_arg, eta_arg, r1_arg, r2_arg,
+                               Avtest_new, vtest, col_samples, row_samples, field_arg, k_arg, eta_arg, rnear, r1_arg,
+                               r2_arg,
                                vertex_arg, topol_arg, edges_arg, ln_arg, n_r1_arg, n_r2_arg, rows_vertex_arg,
                                cols_vertex_arg, rows_topol_arg, cols_topol_arg, rows_edges_arg, cols_edges_arg,
                                n_ln_arg, &err);
@@ -1633,6 +1788,9 @@ struct farcur_batch {
     bool rc_ready = false;
     // ACA of every block (k_aca): per-block regions of d_aca at aca_off
     DBuf d_aca, d_aca_jobs, d_aca_ks;  // d_aca_ks: k then status, nblocks each
+    // learned mode: blocks whose final sample took every row or column (stored exactly)
+    std::vector<ExactJob> exact_jobs;
+    DBuf d_exact;
     std::vector<size_t> aca_off;
     std::vector<int32_t> aca_kmax, aca_k, aca_status;
     bool aca_ready = false;
@@ -1727,7 +1885,9 @@ void advance_learned(BState& s, const JobOut& out, const RoundBufs& rb, double a
     s.rank_eps = out.rank_eps;
     s.sweeps = out.sweeps;
     s.smax = out.sigma_max;
-    s.err = s.h > 0 ? out.err : 0.0;
+    // acceptance on the ratio estimate of the whole block's error; the plain
+    // held-out rows' error is reported beside it
+    s.err = s.h > 0 ? out.err_est : 0.0;
     s.heldout_err = s.h > 0 ? out.err : -1.0;
     s.round += 1;
     if (s.h == 0 || s.err <= accept) {
@@ -2047,6 +2207,12 @@ extern "C" int farcur_batch_create(farcur_ctx* ctx, int nblocks, const int* row_
     if (!row_ids || !col_ids) return fail(FARCUR_EINVAL, "NULL ids");
     TRY(check_ids(row_ids, nr, ctx->ne, "row_ids"));
     TRY(check_ids(col_ids, nc, ctx->ne, "col_ids"));
+    {
+        FarCheck fc;
+        for (int b = 0; b < nblocks; ++b)
+            TRY(check_far_block(ctx, row_ids + row_off[b], row_off[b + 1] - row_off[b], col_ids + col_off[b],
+                                col_off[b + 1] - col_off[b], 0.0, fc, "block", b));
+    }
     TRY(ctx->select());
     auto B = std::make_unique<farcur_batch>();
     B->ctx = ctx;
@@ -2095,7 +2261,7 @@ extern "C" void farcur_batch_destroy(farcur_batch* B) {
     if (!B) return;
     hipSetDevice(B->ctx->device);
     hipStreamSynchronize(B->ctx->stream);
-    DBuf* bufs[] = {&B->d_rows, &B->d_cols, &B->d_rperm, &B->d_cperm, &B->d_v,    &B->d_ya,  &B->d_yb,
+    DBuf* bufs[] = {&B->d_exact, &B->d_rows, &B->d_cols, &B->d_rperm, &B->d_cperm, &B->d_v, &B->d_ya, &B->d_yb,
                     &B->d_vo,   &B->d_plan, &B->d_feat, &B->d_cap, &B->d_rpred, &B->d_rc,  &B->d_rc_ws,
                     &B->d_rc_jobs, &B->d_rc_k, &B->d_aca, &B->d_aca_jobs, &B->d_aca_ks};
     for (DBuf* b : bufs) b->release();
@@ -2338,6 +2504,8 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
         return fail(FARCUR_EINVAL, "bad mode %d", mode);
     if (mode == FARCUR_MODE_LEARNED && opts->row_mode != FARCUR_ROWS_UNIFORM)
         return fail(FARCUR_EINVAL, "learned mode samples rows uniformly (row_mode must be FARCUR_ROWS_UNIFORM)");
+    if (mode == FARCUR_MODE_LEARNED && opts->heldout < 1)
+        return fail(FARCUR_EINVAL, "learned mode needs heldout >= 1: the held-out rows are its acceptance check");
     farcur_ctx* ctx = B->ctx;
     TRY(ctx->select());
     const farcur_opts o = *opts;
@@ -2411,6 +2579,20 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
             first = false;
             active.swap(next);
         }
+        // blocks whose final sample took every row or every column: stored exactly
+        std::vector<ExactJob>& ex = B->exact_jobs;
+        ex.clear();
+        for (int b = 0; b < B->nb; ++b) {
+            const BState& s = B->st[b];
+            if (s.fin_r < std::min(s.m, s.n)) continue;
+            ex.push_back(ExactJob{s.fin.C, s.fin.U, s.fin.R, s.m, s.n, s.fin_r, s.fin_r == s.m ? 0 : 1});
+        }
+        if (!ex.empty()) {
+            TRY(B->d_exact.ensure(ex.size() * sizeof(ExactJob)));
+            HIPTRY(hipMemcpyAsync(B->d_exact.p, ex.data(), ex.size() * sizeof(ExactJob), hipMemcpyHostToDevice, st));
+            launch_exact(B->d_exact.as<ExactJob>(), (int)ex.size(), B->f32, st);
+            HIPTRY(hipGetLastError());
+        }
     } else if (mode == FARCUR_MODE_BATCHED) {
         std::vector<int> active(B->nb);
         for (int b = 0; b < B->nb; ++b) active[b] = b;
@@ -2444,6 +2626,164 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
     if (wall_ms) *wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     if (B->profile) TRY(batch_account(B));
     B->has_result = true;
+    return FARCUR_OK;
+}
+
+// Accuracy of every block of the last compress against its FP64 dense block,
+// on the device (verify.hip). Blocks go in chunks that bound the FP64
+// interpolation matrices P = C U to ~256 MiB at a time.
+extern "C" int farcur_batch_verify(farcur_batch* B, const farcur_c128* V, int nrhs, int symmetric, double* fro_err,
+                                   double* mv_err, farcur_c128* Yref, double* ms) {
+    if (!B || !V) return fail(FARCUR_EINVAL, "NULL argument");
+    if (!B->has_result) return fail(FARCUR_ESTATE, "no compress result to verify");
+    if (nrhs < 1 || nrhs > kVerifyMaxRhs) return fail(FARCUR_EINVAL, "nrhs = %d outside [1, %d]", nrhs, kVerifyMaxRhs);
+    farcur_ctx* ctx = B->ctx;
+    TRY(ctx->select());
+    hipStream_t st = ctx->stream;
+    const int N = ctx->ne, nb = B->nb;
+    const size_t Mtot = (size_t)B->row_off[nb], Ntot = (size_t)B->col_off[nb];
+    DBuf dV, dsums, dyerr, dyref, dys, dyst, dP, djobs, dint, dtiles;
+    TRY(dV.ensure((size_t)N * nrhs * sizeof(c128)));
+    TRY(dsums.ensure((size_t)2 * nb * sizeof(double)));
+    TRY(dyerr.ensure(Mtot * sizeof(c128)));
+    TRY(dyref.ensure(Mtot * sizeof(c128)));
+    if (Yref) {
+        TRY(dys.ensure(Mtot * nrhs * sizeof(c128)));
+        if (symmetric) TRY(dyst.ensure(Ntot * nrhs * sizeof(c128)));
+    }
+    HIPTRY(hipMemcpyAsync(dV.p, V, (size_t)N * nrhs * sizeof(c128), hipMemcpyHostToDevice, st));
+    HIPTRY(hipMemsetAsync(dsums.p, 0, (size_t)2 * nb * sizeof(double), st));
+    HIPTRY(hipEventRecord(ctx->ev0, st));
+    const FillParams fp = ctx->fill_params(0);
+    const size_t kChunkP = (size_t)256 << 20;
+    std::vector<VerifyInterp> ij;
+    std::vector<VerifyJob> vj;
+    std::vector<int32_t> tj, tr;
+    for (int b0 = 0; b0 < nb;) {
+        // chunk [b0, b1): P bytes within kChunkP (at least one block)
+        size_t pbytes = 0;
+        int b1 = b0;
+        while (b1 < nb) {
+            const size_t need = (size_t)B->st[b1].m * B->st[b1].fin_r * sizeof(c128);
+            if (b1 > b0 && pbytes + need > kChunkP) break;
+            pbytes += need;
+            ++b1;
+        }
+        TRY(dP.ensure(pbytes));
+        ij.clear();
+        vj.clear();
+        tj.clear();
+        tr.clear();
+        size_t poff = 0;
+        for (int b = b0; b < b1; ++b) {
+            const BState& S = B->st[b];
+            c128* P = dP.as<c128>() + poff;
+            poff += (size_t)S.m * S.fin_r;
+            ij.push_back(VerifyInterp{S.fin.C, S.fin.U, P, S.m, S.fin_r});
+            const int ro = B->row_off[b], co = B->col_off[b];
+            VerifyJob J{};
+            J.rows = B->d_rows.as<int32_t>() + ro;
+            J.cols = B->d_cols.as<int32_t>() + co;
+            J.m = S.m;
+            J.n = S.n;
+            J.r = S.fin_r;
+            J.ref_only = 0;
+            J.R = S.fin.R;
+            J.P = P;
+            J.sums = dsums.as<double>() + 2 * (size_t)b;
+            J.yerr = dyerr.as<c128>() + ro;
+            J.yref = dyref.as<c128>() + ro;
+            J.ys = Yref ? dys.as<c128>() + (size_t)ro * nrhs : nullptr;
+            for (int t = 0; t < S.m; t += 64) {
+                tj.push_back((int32_t)vj.size());
+                tr.push_back(t);
+            }
+            vj.push_back(J);
+            if (Yref && symmetric) {  // the transposed block Z_b^T = Z(cols, rows), product only
+                VerifyJob T{};
+                T.rows = J.cols;
+                T.cols = J.rows;
+                T.m = S.n;
+                T.n = S.m;
+                T.ref_only = 1;
+                T.ys = dyst.as<c128>() + (size_t)co * nrhs;
+                for (int t = 0; t < S.n; t += 64) {
+                    tj.push_back((int32_t)vj.size());
+                    tr.push_back(t);
+                }
+                vj.push_back(T);
+            }
+        }
+        TRY(dint.ensure(ij.size() * sizeof(VerifyInterp)));
+        TRY(djobs.ensure(vj.size() * sizeof(VerifyJob)));
+        TRY(dtiles.ensure(tj.size() * 2 * sizeof(int32_t)));
+        HIPTRY(hipMemcpyAsync(dint.p, ij.data(), ij.size() * sizeof(VerifyInterp), hipMemcpyHostToDevice, st));
+        HIPTRY(hipMemcpyAsync(djobs.p, vj.data(), vj.size() * sizeof(VerifyJob), hipMemcpyHostToDevice, st));
+        HIPTRY(hipMemcpyAsync(dtiles.p, tj.data(), tj.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        HIPTRY(hipMemcpyAsync(dtiles.as<int32_t>() + tj.size(), tr.data(), tr.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, st));
+        for (size_t j0 = 0; j0 < ij.size(); j0 += 65535)
+            launch_verify_interp(dint.as<VerifyInterp>() + j0, (int)std::min<size_t>(65535, ij.size() - j0), B->f32, st);
+        launch_verify(djobs.as<VerifyJob>(), (int)tj.size(), dtiles.as<int32_t>(), dtiles.as<int32_t>() + tj.size(),
+                      B->f32, dV.as<c128>(), N, nrhs, fp, st);
+        HIPTRY(hipGetLastError());
+        // the job and tile arrays are rewritten by the next chunk
+        HIPTRY(hipStreamSynchronize(st));
+        b0 = b1;
+    }
+    HIPTRY(hipEventRecord(ctx->ev1, st));
+    std::vector<double> sums(2 * (size_t)nb);
+    std::vector<c128> ye(Mtot), yr(Mtot);
+    HIPTRY(hipMemcpyAsync(sums.data(), dsums.p, sums.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPTRY(hipMemcpyAsync(ye.data(), dyerr.p, Mtot * sizeof(c128), hipMemcpyDeviceToHost, st));
+    HIPTRY(hipMemcpyAsync(yr.data(), dyref.p, Mtot * sizeof(c128), hipMemcpyDeviceToHost, st));
+    std::vector<c128> ys, yst;
+    if (Yref) {
+        ys.resize(Mtot * nrhs);
+        HIPTRY(hipMemcpyAsync(ys.data(), dys.p, ys.size() * sizeof(c128), hipMemcpyDeviceToHost, st));
+        if (symmetric) {
+            yst.resize(Ntot * nrhs);
+            HIPTRY(hipMemcpyAsync(yst.data(), dyst.p, yst.size() * sizeof(c128), hipMemcpyDeviceToHost, st));
+        }
+    }
+    HIPTRY(hipStreamSynchronize(st));
+    float t = 0;
+    hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+    if (ms) *ms = t;
+    for (int b = 0; b < nb; ++b) {
+        if (fro_err) fro_err[b] = sums[2 * b + 1] > 0 ? std::sqrt(sums[2 * b] / sums[2 * b + 1]) : 0.0;
+        if (mv_err) {
+            double ne = 0, nr = 0;
+            for (int i = B->row_off[b]; i < B->row_off[b + 1]; ++i) {
+                ne += ye[i].re * ye[i].re + ye[i].im * ye[i].im;
+                nr += yr[i].re * yr[i].re + yr[i].im * yr[i].im;
+            }
+            mv_err[b] = nr > 0 ? std::sqrt(ne / nr) : 0.0;
+        }
+    }
+    if (Yref) {
+        std::fill(Yref, Yref + (size_t)N * nrhs, farcur_c128{0, 0});
+        for (int b = 0; b < nb; ++b) {
+            const int ro = B->row_off[b], m = B->st[b].m;
+            for (int q = 0; q < nrhs; ++q)
+                for (int i = 0; i < m; ++i) {
+                    const c128 v = ys[(size_t)ro * nrhs + (size_t)q * m + i];
+                    farcur_c128& y = Yref[(size_t)q * N + B->row_ids[ro + i]];
+                    y.re += v.re;
+                    y.im += v.im;
+                }
+            if (symmetric) {
+                const int co = B->col_off[b], n = B->st[b].n;
+                for (int q = 0; q < nrhs; ++q)
+                    for (int j = 0; j < n; ++j) {
+                        const c128 v = yst[(size_t)co * nrhs + (size_t)q * n + j];
+                        farcur_c128& y = Yref[(size_t)q * N + B->col_ids[co + j]];
+                        y.re += v.re;
+                        y.im += v.im;
+                    }
+            }
+        }
+    }
     return FARCUR_OK;
 }
 
@@ -3163,6 +3503,388 @@ extern "C" int farcur_batch_matvec(farcur_batch* B, const farcur_c128* V, int s,
     }
     for (int j = 0; j < s; ++j)
         for (int i = 0; i < N; ++i) Y[(size_t)j * N + i] = farcur_c128{hy[(size_t)i * s + j].x, hy[(size_t)i * s + j].y};
+    return FARCUR_OK;
+}
+
+// Single right-hand side of the compressed far field, y = sum_b P_b R_b v[cols_b]
+// (+ transposes), one workgroup per (block, orientation): the HBM-bound CUR
+// matvec (each panel read once).
+extern "C" int farcur_batch_matvec1(farcur_batch* B, const farcur_c128* v, farcur_c128* y, int symmetric,
+                                    double* ms) {
+    if (!B || !v || !y) return fail(FARCUR_EINVAL, "NULL argument");
+    if (!B->has_result) return fail(FARCUR_ESTATE, "no batch result (compress first)");
+    farcur_ctx* ctx = B->ctx;
+    TRY(ctx->select());
+    hipStream_t st = ctx->stream;
+    const int N = ctx->ne;
+    double ms_interp = 0;
+    TRY(batch_interp(B, &ms_interp));
+    std::vector<Mv1Job> jobs;
+    jobs.reserve((size_t)B->nb * (symmetric ? 2 : 1));
+    int rmax = 1;
+    for (int b = 0; b < B->nb; ++b) {
+        const BState& S = B->st[b];
+        if (S.fin_r > 4096) return fail(FARCUR_EINVAL, "block %d: r = %d above 4096", b, S.fin_r);
+        rmax = std::max(rmax, S.fin_r);
+        const int32_t* rows = B->d_rows.as<int32_t>() + B->row_off[b];
+        const int32_t* cols = B->d_cols.as<int32_t>() + B->col_off[b];
+        jobs.push_back(Mv1Job{S.fin.R, B->interp[b], cols, rows, S.fin_r, S.n, S.m, 0});
+        if (symmetric) jobs.push_back(Mv1Job{B->interp[b], S.fin.R, rows, cols, S.fin_r, S.m, S.n, 0});
+    }
+    std::vector<c64> xv(N);
+    for (int i = 0; i < N; ++i) xv[i] = c64{(float)v[i].re, (float)v[i].im};
+    DBuf dj, dx, dy;
+    TRY(dj.ensure(jobs.size() * sizeof(Mv1Job)));
+    TRY(dx.ensure((size_t)N * sizeof(c64)));
+    TRY(dy.ensure((size_t)N * sizeof(c64)));
+    HIPTRY(hipMemcpyAsync(dj.p, jobs.data(), jobs.size() * sizeof(Mv1Job), hipMemcpyHostToDevice, st));
+    HIPTRY(hipMemcpyAsync(dx.p, xv.data(), (size_t)N * sizeof(c64), hipMemcpyHostToDevice, st));
+    HIPTRY(hipMemsetAsync(dy.p, 0, (size_t)N * sizeof(c64), st));
+    HIPTRY(hipEventRecord(ctx->ev0, st));
+    launch_matvec1(dj.as<Mv1Job>(), (int)jobs.size(), rmax, B->f32, dx.as<float2>(), dy.as<float2>(), st);
+    HIPTRY(hipEventRecord(ctx->ev1, st));
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(xv.data(), dy.p, (size_t)N * sizeof(c64), hipMemcpyDeviceToHost, st));
+    HIPTRY(hipStreamSynchronize(st));
+    float t = 0;
+    hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+    if (ms) *ms = t;
+    for (int i = 0; i < N; ++i) y[i] = farcur_c128{xv[i].re, xv[i].im};
+    return FARCUR_OK;
+}
+
+// ------------------------------------------------------------------ near field and operator
+struct farcur_near {
+    farcur_ctx* ctx = nullptr;
+    int nb = 0;
+    double near_factor = 2.0;
+    std::vector<int32_t> row_off, col_off, row_ids, col_ids, transpose;
+    DBuf d_rows, d_cols, d_Z, d_mv;
+    std::vector<size_t> z_off;  // element offset of block b in d_Z
+    int nmv = 0;                // entries of d_mv (block products incl. transposes)
+};
+
+namespace {
+
+// Fills the dense blocks (one wave per 64 rows of a block) on the context stream.
+int near_fill_blocks(farcur_ctx* ctx, const std::vector<NearJob>& jobs, double near_factor) {
+    std::vector<int32_t> tj, tr;
+    for (size_t j = 0; j < jobs.size(); ++j)
+        for (int t = 0; t < jobs[j].m; t += 64) {
+            tj.push_back((int32_t)j);
+            tr.push_back(t);
+        }
+    DBuf dj, dt;
+    TRY(dj.ensure(jobs.size() * sizeof(NearJob)));
+    TRY(dt.ensure(tj.size() * 2 * sizeof(int32_t)));
+    hipStream_t st = ctx->stream;
+    HIPTRY(hipMemcpyAsync(dj.p, jobs.data(), jobs.size() * sizeof(NearJob), hipMemcpyHostToDevice, st));
+    HIPTRY(hipMemcpyAsync(dt.p, tj.data(), tj.size() * 4, hipMemcpyHostToDevice, st));
+    HIPTRY(hipMemcpyAsync(dt.as<int32_t>() + tj.size(), tr.data(), tr.size() * 4, hipMemcpyHostToDevice, st));
+    launch_near_fill(dj.as<NearJob>(), (int)tj.size(), dt.as<int32_t>(), dt.as<int32_t>() + tj.size(),
+                     ctx->rec64.as<double>(), ctx->k, ctx->eta, near_factor, st);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipStreamSynchronize(st));  // dj, dt are freed on return
+    return FARCUR_OK;
+}
+
+}  // namespace
+
+extern "C" int farcur_fill_near(farcur_ctx* ctx, const int* rows, int nr, const int* cols, int nc, double near_factor,
+                                farcur_c128* Z) {
+    if (!ctx || !rows || !cols || !Z) return fail(FARCUR_EINVAL, "NULL argument");
+    if (ctx->ne == 0) return fail(FARCUR_ESTATE, "no geometry");
+    if (nr <= 0 || nc <= 0) return fail(FARCUR_EINVAL, "empty panel");
+    if (!(near_factor >= 0)) return fail(FARCUR_EINVAL, "near_factor must be >= 0");
+    TRY(check_ids(rows, nr, ctx->ne, "rows"));
+    TRY(check_ids(cols, nc, ctx->ne, "cols"));
+    TRY(ctx->select());
+    DBuf dr, dc, dz;
+    TRY(dr.ensure((size_t)nr * 4));
+    TRY(dc.ensure((size_t)nc * 4));
+    TRY(dz.ensure((size_t)nr * nc * sizeof(c128)));
+    HIPTRY(hipMemcpyAsync(dr.p, rows, (size_t)nr * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPTRY(hipMemcpyAsync(dc.p, cols, (size_t)nc * 4, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<NearJob> jobs{NearJob{dr.as<int32_t>(), dc.as<int32_t>(), dz.as<c128>(), nr, nc}};
+    TRY(near_fill_blocks(ctx, jobs, near_factor));
+    HIPTRY(hipMemcpy(Z, dz.p, (size_t)nr * nc * sizeof(c128), hipMemcpyDeviceToHost));
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_near_create(farcur_ctx* ctx, int nblocks, const int* row_off, const int* row_ids,
+                                  const int* col_off, const int* col_ids, const int* transpose, double near_factor,
+                                  farcur_near** out) {
+    if (!ctx || !row_off || !col_off || !row_ids || !col_ids || !out) return fail(FARCUR_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (nblocks <= 0) return fail(FARCUR_EINVAL, "nblocks must be > 0");
+    if (ctx->ne == 0) return fail(FARCUR_ESTATE, "no geometry");
+    if (!(near_factor >= 0)) return fail(FARCUR_EINVAL, "near_factor must be >= 0");
+    if (row_off[0] != 0 || col_off[0] != 0) return fail(FARCUR_EINVAL, "offsets must start at 0");
+    for (int b = 0; b < nblocks; ++b)
+        if (row_off[b + 1] <= row_off[b] || col_off[b + 1] <= col_off[b])
+            return fail(FARCUR_EINVAL, "block %d is empty or offsets decrease", b);
+    const int nr = row_off[nblocks], nc = col_off[nblocks];
+    TRY(check_ids(row_ids, nr, ctx->ne, "row_ids"));
+    TRY(check_ids(col_ids, nc, ctx->ne, "col_ids"));
+    TRY(ctx->select());
+    auto Nr = std::make_unique<farcur_near>();
+    Nr->ctx = ctx;
+    Nr->nb = nblocks;
+    Nr->near_factor = near_factor;
+    Nr->row_off.assign(row_off, row_off + nblocks + 1);
+    Nr->col_off.assign(col_off, col_off + nblocks + 1);
+    Nr->row_ids.assign(row_ids, row_ids + nr);
+    Nr->col_ids.assign(col_ids, col_ids + nc);
+    Nr->transpose.assign(nblocks, 0);
+    size_t total = 0;
+    Nr->z_off.resize(nblocks);
+    for (int b = 0; b < nblocks; ++b) {
+        if (transpose) Nr->transpose[b] = transpose[b] != 0;
+        Nr->z_off[b] = total;
+        total += (size_t)(row_off[b + 1] - row_off[b]) * (col_off[b + 1] - col_off[b]);
+    }
+    TRY(Nr->d_rows.ensure((size_t)nr * 4));
+    TRY(Nr->d_cols.ensure((size_t)nc * 4));
+    TRY(Nr->d_Z.ensure(total * sizeof(c128)));
+    HIPTRY(hipMemcpyAsync(Nr->d_rows.p, row_ids, (size_t)nr * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPTRY(hipMemcpyAsync(Nr->d_cols.p, col_ids, (size_t)nc * 4, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<NearJob> jobs(nblocks);
+    std::vector<NearMv> mv;
+    for (int b = 0; b < nblocks; ++b) {
+        jobs[b] = NearJob{Nr->d_rows.as<int32_t>() + row_off[b], Nr->d_cols.as<int32_t>() + col_off[b],
+                          Nr->d_Z.as<c128>() + Nr->z_off[b], row_off[b + 1] - row_off[b], col_off[b + 1] - col_off[b]};
+        mv.push_back(NearMv{jobs[b], 0, 0});
+        if (Nr->transpose[b]) mv.push_back(NearMv{jobs[b], 1, 0});
+    }
+    TRY(near_fill_blocks(ctx, jobs, near_factor));
+    TRY(Nr->d_mv.ensure(mv.size() * sizeof(NearMv)));
+    HIPTRY(hipMemcpy(Nr->d_mv.p, mv.data(), mv.size() * sizeof(NearMv), hipMemcpyHostToDevice));
+    Nr->nmv = (int)mv.size();
+    *out = Nr.release();
+    return FARCUR_OK;
+}
+
+extern "C" int farcur_near_block(farcur_near* Nr, int b, farcur_c128* Z) {
+    if (!Nr || !Z) return fail(FARCUR_EINVAL, "NULL argument");
+    if (b < 0 || b >= Nr->nb) return fail(FARCUR_EINVAL, "block %d out of range", b);
+    TRY(Nr->ctx->select());
+    const size_t cnt = (size_t)(Nr->row_off[b + 1] - Nr->row_off[b]) * (Nr->col_off[b + 1] - Nr->col_off[b]);
+    HIPTRY(hipMemcpy(Z, Nr->d_Z.as<c128>() + Nr->z_off[b], cnt * sizeof(c128), hipMemcpyDeviceToHost));
+    return FARCUR_OK;
+}
+
+extern "C" void farcur_near_destroy(farcur_near* Nr) {
+    if (!Nr) return;
+    hipSetDevice(Nr->ctx->device);
+    hipStreamSynchronize(Nr->ctx->stream);
+    Nr->d_rows.release();
+    Nr->d_cols.release();
+    Nr->d_Z.release();
+    Nr->d_mv.release();
+    delete Nr;
+}
+
+struct farcur_operator {
+    farcur_ctx* ctx = nullptr;
+    farcur_near* near = nullptr;
+    farcur_batch* far = nullptr;
+    int far_symmetric = 1;
+    int N = 0;
+    DBuf d_x, d_y, d_x32, d_y32, d_far;
+    int nfar = 0, far_rmax = 1;
+    std::vector<c128> h_y;
+    double ms_near = 0, ms_far = 0;
+    int matvecs = 0;
+};
+
+extern "C" int farcur_operator_create(farcur_near* near, farcur_batch* far, int far_symmetric, farcur_operator** out) {
+    if (!out || (!near && !far)) return fail(FARCUR_EINVAL, "need a near part, a far part or both");
+    *out = nullptr;
+    farcur_ctx* ctx = near ? near->ctx : far->ctx;
+    if (near && far && near->ctx != far->ctx) return fail(FARCUR_EINVAL, "near and far parts on different contexts");
+    if (far && !far->has_result) return fail(FARCUR_ESTATE, "far batch has no compress result");
+    TRY(ctx->select());
+    auto op = std::make_unique<farcur_operator>();
+    op->ctx = ctx;
+    op->near = near;
+    op->far = far;
+    op->far_symmetric = far_symmetric ? 1 : 0;
+    op->N = ctx->ne;
+    TRY(op->d_x.ensure((size_t)op->N * sizeof(c128)));
+    TRY(op->d_y.ensure((size_t)op->N * sizeof(c128)));
+    if (far) {
+        double ms_interp = 0;
+        TRY(batch_interp(far, &ms_interp));
+        std::vector<Mv1Job> jobs;
+        for (int b = 0; b < far->nb; ++b) {
+            const BState& S = far->st[b];
+            if (S.fin_r > 4096) return fail(FARCUR_EINVAL, "far block %d: r = %d above 4096", b, S.fin_r);
+            op->far_rmax = std::max(op->far_rmax, S.fin_r);
+            const int32_t* rows = far->d_rows.as<int32_t>() + far->row_off[b];
+            const int32_t* cols = far->d_cols.as<int32_t>() + far->col_off[b];
+            jobs.push_back(Mv1Job{S.fin.R, far->interp[b], cols, rows, S.fin_r, S.n, S.m, 0});
+            if (op->far_symmetric) jobs.push_back(Mv1Job{far->interp[b], S.fin.R, rows, cols, S.fin_r, S.m, S.n, 0});
+        }
+        TRY(op->d_far.ensure(jobs.size() * sizeof(Mv1Job)));
+        HIPTRY(hipMemcpy(op->d_far.p, jobs.data(), jobs.size() * sizeof(Mv1Job), hipMemcpyHostToDevice));
+        op->nfar = (int)jobs.size();
+        TRY(op->d_x32.ensure((size_t)op->N * sizeof(c64)));
+        TRY(op->d_y32.ensure((size_t)op->N * sizeof(c64)));
+    }
+    op->h_y.resize(op->N);
+    *out = op.release();
+    return FARCUR_OK;
+}
+
+extern "C" void farcur_operator_destroy(farcur_operator* op) {
+    if (!op) return;
+    hipSetDevice(op->ctx->device);
+    hipStreamSynchronize(op->ctx->stream);
+    DBuf* bufs[] = {&op->d_x, &op->d_y, &op->d_x32, &op->d_y32, &op->d_far};
+    for (DBuf* b : bufs) b->release();
+    delete op;
+}
+
+namespace {
+
+// y = (near + far) x on the device for one vector (host in, host out).
+int operator_apply(farcur_operator* op, const c128* x, c128* y) {
+    farcur_ctx* ctx = op->ctx;
+    hipStream_t st = ctx->stream;
+    const int N = op->N;
+    HIPTRY(hipMemcpyAsync(op->d_x.p, x, (size_t)N * sizeof(c128), hipMemcpyHostToDevice, st));
+    HIPTRY(hipMemsetAsync(op->d_y.p, 0, (size_t)N * sizeof(c128), st));
+    HIPTRY(hipEventRecord(ctx->ev0, st));
+    if (op->near) launch_near_matvec(op->near->d_mv.as<NearMv>(), op->near->nmv, op->d_x.as<c128>(), op->d_y.as<c128>(), st);
+    HIPTRY(hipEventRecord(ctx->ev1, st));
+    if (op->far) {
+        launch_to_c64(op->d_x.as<c128>(), op->d_x32.as<c64>(), N, st);
+        HIPTRY(hipMemsetAsync(op->d_y32.p, 0, (size_t)N * sizeof(c64), st));
+        launch_matvec1(op->d_far.as<Mv1Job>(), op->nfar, op->far_rmax, op->far->f32, op->d_x32.as<float2>(),
+                       op->d_y32.as<float2>(), st);
+        launch_add_c64(op->d_y.as<c128>(), op->d_y32.as<c64>(), N, st);
+    }
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(y, op->d_y.p, (size_t)N * sizeof(c128), hipMemcpyDeviceToHost, st));
+    HIPTRY(hipStreamSynchronize(st));
+    float t = 0;
+    hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+    op->ms_near += t;
+    op->matvecs += 1;
+    return FARCUR_OK;
+}
+
+}  // namespace
+
+extern "C" int farcur_operator_matvec(farcur_operator* op, const farcur_c128* x, farcur_c128* y) {
+    if (!op || !x || !y) return fail(FARCUR_EINVAL, "NULL argument");
+    farcur_ctx* ctx = op->ctx;
+    TRY(ctx->select());
+    return operator_apply(op, (const c128*)x, (c128*)y);
+}
+
+// Restarted GMRES (Saad and Schultz 1986) with modified Gram-Schmidt and
+// Givens rotations, zero initial guess, per right-hand side; vectors on the
+// host, every product on the device.
+extern "C" int farcur_operator_solve(farcur_operator* op, const farcur_c128* b, farcur_c128* x, int nrhs, double tol,
+                                     int restart, int maxit, int* iters, double* relres) {
+    if (!op || !b || !x) return fail(FARCUR_EINVAL, "NULL argument");
+    if (nrhs <= 0 || restart <= 0 || maxit <= 0 || !(tol > 0)) return fail(FARCUR_EINVAL, "bad solver parameters");
+    farcur_ctx* ctx = op->ctx;
+    TRY(ctx->select());
+    typedef std::complex<double> cd;
+    const int N = op->N;
+    std::vector<cd> Vb((size_t)N * (restart + 1)), w(N), H((size_t)(restart + 1) * restart), g(restart + 1),
+        cs(restart), sn(restart), xv(N);
+    for (int q = 0; q < nrhs; ++q) {
+        const cd* bq = (const cd*)(b + (size_t)q * N);
+        std::fill(xv.begin(), xv.end(), cd(0));
+        double bn = 0;
+        for (int i = 0; i < N; ++i) bn += std::norm(bq[i]);
+        bn = std::sqrt(bn);
+        int it = 0;
+        double res = bn > 0 ? 1.0 : 0.0;
+        while (bn > 0 && res > tol && it < maxit) {
+            // r = b - A x
+            TRY(operator_apply(op, (const c128*)xv.data(), (c128*)w.data()));
+            double rn = 0;
+            for (int i = 0; i < N; ++i) {
+                w[i] = bq[i] - w[i];
+                rn += std::norm(w[i]);
+            }
+            rn = std::sqrt(rn);
+            res = rn / bn;
+            if (res <= tol) break;
+            for (int i = 0; i < N; ++i) Vb[i] = w[i] / rn;
+            std::fill(g.begin(), g.end(), cd(0));
+            g[0] = rn;
+            int j = 0;
+            for (; j < restart && it < maxit; ++j, ++it) {
+                cd* vj = &Vb[(size_t)j * N];
+                TRY(operator_apply(op, (const c128*)vj, (c128*)w.data()));
+                for (int i = 0; i <= j; ++i) {  // modified Gram-Schmidt
+                    const cd* vi = &Vb[(size_t)i * N];
+                    cd h = 0;
+                    for (int t = 0; t < N; ++t) h += std::conj(vi[t]) * w[t];
+                    H[(size_t)j * (restart + 1) + i] = h;
+                    for (int t = 0; t < N; ++t) w[t] -= h * vi[t];
+                }
+                double wn = 0;
+                for (int t = 0; t < N; ++t) wn += std::norm(w[t]);
+                wn = std::sqrt(wn);
+                H[(size_t)j * (restart + 1) + j + 1] = wn;
+                cd* vn = &Vb[(size_t)(j + 1) * N];
+                for (int t = 0; t < N; ++t) vn[t] = wn > 0 ? w[t] / wn : cd(0);
+                cd* hj = &H[(size_t)j * (restart + 1)];
+                for (int i = 0; i < j; ++i) {  // previous rotations
+                    const cd t0 = cs[i] * hj[i] + sn[i] * hj[i + 1];
+                    hj[i + 1] = -std::conj(sn[i]) * hj[i] + cs[i] * hj[i + 1];
+                    hj[i] = t0;
+                }
+                const double a = std::abs(hj[j]), bb = std::abs(hj[j + 1]);
+                const double den = std::sqrt(a * a + bb * bb);
+                if (den == 0) {
+                    cs[j] = 1;
+                    sn[j] = 0;
+                } else {
+                    cs[j] = a / den;
+                    const cd ph = a > 0 ? hj[j] / a : cd(1);
+                    sn[j] = ph * std::conj(hj[j + 1]) / den;
+                }
+                hj[j] = cs[j] * hj[j] + sn[j] * hj[j + 1];
+                hj[j + 1] = 0;
+                g[j + 1] = -std::conj(sn[j]) * g[j];
+                g[j] = cs[j] * g[j];
+                res = std::abs(g[j + 1]) / bn;
+                if (res <= tol || wn == 0) {
+                    ++j;
+                    ++it;
+                    break;
+                }
+            }
+            // x += V y, H y = g (upper triangular j x j)
+            std::vector<cd> yv(j);
+            for (int i = j - 1; i >= 0; --i) {
+                cd s = g[i];
+                for (int l = i + 1; l < j; ++l) s -= H[(size_t)l * (restart + 1) + i] * yv[l];
+                yv[i] = s / H[(size_t)i * (restart + 1) + i];
+            }
+            for (int i = 0; i < j; ++i) {
+                const cd* vi = &Vb[(size_t)i * N];
+                for (int t = 0; t < N; ++t) xv[t] += yv[i] * vi[t];
+            }
+        }
+        // true residual of the returned solution
+        if (bn > 0) {
+            TRY(operator_apply(op, (const c128*)xv.data(), (c128*)w.data()));
+            double rn = 0;
+            for (int i = 0; i < N; ++i) rn += std::norm(bq[i] - w[i]);
+            res = std::sqrt(rn) / bn;
+        }
+        std::copy(xv.begin(), xv.end(), (cd*)(x + (size_t)q * N));
+        if (iters) iters[q] = it;
+        if (relres) relres[q] = res;
+    }
     return FARCUR_OK;
 }
 

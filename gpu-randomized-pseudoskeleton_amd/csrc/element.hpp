@@ -480,4 +480,9 @@ struct FillConsts {
     double scale;  // k eta / (144 pi)
 };
 
+// Fill constants for the wavenumber k and impedance eta.
+__host__ __device__ inline FillConsts fill_consts_f64(double k, double eta) {
+    return FillConsts{k, 4.0 / (k * k), k * eta / (144.0 * M_PI)};
+}
+
 }  // namespace farcur

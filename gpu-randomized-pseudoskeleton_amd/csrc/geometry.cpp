@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -300,6 +301,8 @@ bool build_rwg(Mesh& m, std::string& err) {
 BlockPartition build_partition(const Mesh& m, double leaf_size, double eta_adm) {
     BlockPartition P;
     P.leaf_size = leaf_size;
+    P.eta_adm = eta_adm;
+    P.nrwg_mesh = m.ne();
     const int ne = m.ne();
     std::vector<double> pt(3 * (size_t)ne);
     double bmin[3] = {1e300, 1e300, 1e300};
@@ -362,6 +365,155 @@ BlockPartition build_partition(const Mesh& m, double leaf_size, double eta_adm) 
         }
     }
     return P;
+}
+
+namespace {
+
+constexpr char kBlMagic[8] = {'F', 'A', 'R', 'C', 'U', 'R', 'B', 'L'};
+constexpr uint32_t kBlVersion = 1;
+
+struct BlEntry {
+    int32_t a, b;
+    uint8_t comp, self;
+    uint16_t pad;
+};
+static_assert(sizeof(BlEntry) == 12, "block-list entry layout");
+
+uint64_t fnv1a64(uint64_t h, const void* data, size_t n) {
+    const unsigned char* p = (const unsigned char*)data;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= 0x100000001B3ull;
+    }
+    return h;
+}
+
+}  // namespace
+
+bool save_partition(const BlockPartition& P, const std::string& path, std::string& err) {
+    std::vector<unsigned char> buf;
+    auto put = [&](const void* p, size_t n) {
+        const unsigned char* c = (const unsigned char*)p;
+        buf.insert(buf.end(), c, c + n);
+    };
+    const uint32_t ng = (uint32_t)P.group_diam.size(), nrwg = (uint32_t)P.group_rwg.size();
+    const uint32_t nfar = (uint32_t)(P.far_pairs.size() / 2), nnear = (uint32_t)(P.near_pairs.size() / 2);
+    const uint32_t hdr[6] = {kBlVersion, ng, (uint32_t)P.nrwg_mesh, nrwg, nfar, nnear};
+    put(kBlMagic, 8);
+    put(hdr, sizeof hdr);
+    put(&P.leaf_size, 8);
+    put(&P.eta_adm, 8);
+    put(P.group_offsets.data(), P.group_offsets.size() * 4);
+    put(P.group_rwg.data(), P.group_rwg.size() * 4);
+    put(P.group_centre.data(), P.group_centre.size() * 8);
+    put(P.group_diam.data(), P.group_diam.size() * 8);
+    for (uint32_t i = 0; i < nfar; ++i) {
+        const BlEntry e{P.far_pairs[2 * i], P.far_pairs[2 * i + 1], 1, 0, 0};
+        put(&e, sizeof e);
+    }
+    for (uint32_t i = 0; i < nnear; ++i) {
+        const int32_t a = P.near_pairs[2 * i], b = P.near_pairs[2 * i + 1];
+        const BlEntry e{a, b, 0, (uint8_t)(a == b ? 1 : 0), 0};
+        put(&e, sizeof e);
+    }
+    const uint64_t h = fnv1a64(0xcbf29ce484222325ull, buf.data(), buf.size());
+    put(&h, 8);
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) {
+        err = "cannot open " + path + " for writing";
+        return false;
+    }
+    const bool ok = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    if (std::fclose(f) != 0 || !ok) {
+        err = "write to " + path + " failed";
+        return false;
+    }
+    return true;
+}
+
+bool load_partition(const std::string& path, BlockPartition& P, std::string& err) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) {
+        err = "cannot open " + path;
+        return false;
+    }
+    std::vector<unsigned char> buf;
+    unsigned char tmp[1 << 16];
+    size_t n;
+    while ((n = std::fread(tmp, 1, sizeof tmp, f)) > 0) buf.insert(buf.end(), tmp, tmp + n);
+    std::fclose(f);
+    size_t at = 0;
+    auto get = [&](void* p, size_t cnt) {
+        if (at + cnt > buf.size()) return false;
+        std::memcpy(p, buf.data() + at, cnt);
+        at += cnt;
+        return true;
+    };
+    char magic[8];
+    uint32_t hdr[6];
+    BlockPartition Q;
+    if (!get(magic, 8) || std::memcmp(magic, kBlMagic, 8) != 0) {
+        err = path + ": not a block-list file";
+        return false;
+    }
+    if (!get(hdr, sizeof hdr) || hdr[0] != kBlVersion) {
+        err = path + ": unsupported block-list version";
+        return false;
+    }
+    const uint32_t ng = hdr[1], nrwg = hdr[3], nfar = hdr[4], nnear = hdr[5];
+    Q.nrwg_mesh = (int32_t)hdr[2];
+    // sizes must fit the file before anything is allocated
+    const size_t need = 16 + (size_t)(ng + 1) * 4 + (size_t)nrwg * 4 + (size_t)ng * 32 +
+                        (size_t)(nfar + nnear) * sizeof(BlEntry) + 8;
+    if (buf.size() < sizeof magic + sizeof hdr + need) {
+        err = path + ": truncated block-list file";
+        return false;
+    }
+    uint64_t h_file;
+    std::memcpy(&h_file, buf.data() + buf.size() - 8, 8);
+    if (fnv1a64(0xcbf29ce484222325ull, buf.data(), buf.size() - 8) != h_file) {
+        err = path + ": checksum mismatch (corrupt block-list file)";
+        return false;
+    }
+    Q.group_offsets.resize(ng + 1);
+    Q.group_rwg.resize(nrwg);
+    Q.group_centre.resize(3 * (size_t)ng);
+    Q.group_diam.resize(ng);
+    get(&Q.leaf_size, 8);
+    get(&Q.eta_adm, 8);
+    get(Q.group_offsets.data(), Q.group_offsets.size() * 4);
+    get(Q.group_rwg.data(), Q.group_rwg.size() * 4);
+    get(Q.group_centre.data(), Q.group_centre.size() * 8);
+    get(Q.group_diam.data(), Q.group_diam.size() * 8);
+    if (Q.group_offsets[0] != 0 || Q.group_offsets[ng] != (int32_t)nrwg) {
+        err = path + ": group offsets do not cover the RWG list";
+        return false;
+    }
+    for (uint32_t g = 0; g < ng; ++g)
+        if (Q.group_offsets[g + 1] < Q.group_offsets[g]) {
+            err = path + ": group offsets decrease";
+            return false;
+        }
+    for (int32_t e : Q.group_rwg)
+        if (e < 0 || e >= Q.nrwg_mesh) {
+            err = path + ": RWG id out of range";
+            return false;
+        }
+    for (uint32_t i = 0; i < nfar + nnear; ++i) {
+        BlEntry e;
+        get(&e, sizeof e);
+        const bool far = i < nfar;
+        if (e.a < 0 || e.b < 0 || e.a >= (int32_t)ng || e.b >= (int32_t)ng || e.comp != (far ? 1 : 0) ||
+            (far ? e.a >= e.b : e.a > e.b) || e.self != (e.a == e.b ? 1 : 0)) {
+            err = path + ": bad block entry";
+            return false;
+        }
+        auto& dst = far ? Q.far_pairs : Q.near_pairs;
+        dst.push_back(e.a);
+        dst.push_back(e.b);
+    }
+    P = std::move(Q);
+    return true;
 }
 
 }  // namespace farcur

@@ -55,6 +55,8 @@ bool build_rwg(Mesh& m, std::string& err);
 // Octree leaf groups and admissible (far-field) pairs.
 struct BlockPartition {
     double leaf_size = 0.5;
+    double eta_adm = 2.0;
+    int32_t nrwg_mesh = 0;               // RWGs of the mesh the partition was built on
     std::vector<int32_t> group_offsets;  // ngroups + 1
     std::vector<int32_t> group_rwg;      // RWG ids grouped by leaf
     std::vector<double> group_centre;    // 3 * ngroups
@@ -68,5 +70,22 @@ struct BlockPartition {
 // diameter = 2 * max distance from centre to any vertex of the group's
 // triangles. Pair (a, b) is admissible when |c_a - c_b| >= eta * max(d_a, d_b).
 BlockPartition build_partition(const Mesh& m, double leaf_size, double eta_adm);
+
+// Block-list file: the partition with its far (compressed) and near (dense)
+// block entries, the role of the reference's Matlab Z_comp list of
+// {m, n, comp, self} dicts (codes/GPU_implementation/seed/compres_single_block_spheres.jl:1134-1144,
+// self semantics at :2602-2612). Little-endian binary:
+//   char magic[8] = "FARCURBL", uint32 version = 1, uint32 ngroups,
+//   uint32 nrwg_mesh, uint32 nrwg (grouped), uint32 nfar, uint32 nnear,
+//   double leaf_size, double eta_adm,
+//   int32 group_offsets[ngroups + 1], int32 group_rwg[nrwg] (0-based RWG ids),
+//   double centre[3 * ngroups], double diam[ngroups],
+//   then nfar + nnear entries {int32 a, int32 b, uint8 comp, uint8 self, uint16 pad}:
+//   comp = 1 for a far pair (a < b), 0 for a near pair (a <= b); self = 1 when
+//   the entry stands for block (a, b) alone (a == b), 0 when it also stands for
+//   the transposed block (b, a) = (a, b)^T of the symmetric operator,
+//   uint64 checksum (FNV-1a of every byte before it).
+bool save_partition(const BlockPartition& P, const std::string& path, std::string& err);
+bool load_partition(const std::string& path, BlockPartition& P, std::string& err);
 
 }  // namespace farcur

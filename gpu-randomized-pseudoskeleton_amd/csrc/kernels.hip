@@ -179,9 +179,6 @@ int fill_tiles(int m, int n, int r, int h, int panels) {
 }
 
 // Fill constants for the wavenumber k and impedance eta.
-__device__ inline FillConsts fill_consts_f64(double k, double eta) {
-    return FillConsts{k, 4.0 / (k * k), k * eta / (144.0 * M_PI)};
-}
 __device__ inline F32Consts fill_consts_f32(double k, double eta) {
     const double il = k / (2.0 * M_PI);
     return F32Consts{(float)il, (float)(4.0 / (k * k)), (float)(k * eta / (144.0 * M_PI)), il};
@@ -310,8 +307,12 @@ void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32
 }
 
 // ---------------------------------------------------------------- pinv
+struct HeldErr {
+    double plain;  // ||H - C[hpos, :] U R||_F / ||H||_F
+    double est;    // ratio estimate of ||Z - C U R||_F / ||Z||_F over the whole block
+};
 template <typename PC>
-__device__ double heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, double* s_red);
+__device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, double* s_red);
 
 // Element idx (column-major, r x r) of the core W: the W panel, or the rows
 // rpos of C (the same entries, filled once).
@@ -393,9 +394,12 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
             // group buffers: the two halves of the core buffer (rmax^2 complex,
             // rmax >= 16, so each half holds >= 8 rows of r <= rmax)
             const int cap = rmax * rmax / 2;
-            const double e = f32 ? heldout_error<c64>(J, L.a, L.a + cap, cap, L.norm)
-                                 : heldout_error<c128>(J, L.a, L.a + cap, cap, L.norm);
-            if (threadIdx.x == 0) out[jb].err = e;
+            const HeldErr e = f32 ? heldout_error<c64>(J, L.a, L.a + cap, cap, L.norm)
+                                  : heldout_error<c128>(J, L.a, L.a + cap, cap, L.norm);
+            if (threadIdx.x == 0) {
+                out[jb].err = e.plain;
+                out[jb].err_est = e.est;
+            }
         }
         __syncthreads();
     }
@@ -511,20 +515,62 @@ void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double at
 // ---------------------------------------------------------------- residual
 constexpr int kResidMaxR = 512;
 
-// Relative Frobenius error of the CUR on the held-out rows:
-// ||H - C[hpos, :] U R||_F / ||H||_F. The rows go in groups of up to
-// kHeldGroup (as many as sT / sY, `cap` complex values each, hold), so that U
-// and R are read once per group rather than once per row:
-//   T = C[hpos, :] (g x r), Y = T U, z_j = Y R[:, j] against H[j, :].
+// Relative Frobenius error of the CUR on the held-out rows,
+// ||H - C[hpos, :] U R||_F / ||H||_F, and a ratio estimate of the whole
+// block's error. The rows go in groups of up to kHeldGroup (as many as
+// sT / sY, `cap` complex values each, hold), so that U and R are read once per
+// group rather than once per row: T = C[rows, :] (g x r), Y = T U (the rows'
+// interpolation coefficients P_i = C[i, :] U), z_j = Y R[:, j] against H[j, :].
+//
+// The estimate: a row's error |Z_i - P_i R| grows with |P_i| (the
+// interpolation from the sampled rows amplifies what they miss), so the
+// held-out rows' errors are scaled by the interpolation norms of every
+// unsampled row (a ratio estimator with auxiliary variable |P_i|^2):
+//   est^2 = (sum_h |E_h|^2 / sum_h |P_h|^2) sum_{i unsampled} |P_i|^2
+//           / (|R|_F^2 + (m - r) / h |H|_F^2).
+// Uniform held-out rows alone miss blocks whose error sits in a few rows;
+// those rows have the largest |P_i| and enter the estimate through it.
 constexpr int kHeldGroup = 8;
+
+// sum over `cnt` rows (positions pos(i)) of |C[pos, :] U|^2, rows in groups.
+template <typename PC, typename Pos>
+__device__ __forceinline__ double pnorm_sum(const BlockJob& J, int cnt, Pos pos, c128* sT, int G) {
+    const PC* C = (const PC*)J.C;
+    const int m = J.m, r = J.r, tid = threadIdx.x, nt = blockDim.x;
+    double acc = 0;
+    for (int i0 = 0; i0 < cnt; i0 += G) {
+        const int g = min(G, cnt - i0);
+        for (int idx = tid; idx < g * r; idx += nt) {
+            const int i = idx / r, l = idx - i * r;
+            sT[idx] = to_c128(C[(size_t)l * m + pos(i0 + i)]);
+        }
+        __syncthreads();
+        for (int idx = tid; idx < g * r; idx += nt) {
+            const int i = idx / r, k = idx - i * r;
+            const c128* uc = J.U + (size_t)k * r;
+            const c128* ti = sT + i * r;
+            double ar = 0, ai = 0;
+#pragma unroll 8
+            for (int l = 0; l < r; ++l) {
+                const c128 u = uc[l], t = ti[l];
+                ar += t.re * u.re - t.im * u.im;
+                ai += t.re * u.im + t.im * u.re;
+            }
+            acc += ar * ar + ai * ai;
+        }
+        __syncthreads();
+    }
+    return acc;
+}
+
 template <typename PC>
-__device__ double heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, double* s_red) {
+__device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, double* s_red) {
     const PC* C = (const PC*)J.C;
     const PC* R = (const PC*)J.R;
     const PC* H = (const PC*)J.H;
     const int m = J.m, n = J.n, r = J.r, h = J.h, tid = threadIdx.x, nt = blockDim.x;
     const int G = max(1, min(kHeldGroup, cap / r));
-    double num = 0, den = 0;
+    double num = 0, den = 0, ph = 0;
     for (int i0 = 0; i0 < h; i0 += G) {
         const int g = min(G, h - i0);
         for (int idx = tid; idx < g * r; idx += nt) {
@@ -546,6 +592,7 @@ __device__ double heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, 
                 ai += t.re * u.im + t.im * u.re;
             }
             sY[idx] = c128{ar, ai};
+            ph += ar * ar + ai * ai;
         }
         __syncthreads();
         for (int j = tid; j < n; j += nt) {  // z_i = sum_k Y[i][k] R[k, j]
@@ -576,9 +623,30 @@ __device__ double heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, 
         }
         __syncthreads();
     }
+    // interpolation norms of every row and of the sampled rows (their
+    // difference: the unsampled rows), and |R|_F^2 (the sampled rows' norms)
+    const double pall = pnorm_sum<PC>(J, m, [&](int i) { return i; }, sT, G);
+    const double psam = pnorm_sum<PC>(J, r, [&](int i) { return J.rpos[i]; }, sT, G);
+    double rr = 0;
+    for (int idx = tid; idx < r * n; idx += nt) {
+        const c128 b = to_c128(R[idx]);
+        rr += b.re * b.re + b.im * b.im;
+    }
     num = block_sum(num, s_red);
     den = block_sum(den, s_red);
-    return den > 0 ? sqrt(num / den) : (num > 0 ? INFINITY : 0.0);
+    ph = block_sum(ph, s_red);
+    const double pun = fmax(block_sum(pall, s_red) - block_sum(psam, s_red), 0.0);
+    rr = block_sum(rr, s_red);
+    HeldErr e;
+    e.plain = den > 0 ? sqrt(num / den) : (num > 0 ? INFINITY : 0.0);
+    const double ztot = rr + (double)(m - r) / h * den;
+    if (num == 0)
+        e.est = 0.0;
+    else if (ph > 0 && ztot > 0)
+        e.est = sqrt(num / ph * pun / ztot);
+    else
+        e.est = INFINITY;
+    return e;
 }
 
 // skip_heldout: the pinv launches computed the held-out error of the jobs with
@@ -595,9 +663,12 @@ __global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, int ski
         if (J.h > 0) {
             // Beyond kResidMaxR the group buffers use the W panel (r*r >= 2r),
             // which the pinv has already consumed.
-            const double e = J.r <= kResidMaxR ? heldout_error<PC>(J, s_t1, s_t2, kResidMaxR, s_red)
-                                               : heldout_error<PC>(J, J.W, J.W + J.r, J.r, s_red);
-            if (threadIdx.x == 0) out[jb].err = e;
+            const HeldErr e = J.r <= kResidMaxR ? heldout_error<PC>(J, s_t1, s_t2, kResidMaxR, s_red)
+                                                : heldout_error<PC>(J, J.W, J.W + J.r, J.r, s_red);
+            if (threadIdx.x == 0) {
+                out[jb].err = e.plain;
+                out[jb].err_est = e.est;
+            }
             __syncthreads();
             continue;
         }
@@ -734,6 +805,10 @@ __global__ __launch_bounds__(kFillThreads) void k_aca(const AcaJob* __restrict__
                 }
             }
         block_argmax(bv, Jk, s_v, s_i, bv, Jk);
+        if (Jk >= N) {  // no finite residual entry left in the row: stop, status 3
+            status = 3;
+            break;
+        }
         const c128 piv = J.row[Jk];
         if (piv.re == 0.0 && piv.im == 0.0) {  // R(Ik, Jk) == 0 (ACA.m:91)
             status = 1;
@@ -793,6 +868,10 @@ __global__ __launch_bounds__(kFillThreads) void k_aca(const AcaJob* __restrict__
         cross = block_sum(cross, s_red);
         normZ += 2.0 * cross + nu * nv;
         ++kk;
+        if (!(nu * nv < INFINITY) || !(normZ < INFINITY)) {  // NaN or Inf in the cross: status 3
+            status = 3;
+            break;
+        }
         if (kk > 1 && sqrt(nu * nv) <= tol * sqrt(normZ)) {  // ACA.m:114
             status = 1;
             break;
@@ -817,6 +896,10 @@ __global__ __launch_bounds__(kFillThreads) void k_aca(const AcaJob* __restrict__
                 }
             }
         block_argmax(bv, In, s_v, s_i, bv, In);
+        if (In >= M) {  // no finite entry left in the column: stop, status 3
+            status = 3;
+            break;
+        }
         __syncthreads();
         Ik = In;
         if (tid == 0) used_r[Ik] = 1;
@@ -831,6 +914,30 @@ __global__ __launch_bounds__(kFillThreads) void k_aca(const AcaJob* __restrict__
 void launch_aca(const AcaJob* jobs, int njobs, double tol, const FillParams& fp, hipStream_t s) {
     if (njobs <= 0) return;
     hipLaunchKernelGGL(k_aca, dim3(njobs), dim3(kFillThreads), 0, s, jobs, tol, (const double*)fp.rec, fp.k, fp.eta);
+}
+
+// ---------------------------------------------------------------- exact blocks
+template <typename PC>
+__global__ void k_exact(const ExactJob* __restrict__ jobs, int njobs) {
+    for (int b = blockIdx.x; b < njobs; b += gridDim.x) {
+        const ExactJob J = jobs[b];
+        const int r = J.r;
+        for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x)
+            J.U[idx] = c128{(idx % r) == (idx / r) ? 1.0 : 0.0, 0.0};
+        // side 0: C is m x r with r = m; side 1: R is r x n with r = n
+        PC* X = (PC*)(J.side == 0 ? J.C : J.R);
+        for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x)
+            store_c(&X[idx], (idx % r) == (idx / r) ? 1.0 : 0.0, 0.0);
+    }
+}
+
+void launch_exact(const ExactJob* jobs, int njobs, int f32, hipStream_t s) {
+    if (njobs <= 0) return;
+    const int grid = njobs < 4096 ? njobs : 4096;
+    if (f32)
+        hipLaunchKernelGGL(k_exact<c64>, dim3(grid), dim3(256), 0, s, jobs, njobs);
+    else
+        hipLaunchKernelGGL(k_exact<c128>, dim3(grid), dim3(256), 0, s, jobs, njobs);
 }
 
 // ---------------------------------------------------------------- test vectors

@@ -52,6 +52,8 @@ struct JobOut {
     int32_t rank_eps; // singular values >= label_eps * sigma_max (rank label)
     int32_t sweeps;   // Jacobi sweeps run
     int32_t pad;
+    double err_est;   // h > 0: estimated relative Frobenius error of the whole block
+                      // (ratio estimator over the interpolation norms |C[i, :] U|^2, heldout_error)
 };
 
 // One block of an ACA (k_aca): U (m x kmax, ld m), V rows as the columns of
@@ -137,6 +139,18 @@ void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double at
                  int rmax, int threads, int f32, int one_buf, int heldout, JobOut* out, hipStream_t s);
 void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, int skip_heldout, JobOut* out,
                      hipStream_t s);
+
+// A block whose sample took every row (r = m: R = Z[rows, :] exactly) or
+// every column (r = n: C = Z[:, cols] exactly) is stored exactly: C = I, U = I
+// (rows, side 0) or U = I, R = I (columns, side 1), so that C U R is the block
+// itself instead of a truncated projection of it.
+struct ExactJob {
+    void* C;  // m x r, panel precision
+    c128* U;  // r x r
+    void* R;  // r x n, panel precision
+    int32_t m, n, r, side;
+};
+void launch_exact(const ExactJob* jobs, int njobs, int f32, hipStream_t s);
 
 // Plain panel fill Z[i + j*ld] = Z(rows[i], cols[j]) (standalone, one panel).
 void launch_fill_panel(void* Z, int ld, const int32_t* rows, int nr, const int32_t* cols, int nc,

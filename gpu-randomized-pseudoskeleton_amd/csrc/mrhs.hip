@@ -385,7 +385,79 @@ __global__ __launch_bounds__(256) void k_mrhs_gemm(const MrhsProb* __restrict__ 
     }
 }
 
+// ---------------------------------------------------------------- one RHS
+// y[dst] += A2^T (A1 x[src]) per contribution, one workgroup each: the pass-A
+// vector t (r values) stays in LDS, so each panel is read exactly once and
+// nothing intermediate goes to HBM. t = A1 x: a thread owns one k and one
+// slice of the columns (consecutive threads read consecutive entries of a
+// column), the slices are summed in LDS. y: a thread owns one output row.
+template <typename PC>
+__global__ __launch_bounds__(256) void k_matvec1(const Mv1Job* __restrict__ jobs, const float2* __restrict__ x,
+                                                 float* __restrict__ y) {
+    extern __shared__ float2 lds1[];
+    const Mv1Job J = jobs[blockIdx.x];
+    const PC* A1 = (const PC*)J.a1;
+    const PC* A2 = (const PC*)J.a2;
+    const int r = J.r, K = J.K, M = J.M, tid = threadIdx.x;
+    float2* t = lds1;
+    float2* part = lds1 + r;
+    if (r <= 256) {
+        const int ns = 256 / r, k = tid % r, sl = tid / r;
+        float ar = 0, ai = 0;
+        if (sl < ns)
+            for (int j = sl; j < K; j += ns) {
+                const float2 a = ld_c(A1, (size_t)j * r + k), v = x[J.src[j]];
+                ar = fmaf(a.x, v.x, fmaf(-a.y, v.y, ar));
+                ai = fmaf(a.x, v.y, fmaf(a.y, v.x, ai));
+            }
+        part[tid] = make_float2(ar, ai);
+        __syncthreads();
+        if (tid < r) {
+            float sr = 0, si = 0;
+            for (int q = 0; q < ns; ++q) {
+                sr += part[q * r + tid].x;
+                si += part[q * r + tid].y;
+            }
+            t[tid] = make_float2(sr, si);
+        }
+    } else {
+        for (int k = tid; k < r; k += 256) {
+            float ar = 0, ai = 0;
+            for (int j = 0; j < K; ++j) {
+                const float2 a = ld_c(A1, (size_t)j * r + k), v = x[J.src[j]];
+                ar = fmaf(a.x, v.x, fmaf(-a.y, v.y, ar));
+                ai = fmaf(a.x, v.y, fmaf(a.y, v.x, ai));
+            }
+            t[k] = make_float2(ar, ai);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < M; i += 256) {
+        const PC* row = A2 + (size_t)i * r;
+        float ar = 0, ai = 0;
+        for (int k = 0; k < r; ++k) {
+            const float2 a = ld_c(row, k), v = t[k];
+            ar = fmaf(a.x, v.x, fmaf(-a.y, v.y, ar));
+            ai = fmaf(a.x, v.y, fmaf(a.y, v.x, ai));
+        }
+        const int d = J.dst[i];
+        atomicAdd(&y[2 * d], ar);
+        atomicAdd(&y[2 * d + 1], ai);
+    }
+}
+
 }  // namespace
+
+size_t matvec1_lds_bytes(int rmax) { return ((size_t)rmax + 256) * sizeof(float2); }
+
+void launch_matvec1(const Mv1Job* jobs, int njobs, int rmax, int f32, const float2* x, float2* y, hipStream_t st) {
+    if (njobs <= 0) return;
+    const size_t lds = matvec1_lds_bytes(rmax);
+    if (f32)
+        hipLaunchKernelGGL(k_matvec1<c64>, dim3(njobs), dim3(256), lds, st, jobs, x, (float*)y);
+    else
+        hipLaunchKernelGGL(k_matvec1<c128>, dim3(njobs), dim3(256), lds, st, jobs, x, (float*)y);
+}
 
 int mrhs_chunk() { return kNC; }
 

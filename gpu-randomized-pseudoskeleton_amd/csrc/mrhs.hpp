@@ -51,6 +51,20 @@ struct MrhsProb {
     int32_t m, k, seg_begin, seg_end;
 };
 
+// Single right-hand side (k_matvec1): y[dst] += A2^T (A1 x[src]) with A1 r x K
+// and A2 r x M column-major in the panel precision (o = 0: A1 = R_b, A2 = P_b^T,
+// src = cols_b, dst = rows_b; o = 1: A1 = P_b^T, A2 = R_b, src = rows_b,
+// dst = cols_b). y is accumulated with FP32 atomics (zero it first).
+struct Mv1Job {
+    const void* a1;
+    const void* a2;
+    const int32_t* src;
+    const int32_t* dst;
+    int32_t r, K, M, pad;
+};
+size_t matvec1_lds_bytes(int rmax);
+void launch_matvec1(const Mv1Job* jobs, int njobs, int rmax, int f32, const float2* x, float2* y, hipStream_t st);
+
 int mrhs_chunk();
 void launch_mrhs_interp(const MrhsInterp* jobs, int nb, int f32, hipStream_t st);
 // ntiles 64-row tiles (problem tile_prob[t], first row tile_row0[t]) times

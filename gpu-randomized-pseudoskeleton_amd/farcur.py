@@ -133,6 +133,10 @@ def _setup(L: ctypes.CDLL) -> None:
     L.farcur_partition_build.argtypes = [vp, cd, cd, vp]
     L.farcur_partition_sizes.argtypes = [vp, vp, vp, vp, vp]
     L.farcur_partition_tables.argtypes = [vp] * 7
+    L.farcur_partition_blocks.argtypes = [vp, vp, vp, vp, vp]
+    L.farcur_partition_save.argtypes = [vp, ctypes.c_char_p]
+    L.farcur_partition_load.argtypes = [ctypes.c_char_p, vp]
+    L.farcur_partition_params.argtypes = [vp, vp, vp, vp]
     L.farcur_partition_destroy.argtypes = [vp]
     L.farcur_partition_destroy.restype = None
     L.farcur_create.argtypes = [ci, vp]
@@ -179,6 +183,18 @@ def _setup(L: ctypes.CDLL) -> None:
         L.farcur_batch_set_graphs.argtypes = [vp, ci]
         L.farcur_batch_graph_stats.argtypes = [vp, vp, vp]
         L.farcur_batch_matvec.argtypes = [vp, vp, ci, vp, ci, ci, vp]
+        L.farcur_batch_verify.argtypes = [vp, vp, ci, ci, vp, vp, vp, vp]
+        L.farcur_batch_matvec1.argtypes = [vp, vp, vp, ci, vp]
+    L.farcur_fill_near.argtypes = [vp, vp, ci, vp, ci, cd, vp]
+    L.farcur_near_create.argtypes = [vp, ci, vp, vp, vp, vp, vp, cd, vp]
+    L.farcur_near_block.argtypes = [vp, ci, vp]
+    L.farcur_near_destroy.argtypes = [vp]
+    L.farcur_near_destroy.restype = None
+    L.farcur_operator_create.argtypes = [vp, vp, ci, vp]
+    L.farcur_operator_destroy.argtypes = [vp]
+    L.farcur_operator_destroy.restype = None
+    L.farcur_operator_matvec.argtypes = [vp, vp, vp]
+    L.farcur_operator_solve.argtypes = [vp, vp, vp, ci, cd, ci, ci, vp, vp]
     L.farcur_model_create.argtypes = [vp, cu64, vp]
     L.farcur_model_destroy.argtypes = [vp]
     L.farcur_model_destroy.restype = None
@@ -295,9 +311,41 @@ def two_octaspheres(radius, n_levels, d) -> Mesh:
 
 
 class Partition:
-    def __init__(self, mesh: Mesh, leaf_size=0.5, eta_adm=2.0):
-        self._h = vp()
-        check(lib().farcur_partition_build(mesh._h, leaf_size, eta_adm, ctypes.byref(self._h)))
+    """Octree leaf groups and the far/near block list (the reference's Z_comp)."""
+
+    def __init__(self, mesh: Mesh = None, leaf_size=0.5, eta_adm=2.0, _handle=None):
+        if _handle is not None:
+            self._h = _handle
+        else:
+            self._h = vp()
+            check(lib().farcur_partition_build(mesh._h, leaf_size, eta_adm, ctypes.byref(self._h)))
+        self._read()
+
+    @classmethod
+    def load(cls, path) -> "Partition":
+        """A block-list file written by save()."""
+        h = vp()
+        check(lib().farcur_partition_load(str(path).encode(), ctypes.byref(h)))
+        return cls(_handle=h)
+
+    def save(self, path) -> None:
+        check(lib().farcur_partition_save(self._h, str(path).encode()))
+
+    def params(self) -> dict:
+        leaf, eta, nr = cd(), cd(), ci()
+        check(lib().farcur_partition_params(self._h, ctypes.byref(leaf), ctypes.byref(eta), ctypes.byref(nr)))
+        return {"leaf_size": leaf.value, "eta_adm": eta.value, "nrwg_mesh": nr.value}
+
+    def zcomp(self) -> list:
+        """The block list as the reference's Z_comp dicts {m, n (1-based RWG ids),
+        comp, self} (far blocks first)."""
+        nb = len(self.far_pairs) + len(self.near_pairs)
+        a, b, comp, slf = (np.empty(nb, np.int32) for _ in range(4))
+        check(lib().farcur_partition_blocks(self._h, _p(a), _p(b), _p(comp), _p(slf)))
+        return [{"m": self.group(a[i]) + 1, "n": self.group(b[i]) + 1, "comp": int(comp[i]), "self": int(slf[i])}
+                for i in range(nb)]
+
+    def _read(self):
         ng, nr, nf, nn = ci(), ci(), ci(), ci()
         check(lib().farcur_partition_sizes(self._h, ctypes.byref(ng), ctypes.byref(nr), ctypes.byref(nf),
                                            ctypes.byref(nn)))
@@ -385,6 +433,14 @@ class Context:
         rows, cols = _ivec(rows), _ivec(cols)
         Z = np.empty((rows.size, cols.size), dtype=np.complex128, order="F")
         check(lib().farcur_fill(self._h, _p(rows), rows.size, _p(cols), cols.size, precision, _p(Z)))
+        return Z
+
+    def fill_near(self, rows, cols, near_factor=2.0) -> np.ndarray:
+        """Panel with the near-field element (singular extraction for close
+        triangle pairs), FP64."""
+        rows, cols = _ivec(rows), _ivec(cols)
+        Z = np.empty((rows.size, cols.size), dtype=np.complex128, order="F")
+        check(lib().farcur_fill_near(self._h, _p(rows), rows.size, _p(cols), cols.size, near_factor, _p(Z)))
         return Z
 
     def pinv(self, W, atol=1e-10, rtol=0.0):
@@ -590,6 +646,32 @@ class Batch:
         check(lib().farcur_batch_matvec(self._h, _p(V), V.shape[1], _p(Y), mode, 1 if symmetric else 0, _p(ms)))
         return Y, ms
 
+    def matvec1(self, v, symmetric=True):
+        """y (N) = compressed far field times one vector v (N), HBM-bound
+        single-RHS kernel; returns (y, device ms of the product kernel)."""
+        v = _cvec(v)
+        y = np.empty_like(v)
+        ms = cd()
+        check(lib().farcur_batch_matvec1(self._h, _p(v), _p(y), int(symmetric), ctypes.byref(ms)))
+        return y, ms.value
+
+    def verify(self, V, symmetric=True, want_product=True):
+        """Accuracy of every block of the last compress against its FP64 dense
+        block, on the device: (relative Frobenius error per block, matvec error
+        per block for v = V[:, 0], the FP64 dense far-field product Z V (N x s,
+        with the transposed blocks when symmetric) or None, device ms).
+        V: N x s with 1 <= s <= 16."""
+        V = np.asfortranarray(V, dtype=np.complex128)
+        if V.ndim == 1:
+            V = V[:, None]
+        fro = np.empty(self.nblocks)
+        mv = np.empty(self.nblocks)
+        Y = np.zeros(V.shape, dtype=np.complex128, order="F") if want_product else None
+        ms = cd()
+        check(lib().farcur_batch_verify(self._h, _p(V), V.shape[1], int(symmetric), _p(fro), _p(mv),
+                                        _p(Y) if Y is not None else None, ctypes.byref(ms)))
+        return fro, mv, Y, ms.value
+
     def misses(self) -> int:
         return lib().farcur_batch_misses(self._h)
 
@@ -688,6 +770,118 @@ class Model:
             raise FarcurError(f"{path}: unknown model architecture {d.get('arch')}")
         self.set(np.array(d["params"], np.float32), np.array(d["norm"], np.float32))
         return self
+
+
+class NearField:
+    """Dense near-field blocks on the device (the partition's non-admissible
+    pairs), filled with the near-field element."""
+
+    def __init__(self, ctx: Context, row_off, row_ids, col_off, col_ids, transpose=None, near_factor=2.0):
+        self.row_off, self.col_off = _ivec(row_off), _ivec(col_off)
+        row_ids, col_ids = _ivec(row_ids), _ivec(col_ids)
+        tr = None if transpose is None else _ivec(transpose)
+        self._h = vp()
+        self.ctx = ctx
+        check(lib().farcur_near_create(ctx._h, self.row_off.size - 1, _p(self.row_off), _p(row_ids),
+                                       _p(self.col_off), _p(col_ids), _p(tr) if tr is not None else None,
+                                       near_factor, ctypes.byref(self._h)))
+        ctx._children.add(self)
+
+    @classmethod
+    def from_partition(cls, ctx, part: "Partition", near_factor=2.0):
+        rows, cols, tr = [], [], []
+        for a, b in part.near_pairs:
+            rows.append(part.group(a))
+            cols.append(part.group(b))
+            tr.append(int(a != b))
+        ro = np.concatenate([[0], np.cumsum([r.size for r in rows])])
+        co = np.concatenate([[0], np.cumsum([c.size for c in cols])])
+        return cls(ctx, ro, np.concatenate(rows), co, np.concatenate(cols), tr, near_factor)
+
+    @property
+    def nblocks(self):
+        return self.row_off.size - 1
+
+    def block(self, b):
+        m = int(self.row_off[b + 1] - self.row_off[b])
+        n = int(self.col_off[b + 1] - self.col_off[b])
+        Z = np.empty((m, n), dtype=np.complex128, order="F")
+        check(lib().farcur_near_block(self._h, b, _p(Z)))
+        return Z
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().farcur_near_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+
+class Operator:
+    """Near dense blocks + far CUR blocks: products and GMRES solves on the device."""
+
+    def __init__(self, near: NearField = None, far: "Batch" = None, far_symmetric=True):
+        self._h = vp()
+        self._keep = (near, far)
+        check(lib().farcur_operator_create(near._h if near else None, far._h if far else None, int(far_symmetric),
+                                           ctypes.byref(self._h)))
+        (near or far).ctx._children.add(self)
+
+    def matvec(self, x):
+        x = _cvec(x)
+        y = np.empty_like(x)
+        check(lib().farcur_operator_matvec(self._h, _p(x), _p(y)))
+        return y
+
+    def solve(self, b, tol=1e-5, restart=100, maxit=2000):
+        """x with Z~ x = b (N or N x s), restarted GMRES; returns (x, iters, relres)."""
+        b = np.asfortranarray(b, dtype=np.complex128)
+        one = b.ndim == 1
+        B = b[:, None] if one else b
+        X = np.empty_like(B, order="F")
+        it = np.empty(B.shape[1], np.int32)
+        rr = np.empty(B.shape[1])
+        check(lib().farcur_operator_solve(self._h, _p(np.asfortranarray(B)), _p(X), B.shape[1], tol, restart, maxit,
+                                          _p(it), _p(rr)))
+        return (X[:, 0] if one else X), it, rr
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().farcur_operator_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+
+def rwg_far_field(mesh: "Mesh", J, k, rhat):
+    """N(rhat) = sum_n J_n int f_n(r') e^{jk rhat.r'} dS' (7-point rule per
+    triangle; rhat 3 x nd), and the bistatic radar cross section
+    k^2 eta^2 |N_perp|^2 / (4 pi) of the currents for a unit incident field."""
+    t = mesh.tables()
+    V, T, E, ln = t["vertex"], t["topol"].astype(int) - 1, t["edges"].astype(int) - 1, t["ln"]
+    rhat = np.asarray(rhat, float)
+    bary = [(1 / 3, 1 / 3, 1 / 3, 0.225)]  # Dunavant's 7-point rule
+    for a, b, w in ((0.059715871789770, 0.470142064105115, 0.132394152788506),
+                    (0.797426985353087, 0.101286507323456, 0.125939180544827)):
+        bary += [(a, b, b, w), (b, a, b, w), (b, b, a, w)]
+    N = np.zeros((3, rhat.shape[1]), complex)
+    for h, s in ((0, 1.0), (1, -1.0)):
+        P = V[:, T[:, E[h]]]
+        vf = V[:, E[2 + h]]
+        for a, b, c, w in bary:
+            q = a * P[:, 0] + b * P[:, 1] + c * P[:, 2]
+            N += ((q - vf) * (s * ln * np.asarray(J) / 2 * w)) @ np.exp(1j * k * (rhat.T @ q)).T
+    return N
+
+
+def bistatic_rcs(mesh: "Mesh", J, k, rhat, eta=120 * math.pi):
+    N = rwg_far_field(mesh, J, k, rhat)
+    Np = N - rhat * np.sum(N * rhat, axis=0)
+    return k ** 2 * eta ** 2 * np.sum(np.abs(Np) ** 2, axis=0) / (4 * math.pi)
 
 
 # ---------------------------------------------------------------- reference driver, restated

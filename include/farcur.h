@@ -118,6 +118,23 @@ int farcur_partition_sizes(const farcur_partition* p, int* ngroups, int* nrwg, i
  * diam ngroups, far_pairs 2 x nfar, near_pairs 2 x nnear (0-based group ids). */
 int farcur_partition_tables(const farcur_partition* p, int* group_offsets, int* group_rwg, double* centre,
                             double* diam, int* far_pairs, int* near_pairs);
+/* The block list as the reference's Z_comp entries {m, n, comp, self}
+ * (codes/GPU_implementation/seed/compres_single_block_spheres.jl:1134-1144,
+ * consumed at :2539-2696): nfar + nnear entries, far pairs first. Entry i is
+ * group pair (a[i], b[i]); comp[i] = 1 for a far (compressed) block, 0 for a
+ * near (dense) one; self[i] = 1 when the entry is the block (a, a) alone, 0
+ * when it also stands for the transposed block (b, a) = (a, b)^T (the
+ * reference fills transpose(Cs*Us*Rs) for self == 0, :2608-2612). Any
+ * pointer may be NULL. */
+int farcur_partition_blocks(const farcur_partition* p, int* a, int* b, int* comp, int* self);
+/* Block-list file (the partition and its entries; layout in
+ * gpu-randomized-pseudoskeleton_amd/csrc/geometry.hpp, with a checksum). load
+ * checks the magic, version, checksum, id ranges and the entry flags, and
+ * fails with FARCUR_EINVAL on any mismatch. */
+int farcur_partition_save(const farcur_partition* p, const char* path);
+int farcur_partition_load(const char* path, farcur_partition** out);
+/* Build parameters and the RWG count of the mesh it was built on. */
+int farcur_partition_params(const farcur_partition* p, double* leaf_size, double* eta_adm, int* nrwg_mesh);
 void farcur_partition_destroy(farcur_partition* p);
 
 /* ------------------------------------------------------------------ context
@@ -205,7 +222,9 @@ typedef struct {
                         m > 32 * heldout rows use ceil(m / 32), at most 4 * heldout) */
     int oversample;  /* p added to the predicted rank in FARCUR_MODE_LEARNED (default 14) */
     double label_eps;/* singular values >= label_eps * sigma_max are counted as rank_eps */
-    double heldout_safety; /* learned mode accepts a round when held-out error <= heldout_safety * tol (0.5) */
+    double heldout_safety; /* learned mode accepts a round when its estimated block error <= heldout_safety * tol
+                              (0.25); the estimate scales the held-out rows' error by the interpolation norms
+                              |C[i, :] U|^2 of every unsampled row (a ratio estimator) */
     double growth;   /* learned mode: sample count factor after a failed check (1.5; the plain loop doubles) */
 } farcur_opts;
 
@@ -364,6 +383,76 @@ int farcur_batch_graph_stats(farcur_batch* batch, int* captures, int* replays);
 enum { FARCUR_MV_VALU = 0, FARCUR_MV_MFMA_F32 = 1, FARCUR_MV_MFMA_BF16X3 = 2 };
 int farcur_batch_matvec(farcur_batch* batch, const farcur_c128* V, int nrhs, farcur_c128* Y, int mode, int symmetric,
                         double* ms);
+
+/* Single right-hand side: y = sum_b C_b U_b R_b v[cols_b] added to rows_b (and
+ * with symmetric = 1 the transposed blocks), the residual product of CU:1928-1966
+ * (three Zgemm with one column) over the whole far field at once. v and y are
+ * N complex128 (cast to complex64 on the device, y accumulated in FP32); the
+ * panels are read once, one workgroup per block and orientation, with the
+ * interpolation matrix P_b = C_b U_b of farcur_batch_matvec (formed on the first
+ * product call after a compress). *ms (may be NULL) receives the device time of
+ * the product kernel (P not included). Blocks with r above 4096 are rejected. */
+int farcur_batch_matvec1(farcur_batch* batch, const farcur_c128* v, farcur_c128* y, int symmetric, double* ms);
+
+/* Accuracy of every block of the last compress against its FP64 dense block,
+ * computed on the device (no counterpart kernel in the reference, whose check
+ * is norm(C*U*R - Z) / norm(Z) on the host against a dense block,
+ * codes/ACA_vs_CUR/seed/compres_single_block_spheres_for_lambda_sweep.jl:515-544).
+ * Z_b's entries come from the FP64 element; C U R is formed as (C U) R in
+ * FP64 from the stored panels. V: N x nrhs column-major (N = RWGs of the
+ * geometry, 1 <= nrhs <= 16). Outputs (any may be NULL except V):
+ *   fro_err[b] = ||Z_b - C_b U_b R_b||_F / ||Z_b||_F,
+ *   mv_err[b]  = ||(Z_b - C_b U_b R_b) v_b|| / ||Z_b v_b||, v_b = V[cols_b, 0],
+ *   Yref (N x nrhs) = sum_b Z_b V[cols_b, :] added to rows_b, and with
+ *   symmetric = 1 also Z_b^T V[rows_b, :] added to cols_b: the FP64 dense
+ *   far-field product farcur_batch_matvec approximates.
+ * *ms (may be NULL) receives the device time. */
+int farcur_batch_verify(farcur_batch* batch, const farcur_c128* V, int nrhs, int symmetric, double* fro_err,
+                        double* mv_err, farcur_c128* Yref, double* ms);
+
+/* ------------------------------------------------------------------ near field and operator
+ * The non-admissible (near) blocks of the partition are dense, as the
+ * reference's multi-block loop fills them (comp == 0 branch,
+ * codes/GPU_implementation/seed/compres_single_block_spheres.jl:2639-2662,
+ * self semantics :2602-2612). Their element integrates triangle pairs whose
+ * centroids are closer than near_factor x the longer longest edge by
+ * singularity extraction (closed-form 1/R potentials, Wilton et al. 1984;
+ * 7-point rules for the remainder and the test triangle), the role of the
+ * reference's Rinteg_s / Ranal_s singular rules
+ * (compres_single_block_spheres_for_sweep2.jl:1095-1098); other pairs keep the
+ * far element's 3 x 3 rule. FP64. */
+
+/* Panel Z[i + j*nr] with the near-field element (host output). */
+int farcur_fill_near(farcur_ctx* ctx, const int* rows, int nr, const int* cols, int nc, double near_factor,
+                     farcur_c128* Z);
+
+typedef struct farcur_near farcur_near;
+/* Fills every block (rows x cols lists as in farcur_batch_create) into device
+ * memory (complex128, column-major). transpose[b] = 1 (may be NULL: all 0)
+ * when the block also stands for its transpose at (cols, rows), the
+ * reference's self == 0. */
+int farcur_near_create(farcur_ctx* ctx, int nblocks, const int* row_off, const int* row_ids, const int* col_off,
+                       const int* col_ids, const int* transpose, double near_factor, farcur_near** out);
+int farcur_near_block(farcur_near* near, int b, farcur_c128* Z);
+void farcur_near_destroy(farcur_near* near);
+
+/* The compressed operator: near dense blocks plus the far CUR blocks of a
+ * compressed batch (far_symmetric = 1: each far block also applies its
+ * transpose, self == 0). Either part may be NULL. */
+typedef struct farcur_operator farcur_operator;
+int farcur_operator_create(farcur_near* near, farcur_batch* far, int far_symmetric, farcur_operator** out);
+void farcur_operator_destroy(farcur_operator* op);
+/* y = Z~ x (N complex128 each, host). The near part runs in FP64, the far
+ * part in the compressed panels' precision (FP32 accumulation). */
+int farcur_operator_matvec(farcur_operator* op, const farcur_c128* x, farcur_c128* y);
+/* Z~ x = b for nrhs right-hand sides (N x nrhs column-major) by restarted
+ * GMRES(restart) from x = 0, every product on the device: stops at a relative
+ * residual <= tol or after maxit products; iters and relres (nrhs each, may
+ * be NULL) receive the products used and the final true relative residual.
+ * The reference solves the same system directly, J = -Z \ Ei
+ * (compres_single_block_spheres.jl:3773). */
+int farcur_operator_solve(farcur_operator* op, const farcur_c128* b, farcur_c128* x, int nrhs, double tol,
+                          int restart, int maxit, int* iters, double* relres);
 
 /* Device time (ms) and launch count of each kernel class over the last
  * batch_compress call with profiling on: names "sample", "fill", "pinv",

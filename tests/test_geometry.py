@@ -125,3 +125,101 @@ def test_plane_waves_match_user_compute_Ei(farcur):
     for j in range(5):
         ref = excitation.plane_wave(t["vertex"], t["edges"], t["ln"], t["cent"], 2 * np.pi, th[j], ph[j])
         np.testing.assert_allclose(V[:, j], ref, rtol=1e-12, atol=1e-14)
+
+
+def _brute_force_groups(m, leaf):
+    """Leaf groups, centres and diameters recomputed from the mesh tables in
+    numpy: RWG point = midpoint of the shared edge, cells of side `leaf`
+    anchored at the bounding-box minimum, centre = mean RWG point, diameter =
+    2 x the largest centre-to-vertex distance over the group's triangles."""
+    t = m.tables()
+    V = t["vertex"].T
+    T = t["topol"].T.astype(int) - 1
+    E = t["edges"].T.astype(int) - 1
+    tp, vp = E[:, 0], E[:, 2]
+    pts = np.empty((E.shape[0], 3))
+    for e in range(E.shape[0]):
+        ab = [v for v in T[tp[e]] if v != vp[e]]
+        pts[e] = 0.5 * (V[ab[0]] + V[ab[1]])
+    cell = np.floor((pts - pts.min(axis=0)) / leaf).astype(np.int64)
+    return pts, cell, V, T, E
+
+
+def test_partition_brute_force_all_pairs(farcur):
+    # Every group against the mesh, and every unordered pair of groups (not a
+    # sample) against the admissibility rule |c_a - c_b| >= eta max(d_a, d_b).
+    m = farcur.Mesh.icosphere(2.0, 1.5, 1.0, 4, 0.01, 3).build_rwg()
+    leaf, eta = 0.5, 2.0
+    p = farcur.Partition(m, leaf, eta)
+    pts, cell, V, T, E = _brute_force_groups(m, leaf)
+    ng = p.ngroups
+    for g in range(ng):
+        ids = p.group(g)
+        assert len({tuple(c) for c in cell[ids]}) == 1  # one leaf cell per group
+        c = pts[ids].mean(axis=0)
+        np.testing.assert_allclose(p.centre[g], c, rtol=0, atol=1e-12)
+        verts = T[np.concatenate([E[ids, 0], E[ids, 1]])].ravel()
+        np.testing.assert_allclose(p.diam[g], 2 * np.max(np.linalg.norm(V[verts] - c, axis=1)), rtol=1e-12)
+    # distinct groups have distinct cells, and the groups cover every RWG once
+    assert len({tuple(cell[p.group(g)[0]]) for g in range(ng)}) == ng
+    assert np.array_equal(np.sort(p.group_rwg), np.arange(m.sizes()[2]))
+    d = np.linalg.norm(p.centre[:, None, :] - p.centre[None, :, :], axis=2)
+    dm = np.maximum(p.diam[:, None], p.diam[None, :])
+    far = {(int(a), int(b)) for a, b in p.far_pairs}
+    near = {(int(a), int(b)) for a, b in p.near_pairs}
+    assert len(far) == len(p.far_pairs) and len(near) == len(p.near_pairs) and not far & near
+    for a in range(ng):
+        for b in range(a, ng):
+            admissible = a != b and d[a, b] >= eta * dm[a, b]
+            assert ((a, b) in far) == admissible and ((a, b) in near) == (not admissible), (a, b)
+
+
+def test_block_list_file_round_trip(farcur, tmp_path):
+    m = farcur.Mesh.icosphere(1.0, 1.0, 1.0, 3, 0.01, 1).build_rwg()
+    p = farcur.Partition(m, 0.5, 2.0)
+    path = tmp_path / "blocks.fcbl"
+    p.save(path)
+    q = farcur.Partition.load(path)
+    assert q.params() == {"leaf_size": 0.5, "eta_adm": 2.0, "nrwg_mesh": m.sizes()[2]}
+    for k in ("group_offsets", "group_rwg", "centre", "diam", "far_pairs", "near_pairs"):
+        np.testing.assert_array_equal(getattr(q, k), getattr(p, k))
+    for x, y in zip(q.far_blocks(), p.far_blocks()):
+        np.testing.assert_array_equal(x, y)
+    # the reference's Z_comp view: far blocks compressed (comp = 1) with their
+    # transposes implied (self = 0); near blocks dense; diagonal blocks self = 1
+    zc = q.zcomp()
+    assert len(zc) == len(p.far_pairs) + len(p.near_pairs)
+    nf = len(p.far_pairs)
+    assert all(e["comp"] == 1 and e["self"] == 0 for e in zc[:nf])
+    assert all(e["comp"] == 0 for e in zc[nf:])
+    assert sum(e["self"] for e in zc) == p.ngroups
+    a, b = p.far_pairs[0]
+    np.testing.assert_array_equal(zc[0]["m"], p.group(a) + 1)
+    np.testing.assert_array_equal(zc[0]["n"], p.group(b) + 1)
+    # every entry once: far + near blocks and their transposes tile the matrix
+    N = m.sizes()[2]
+    cover = np.zeros((N, N), np.int8)
+    for e in zc:
+        cover[np.ix_(e["m"] - 1, e["n"] - 1)] += 1
+        if e["self"] == 0:
+            cover[np.ix_(e["n"] - 1, e["m"] - 1)] += 1
+    assert cover.min() == 1 and cover.max() == 1
+
+
+def test_block_list_file_rejects_damage(farcur, tmp_path):
+    m = farcur.Mesh.icosphere(1.0, 1.0, 1.0, 2, 0.01, 1).build_rwg()
+    p = farcur.Partition(m, 0.5, 2.0)
+    path = tmp_path / "blocks.fcbl"
+    p.save(path)
+    raw = bytearray(path.read_bytes())
+    bad = tmp_path / "bad.fcbl"
+    for mutate, what in ((lambda b: b.__setitem__(40, b[40] ^ 1), "checksum"),
+                         (lambda b: b.__setitem__(0, ord("X")), "not a block-list"),
+                         (lambda b: b.__delitem__(slice(len(b) - 20, len(b))), "")):
+        x = bytearray(raw)
+        mutate(x)
+        bad.write_bytes(bytes(x))
+        with pytest.raises(farcur.FarcurError, match=what):
+            farcur.Partition.load(bad)
+    with pytest.raises(farcur.FarcurError, match="cannot open"):
+        farcur.Partition.load(tmp_path / "missing.fcbl")

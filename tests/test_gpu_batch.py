@@ -71,7 +71,7 @@ def test_batch_matches_oracle_and_dense(farcur, sphere, oracle):
         Z = g.fill(rows, cols)
         assert abs(_relfro(A, Z) - _relfro(Ao, Z)) <= 1e-6
         worst = max(worst, _relfro(A, Z))
-    assert worst <= 1e-3
+    assert worst <= 1e-4
 
 
 def test_batch_f32_accuracy(farcur, sphere):
@@ -83,7 +83,7 @@ def test_batch_f32_accuracy(farcur, sphere):
         rows, cols = ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]]
         out = bt.block(b)
         errs.append(_relfro(out["C"] @ out["U"] @ out["R"], g.fill(rows, cols)))
-    assert max(errs) <= 1e-3
+    assert max(errs) <= 1e-4
 
 
 def test_batch_profile_accounting(farcur, sphere):
@@ -112,11 +112,45 @@ def test_batch_every_block_accurate(farcur, sphere, mode, precision):
         assert np.all(info["converged"] == 1)
     # (the reference loop ends blocks that reach r = min(m, n) as "stopped at
     # the cap"; their CUR is then exact up to the pinv threshold)
+    V = np.random.default_rng(4).standard_normal((1920, 2)) + 0j
+    fro, mv, _, _ = bt.verify(V, want_product=False)
     for b in range(bt.nblocks):
         rows, cols = ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]]
         out = bt.block(b)
-        e = _relfro(out["C"] @ out["U"] @ out["R"], g.fill(rows, cols))
-        assert e <= 1e-3, (b, e, info["r"][b], info["kept"][b])
+        Z = g.fill(rows, cols)
+        D = out["C"] @ out["U"] @ out["R"] - Z
+        e = np.linalg.norm(D) / np.linalg.norm(Z)
+        assert e <= 1e-4, (b, e, info["r"][b], info["kept"][b])
+        # the device check agrees with this host computation: (C U) R on the
+        # device and C (U R) here differ by the rounding of forming the
+        # product, ~eps |C| |U| |R| (|U| reaches ~1e8 at the absolute 1e-10
+        # pinv threshold)
+        fwd = 64 * np.finfo(float).eps * np.prod([np.linalg.norm(out[x], 2) for x in ("C", "U", "R")])
+        assert abs(fro[b] - e) <= 1e-6 * e + fwd / np.linalg.norm(Z), (b, fro[b], e)
+        v = V[cols, 0]
+        emv = np.linalg.norm(D @ v) / np.linalg.norm(Z @ v)
+        assert abs(mv[b] - emv) <= 1e-6 * emv + fwd * np.linalg.norm(v) / np.linalg.norm(Z @ v), (b, mv[b], emv)
+    assert fro.max() <= 1e-4
+
+
+def test_verify_dense_product_matches_oracle(farcur, sphere):
+    # The device's FP64 dense far-field product (both orientations of every
+    # block) against the oracle's dense blocks times the same vectors.
+    ctx, g, part, (ro, ri, co, cix) = sphere
+    bt = farcur.Batch(ctx, ro, ri, co, cix)
+    bt.compress(farcur.MODE_BATCHED, tol=1e-4, seed=2, precision=farcur.F32, pinv_rtol=1e-6)
+    rng = np.random.default_rng(8)
+    V = rng.standard_normal((1920, 5)) + 1j * rng.standard_normal((1920, 5))
+    for symmetric in (False, True):
+        _, _, Y, _ = bt.verify(V, symmetric=symmetric)
+        Yo = np.zeros_like(V)
+        for b in range(bt.nblocks):
+            rows, cols = ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]]
+            Z = g.fill(rows, cols)
+            Yo[rows] += Z @ V[cols]
+            if symmetric:
+                Yo[cols] += Z.T @ V[rows]
+        assert np.linalg.norm(Y - Yo) <= 1e-10 * np.linalg.norm(Yo)
 
 
 @pytest.mark.parametrize("mode", [1, 2])

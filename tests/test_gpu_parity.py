@@ -253,3 +253,49 @@ def test_drop_in_rejects_bad_input(farcur, plates):
         farcur.pure_cuda_compression2(np.ones(3), np.ones(10), 2, 1e-10, r1, r2, t, farcur.EMData(), rng)
     with pytest.raises(farcur.FarcurError):
         ctx.fill([0, 5000], [1])
+
+
+def test_near_field_blocks_are_refused(farcur, plates):
+    # The 3-point far-field quadrature cannot integrate touching or coincident
+    # triangle pairs (the reference switches those to singular integration):
+    # every entry point that compresses a block refuses one whose row and
+    # column triangles share a vertex, and the drop-in also refuses centroid
+    # pairs closer than its Rinteg_s / Ranal_s / Rinteg_f radii.
+    ctx, g, t = plates
+    same_plate = (np.arange(0, 40, dtype=np.int32), np.arange(20, 80, dtype=np.int32))  # shares RWGs
+    with pytest.raises(farcur.FarcurError, match="near-field"):
+        ctx.compress_block(*same_plate, tol=1e-4, seed=1)
+    far = (np.arange(0, 40, dtype=np.int32), np.arange(1160, 1200, dtype=np.int32))
+    ro = np.array([0, 40, 80], np.int32)
+    ri = np.concatenate([far[0], same_plate[0]])
+    co = np.array([0, 40, 100], np.int32)
+    ci = np.concatenate([far[1], same_plate[1]])
+    with pytest.raises(farcur.FarcurError, match="block 1"):
+        farcur.Batch(ctx, ro, ri, co, ci)
+    # neighbouring RWGs that only share a vertex or an edge (no common RWG)
+    rows = np.array([0], np.int32)
+    tri = t["edges"][:2].T.astype(int)  # 1-based T+, T- per RWG
+    verts0 = set(t["topol"][:, tri[0] - 1].ravel())
+    touch = [j for j in range(1, 1160) if not set(tri[j]) & set(tri[0]) and set(t["topol"][:, tri[j] - 1].ravel()) & verts0]
+    assert touch
+    with pytest.raises(farcur.FarcurError, match="near-field"):
+        ctx.compress_block(rows, np.array(touch, np.int32), tol=1e-4, seed=1)
+    # drop-in: the plates are 4 apart, so a 5 m integration radius makes the block near
+    rng = np.random.default_rng(4)
+    r1, r2 = np.arange(1, 1161), np.arange(1161, 2321)
+    em = farcur.EMData(Rint_s=5.0, Ranal_s=5.0, Rint_f=5.0)
+    with pytest.raises(farcur.FarcurError, match="integration radius"):
+        farcur.pure_cuda_compression2(rng.random(1160) + 0j, np.ones(1160), 20, 1e-10, r1, r2, t, em, rng)
+    em = farcur.EMData(Rint_s=1.0, Ranal_s=1.0, Rint_f=1.0)  # the reference's settings (SW2:1092-1106)
+    C, U, R, *_ = farcur.pure_cuda_compression2(rng.random(1160) + 0j, np.ones(1160), 20, 1e-10, r1, r2, t, em, rng)
+    assert np.all(np.isfinite(C))
+
+
+def test_learned_mode_needs_heldout_rows(farcur, plates):
+    ctx, g, t = plates
+    ro = np.array([0, 40], np.int32)
+    co = np.array([0, 40], np.int32)
+    bt = farcur.Batch(ctx, ro, np.arange(40, dtype=np.int32), co, np.arange(1160, 1200, dtype=np.int32))
+    with pytest.raises(farcur.FarcurError, match="heldout"):
+        bt.compress(farcur.MODE_LEARNED, tol=1e-4, seed=1, heldout=0)
+    bt.close()

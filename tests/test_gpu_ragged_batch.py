@@ -89,4 +89,4 @@ def test_ragged_f32_every_block_accurate(farcur, ragged, mode):
         assert out["C"].shape == (m, info["r"][b]) and out["R"].shape == (info["r"][b], n)
         assert 1 <= info["r"][b] <= min(m, n)
         e = _relfro(out["C"] @ out["U"] @ out["R"], dense[b])
-        assert e <= 1e-3, (mode, m, n, e, info["r"][b])
+        assert e <= 1e-4, (mode, m, n, e, info["r"][b])

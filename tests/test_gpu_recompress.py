@@ -95,3 +95,30 @@ def test_recompress_needs_a_result(farcur, sphere):
     bt.compress(farcur.MODE_BATCHED, tol=1e-4, seed=2)
     with pytest.raises(farcur.FarcurError):
         bt.recompressed_block(0)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_recompress_cores_above_64(farcur, oracle, precision):
+    # Cores with r > 64 take the recompression's second launch (streamed
+    # Jacobi, register-free): a 400 x 400 plate block started at r = 100 ends
+    # its reference loop at r = 200.
+    mesh = farcur.plates_config(1.0, 20, 4.0)
+    ctx = farcur.Context(0)
+    ctx.set_mesh(mesh)
+    rng = np.random.default_rng(11)
+    rows = np.sort(rng.choice(1160, 400, replace=False)).astype(np.int32)
+    cols = (1160 + np.sort(rng.choice(1160, 400, replace=False))).astype(np.int32)
+    ro = np.array([0, 400], np.int32)
+    bt = farcur.Batch(ctx, ro, rows, ro.copy(), cols)
+    opts = dict(tol=1e-4, seed=5, precision=precision, r_start=100)
+    if precision == farcur.F32:
+        opts["pinv_rtol"] = 1e-6
+    bt.compress(farcur.MODE_BATCHED, **opts)
+    info = bt.info_arrays()
+    assert info["r"][0] > 64
+    for thresh in (1e-5, 1e-8):
+        ranks, _ = bt.recompress(thresh)
+        prod_tol, orth_tol = (1e-9, 1e-10) if precision == farcur.F64 else (2e-6, 2e-6)
+        _check_block(oracle, bt.block(0), bt.recompressed_block(0), thresh, prod_tol, orth_tol)
+    bt.close()
+    ctx.close()
