@@ -214,7 +214,7 @@ struct KTimer {
 
 // Launch shapes, chosen by the autotuner (farcur_autotune) or defaults.
 struct Tuning {
-    int pinv_threads[kPinvClasses] = {64, 128, 128, 256, 256, 256, 512, 512, 512, 512};
+    int pinv_threads[kPinvClasses] = {64, 128, 128, 256, 256, 256, 512, 512, 512, 1024, 512};
     // fill workgroups per launch, capped at the launch's tile count: one per
     // tile balances tiles of unequal cost best (sphere: fill 5.40 ms with a
     // persistent grid of 4096, 5.10 with one workgroup per tile)
@@ -585,7 +585,7 @@ struct RoundPlan {
     int f32 = 0;
     bool pivot = false;
     int fill_grid = 0;
-    int pinv_threads[kPinvClasses] = {64, 128, 128, 256, 256, 256, 512, 512, 512, 512};
+    int pinv_threads[kPinvClasses] = {64, 128, 128, 256, 256, 256, 512, 512, 512, 1024, 512};
     int resid_threads = 256;
     double atol = 1e-10, rtol = 0, label_eps = 1e-4;
     int one_buf = 0;  // every job gathers its core from C (one-buffer pinv allowed)
@@ -650,7 +650,7 @@ void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt, hipStream_
     }
     kt.begin(3, s, &a);
     // held-out errors of the LDS-variant jobs came with their pinv launch
-    launch_residual(P.jobs, P.njobs, P.f32, P.resid_threads, 1, P.out, s);
+    launch_residual(P.jobs, P.njobs, P.f32, P.resid_threads, pinv_lds_limit(P.one_buf, P.rtol), P.out, s);
     kt.end(3, s, a);
 }
 
@@ -806,10 +806,10 @@ void farcur_opts_default(farcur_opts* o) {
     o->seed = 12345;
     o->precision = FARCUR_F64;
     o->row_mode = FARCUR_ROWS_UNIFORM;
-    o->heldout = 8;
-    o->oversample = 14;
+    o->heldout = 16;
+    o->oversample = 18;
     o->label_eps = 1e-4;
-    o->heldout_safety = 0.25;
+    o->heldout_safety = 0.3;
     o->growth = 1.5;
 }
 
@@ -1309,7 +1309,7 @@ int block_loop(farcur_ctx* ctx, const BlockLoopIO& io, const farcur_opts& o, All
         const bool big = (size_t)io.m * r > (size_t)256 * 1024 || r > 128;
         P.tune(ctx->tune);
         P.label_eps = o.label_eps;
-        if (r > kPinvLdsMaxR) P.pinv_threads[kPinvClasses - 1] = r > 128 ? 1024 : 256;
+        if (r > kPinvLdsMaxR) P.pinv_threads[kPinvClasses - 2] = P.pinv_threads[kPinvClasses - 1] = r > 128 ? 1024 : 256;
         if (big) P.resid_threads = 1024;
         HIPTRY(hipEventRecord(ctx->ev0, ctx->stream));
         run_round(ctx, P, kt, ctx->stream);
@@ -1568,7 +1568,7 @@ int compat_impl(farcur_c128* Cc, farcur_c128* Uc, farcur_c128* Rc, int r, double
     P.atol = pinv_tol;
     P.rtol = 0;
     P.tune(ctx->tune);
-    if (r > kPinvLdsMaxR) P.pinv_threads[kPinvClasses - 1] = r > 128 ? 1024 : 256;
+    if (r > kPinvLdsMaxR) P.pinv_threads[kPinvClasses - 2] = P.pinv_threads[kPinvClasses - 1] = r > 128 ? 1024 : 256;
     if ((size_t)m * r > (size_t)256 * 1024 || r > 128) P.resid_threads = 1024;
     KTimer kt;
     HIPTRY(hipEventRecord(ctx->ev0, s));
@@ -1890,7 +1890,14 @@ void advance_learned(BState& s, const JobOut& out, const RoundBufs& rb, double a
     s.err = s.h > 0 ? out.err_est : 0.0;
     s.heldout_err = s.h > 0 ? out.err : -1.0;
     s.round += 1;
-    if (s.h == 0 || s.err <= accept) {
+    if (s.r >= std::min(s.m, s.n)) {
+        // the sample took every row or every column: the block is stored
+        // exactly once the compress ends (ExactJob), nothing to check
+        s.status = 1;
+        s.err = 0.0;
+        return;
+    }
+    if (s.err <= accept) {
         s.status = 1;
         return;
     }
@@ -2834,7 +2841,7 @@ extern "C" int farcur_batch_kernel_times(farcur_batch* B, const char** names, do
     static const char* kNames[KTimer::kClasses] = {"sample",   "fill",     "pinv",     "residual",
                                                    "pinv_r16", "pinv_r20", "pinv_r24", "pinv_r28",
                                                    "pinv_r32", "pinv_r36", "pinv_r40", "pinv_r48",
-                                                   "pinv_r64", "pinv_global"};
+                                                   "pinv_r64", "pinv_r96", "pinv_global"};
     static const char* kHost[4] = {"host_predict", "host_plan", "host_wait", "host_advance"};
     int n = 0;
     for (int i = 0; i < KTimer::kClasses && n < cap; ++i, ++n) {
@@ -2936,7 +2943,9 @@ extern "C" int farcur_pinv_bench(farcur_ctx* ctx, int r, int nprob, int threads,
     if (!ctx || !ms) return fail(FARCUR_EINVAL, "NULL argument");
     if (r <= 0 || r > 1024 || nprob <= 0 || nprob > 65536) return fail(FARCUR_EINVAL, "bad sizes");
     if (threads <= 0 || threads > 1024 || threads % 64) return fail(FARCUR_EINVAL, "threads must be a multiple of 64");
-    if (use_lds && r > kPinvLdsMaxR) return fail(FARCUR_EINVAL, "LDS variant needs r <= %d", kPinvLdsMaxR);
+    if (use_lds && r > kPinvLdsOneBufMaxR)
+        return fail(FARCUR_EINVAL, "LDS variant needs r <= %d (above %d: one-buffer form)", kPinvLdsOneBufMaxR,
+                    kPinvLdsMaxR);
     TRY(ctx->select());
     return time_pinv(ctx, r, nprob, threads, use_lds ? r : 0, ms, sweeps);
 }
@@ -2948,7 +2957,7 @@ extern "C" int farcur_autotune(farcur_ctx* ctx, farcur_tuning* out) {
     farcur_tuning rep{};
     for (int c = 0; c < kPinvClasses; ++c) {
         const bool glb = c == kPinvClasses - 1;
-        const int r = glb ? 96 : kPinvClassMax[c];
+        const int r = glb ? 128 : kPinvClassMax[c];
         const int nprob = glb ? 1024 : 4096;
         // cores up to 32: 64..256 threads; larger LDS classes and the global
         // variant: 128..512 (64 threads would leave one lane pair per column pair)

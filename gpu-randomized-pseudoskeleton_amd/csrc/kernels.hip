@@ -312,7 +312,8 @@ struct HeldErr {
     double est;    // ratio estimate of ||Z - C U R||_F / ||Z||_F over the whole block
 };
 template <typename PC>
-__device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, double* s_red);
+__device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, double* s_red,
+                                                 c128* sU);
 
 // Element idx (column-major, r x r) of the core W: the W panel, or the rows
 // rpos of C (the same entries, filled once).
@@ -394,8 +395,8 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
             // group buffers: the two halves of the core buffer (rmax^2 complex,
             // rmax >= 16, so each half holds >= 8 rows of r <= rmax)
             const int cap = rmax * rmax / 2;
-            const HeldErr e = f32 ? heldout_error<c64>(J, L.a, L.a + cap, cap, L.norm)
-                                  : heldout_error<c128>(J, L.a, L.a + cap, cap, L.norm);
+            const HeldErr e = f32 ? heldout_error<c64>(J, L.a, L.a + cap, cap, L.norm, L.a)
+                                  : heldout_error<c128>(J, L.a, L.a + cap, cap, L.norm, L.a);
             if (threadIdx.x == 0) {
                 out[jb].err = e.plain;
                 out[jb].err_est = e.est;
@@ -463,8 +464,9 @@ void launch_lds(int grid, int threads, size_t lds, hipStream_t s, const BlockJob
                 JobOut* out) {
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)k_pinv_lds<kT, kE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)pinv_lds_bytes(kPinvLdsMaxR, false));
+        (void)hipFuncSetAttribute(
+            (const void*)k_pinv_lds<kT, kE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+            (int)std::max(pinv_lds_bytes(kPinvLdsMaxR, false), pinv_lds_bytes(kPinvLdsOneBufMaxR, true)));
         attr_set = true;
     }
     hipLaunchKernelGGL((k_pinv_lds<kT, kE>), dim3(grid), dim3(threads), lds, s, jobs, list, nlist, rmax, atol, rtol,
@@ -493,7 +495,7 @@ void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double at
                  int rmax, int threads, int f32, int one_buf, int heldout, JobOut* out, hipStream_t s) {
     if (nlist <= 0) return;
     const int grid = nlist < 16384 ? nlist : 16384;
-    if (rmax > 0 && rmax <= kPinvLdsMaxR) {
+    if (rmax > 0 && rmax <= pinv_lds_limit(one_buf, rtol)) {
         int T, E;
         pinv_lds_shape(rmax, threads, &T, &E);
         one_buf = one_buf && rtol >= 1e-8;
@@ -532,44 +534,62 @@ constexpr int kResidMaxR = 512;
 // those rows have the largest |P_i| and enter the estimate through it.
 constexpr int kHeldGroup = 8;
 
-// sum over `cnt` rows (positions pos(i)) of |C[pos, :] U|^2, rows in groups.
+// sum over `cnt` rows (positions pos(i)) of |C[pos, :] U|^2. A work item is
+// (row, chunk of 8 columns of U): consecutive threads take consecutive rows of
+// one chunk (coalesced C loads, U entries broadcast), each C element is read
+// once per chunk, and the 8 products accumulate in registers. U is read from
+// sU (LDS, r x r) when given, else from the job's U panel.
+constexpr int kPnChunk = 8;
 template <typename PC, typename Pos>
-__device__ __forceinline__ double pnorm_sum(const BlockJob& J, int cnt, Pos pos, c128* sT, int G) {
+__device__ __forceinline__ double pnorm_sum(const BlockJob& J, int cnt, Pos pos, const c128* sU) {
     const PC* C = (const PC*)J.C;
+    const c128* U = sU ? sU : J.U;
     const int m = J.m, r = J.r, tid = threadIdx.x, nt = blockDim.x;
+    const int nch = (r + kPnChunk - 1) / kPnChunk;
     double acc = 0;
-    for (int i0 = 0; i0 < cnt; i0 += G) {
-        const int g = min(G, cnt - i0);
-        for (int idx = tid; idx < g * r; idx += nt) {
-            const int i = idx / r, l = idx - i * r;
-            sT[idx] = to_c128(C[(size_t)l * m + pos(i0 + i)]);
-        }
-        __syncthreads();
-        for (int idx = tid; idx < g * r; idx += nt) {
-            const int i = idx / r, k = idx - i * r;
-            const c128* uc = J.U + (size_t)k * r;
-            const c128* ti = sT + i * r;
-            double ar = 0, ai = 0;
-#pragma unroll 8
-            for (int l = 0; l < r; ++l) {
-                const c128 u = uc[l], t = ti[l];
-                ar += t.re * u.re - t.im * u.im;
-                ai += t.re * u.im + t.im * u.re;
+    for (int item = tid; item < cnt * nch; item += nt) {
+        const int kc = item / cnt, i = pos(item - kc * cnt), k0 = kc * kPnChunk;
+        const int kn = min(kPnChunk, r - k0);
+        double ar[kPnChunk], ai[kPnChunk];
+#pragma unroll
+        for (int q = 0; q < kPnChunk; ++q) ar[q] = ai[q] = 0.0;
+        for (int l = 0; l < r; ++l) {
+            const c128 c = to_c128(C[(size_t)l * m + i]);
+            const c128* ul = U + (size_t)k0 * r + l;
+#pragma unroll
+            for (int q = 0; q < kPnChunk; ++q) {
+                if (q < kn) {
+                    const c128 u = ul[(size_t)q * r];
+                    ar[q] += c.re * u.re - c.im * u.im;
+                    ai[q] += c.re * u.im + c.im * u.re;
+                }
             }
-            acc += ar * ar + ai * ai;
         }
-        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPnChunk; ++q) acc += ar[q] * ar[q] + ai[q] * ai[q];
     }
     return acc;
 }
 
+// sU: LDS room for U (r x r) used before the group buffers, or NULL (U read
+// from the job's panel).
 template <typename PC>
-__device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, double* s_red) {
+__device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, double* s_red,
+                                                 c128* sU) {
     const PC* C = (const PC*)J.C;
     const PC* R = (const PC*)J.R;
     const PC* H = (const PC*)J.H;
     const int m = J.m, n = J.n, r = J.r, h = J.h, tid = threadIdx.x, nt = blockDim.x;
     const int G = max(1, min(kHeldGroup, cap / r));
+    // interpolation norms of every row and of the sampled rows (their
+    // difference: the unsampled rows), with U staged in LDS when there is room
+    if (sU) {
+        for (int idx = tid; idx < r * r; idx += nt) sU[idx] = J.U[idx];
+        __syncthreads();
+    }
+    const double pall = pnorm_sum<PC>(J, m, [&](int i) { return i; }, sU);
+    const double psam = pnorm_sum<PC>(J, r, [&](int i) { return J.rpos[i]; }, sU);
+    if (sU) __syncthreads();  // sU overlaps the group buffers
     double num = 0, den = 0, ph = 0;
     for (int i0 = 0; i0 < h; i0 += G) {
         const int g = min(G, h - i0);
@@ -623,10 +643,7 @@ __device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, c128* sT, c1
         }
         __syncthreads();
     }
-    // interpolation norms of every row and of the sampled rows (their
-    // difference: the unsampled rows), and |R|_F^2 (the sampled rows' norms)
-    const double pall = pnorm_sum<PC>(J, m, [&](int i) { return i; }, sT, G);
-    const double psam = pnorm_sum<PC>(J, r, [&](int i) { return J.rpos[i]; }, sT, G);
+    // |R|_F^2: the sampled rows' norms
     double rr = 0;
     for (int idx = tid; idx < r * n; idx += nt) {
         const c128 b = to_c128(R[idx]);
@@ -649,22 +666,23 @@ __device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, c128* sT, c1
     return e;
 }
 
-// skip_heldout: the pinv launches computed the held-out error of the jobs with
-// h > 0 and r <= kPinvLdsMaxR (their LDS variant); those are skipped here.
+// skip_heldout_rmax: the pinv launches computed the held-out error of the jobs
+// with h > 0 and r <= skip_heldout_rmax (their LDS variant); those are skipped here.
 template <typename PC>
-__global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, int skip_heldout, JobOut* __restrict__ out) {
+__global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, int skip_heldout_rmax,
+                           JobOut* __restrict__ out) {
     __shared__ c128 s_t1[kResidMaxR];
     __shared__ c128 s_t2[kResidMaxR];
     __shared__ c128 s_part[1024];
     __shared__ double s_red[16];
     for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
         const BlockJob& J = jobs[jb];
-        if (J.h > 0 && skip_heldout && J.r <= kPinvLdsMaxR) continue;
+        if (J.h > 0 && J.r <= skip_heldout_rmax) continue;
         if (J.h > 0) {
             // Beyond kResidMaxR the group buffers use the W panel (r*r >= 2r),
             // which the pinv has already consumed.
-            const HeldErr e = J.r <= kResidMaxR ? heldout_error<PC>(J, s_t1, s_t2, kResidMaxR, s_red)
-                                                : heldout_error<PC>(J, J.W, J.W + J.r, J.r, s_red);
+            const HeldErr e = J.r <= kResidMaxR ? heldout_error<PC>(J, s_t1, s_t2, kResidMaxR, s_red, nullptr)
+                                                : heldout_error<PC>(J, J.W, J.W + J.r, J.r, s_red, nullptr);
             if (threadIdx.x == 0) {
                 out[jb].err = e.plain;
                 out[jb].err_est = e.est;
@@ -685,14 +703,14 @@ __global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, int ski
     }
 }
 
-void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, int skip_heldout, JobOut* out,
+void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, int skip_heldout_rmax, JobOut* out,
                      hipStream_t s) {
     if (njobs <= 0) return;
     const int grid = njobs < 65536 ? njobs : 65536;  // one workgroup per block up to 65536
     if (f32)
-        hipLaunchKernelGGL(k_residual<c64>, dim3(grid), dim3(threads), 0, s, jobs, njobs, skip_heldout, out);
+        hipLaunchKernelGGL(k_residual<c64>, dim3(grid), dim3(threads), 0, s, jobs, njobs, skip_heldout_rmax, out);
     else
-        hipLaunchKernelGGL(k_residual<c128>, dim3(grid), dim3(threads), 0, s, jobs, njobs, skip_heldout, out);
+        hipLaunchKernelGGL(k_residual<c128>, dim3(grid), dim3(threads), 0, s, jobs, njobs, skip_heldout_rmax, out);
 }
 
 // ---------------------------------------------------------------- ACA

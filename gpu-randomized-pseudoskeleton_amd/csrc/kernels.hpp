@@ -114,11 +114,15 @@ void launch_fill(const BlockJob* jobs, int njobs, const int32_t* tile_prefix, in
 // selects the LDS variant sized for rmax (every listed job has r <= rmax);
 // rmax = 0 selects the global-memory variant.
 constexpr int kPinvLdsMaxR = 64;
-// r classes (0,16], (16,24], (24,32], (32,40], (40,48], (48,64], > 64: the LDS of
-// a launch is sized for its class maximum, so finer classes let small cores
-// pack more workgroups per CU.
-constexpr int kPinvClasses = 10;
-constexpr int kPinvClassMax[kPinvClasses] = {16, 20, 24, 28, 32, 36, 40, 48, 64, 0};
+// The one-buffer LDS form (every core gathered from C, relative threshold
+// >= 1e-8: the learned FP32 rounds) fits cores up to this size in 160 KB.
+constexpr int kPinvLdsOneBufMaxR = 96;
+// r classes (0,16], (16,20], ..., (48,64], (64,96], > 96: the LDS of a launch is
+// sized for its class maximum, so finer classes let small cores pack more
+// workgroups per CU. The (64,96] class runs in LDS only in the one-buffer form,
+// otherwise in global memory.
+constexpr int kPinvClasses = 11;
+constexpr int kPinvClassMax[kPinvClasses] = {16, 20, 24, 28, 32, 36, 40, 48, 64, 96, 0};
 inline int pinv_class(int r) {
     for (int c = 0; c < kPinvClasses - 1; ++c)
         if (r <= kPinvClassMax[c]) return c;
@@ -134,10 +138,16 @@ constexpr int kMaxSamples = 8192;
 // every listed job gathers W from C (rpos set), so with rtol >= 1e-8 the LDS
 // variant may use one r x r buffer (W panel and U as global scratch).
 // heldout: the LDS variant also computes the held-out error of jobs with h > 0
-// (launch_residual with skip_heldout then skips them).
+// (launch_residual then skips them: skip_heldout_rmax = pinv_lds_limit).
 void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, double label_eps,
                  int rmax, int threads, int f32, int one_buf, int heldout, JobOut* out, hipStream_t s);
-void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, int skip_heldout, JobOut* out,
+// Largest core the LDS variant takes for a launch with these flags.
+inline int pinv_lds_limit(int one_buf, double rtol) {
+    return one_buf && rtol >= 1e-8 ? kPinvLdsOneBufMaxR : kPinvLdsMaxR;
+}
+// skip_heldout_rmax > 0: jobs with h > 0 and r <= skip_heldout_rmax had their
+// held-out error computed by the pinv launch and are skipped.
+void launch_residual(const BlockJob* jobs, int njobs, int f32, int threads, int skip_heldout_rmax, JobOut* out,
                      hipStream_t s);
 
 // A block whose sample took every row (r = m: R = Z[rows, :] exactly) or

@@ -65,7 +65,7 @@ def _close_all():
 
 
 vp, ci, cd, cu64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint64
-PINV_CLASSES = 10  # FARCUR_PINV_CLASSES
+PINV_CLASSES = 11  # FARCUR_PINV_CLASSES
 
 
 class Opts(ctypes.Structure):

@@ -218,12 +218,12 @@ typedef struct {
     int r_start;     /* 0: reference rule r0 = D < 10 ? 1 : D / 10 (SW2:889-894) */
     int max_rank;    /* cap on the sample count (0: min(m, n), the reference cap) */
     int block_id;    /* stream id of a single-block call */
-    int heldout;     /* held-out rows per block in FARCUR_MODE_LEARNED (default 8; blocks of
+    int heldout;     /* held-out rows per block in FARCUR_MODE_LEARNED (default 16; blocks of
                         m > 32 * heldout rows use ceil(m / 32), at most 4 * heldout) */
-    int oversample;  /* p added to the predicted rank in FARCUR_MODE_LEARNED (default 14) */
+    int oversample;  /* p added to the predicted rank in FARCUR_MODE_LEARNED (default 18) */
     double label_eps;/* singular values >= label_eps * sigma_max are counted as rank_eps */
     double heldout_safety; /* learned mode accepts a round when its estimated block error <= heldout_safety * tol
-                              (0.25); the estimate scales the held-out rows' error by the interpolation norms
+                              (0.3); the estimate scales the held-out rows' error by the interpolation norms
                               |C[i, :] U|^2 of every unsampled row (a ratio estimator) */
     double growth;   /* learned mode: sample count factor after a failed check (1.5; the plain loop doubles) */
 } farcur_opts;
@@ -474,8 +474,10 @@ int farcur_batch_kernel_times(farcur_batch* batch, const char** names, double* m
  * times candidate shapes on the device and keeps the fastest per context.
  * pinv classes: sample counts (0,16], (16,20], (20,24], (24,28], (28,32],
  * (32,36], (36,40], (40,48], (48,64] (LDS variant sized for the class
- * maximum, so that small cores fit more workgroups per CU) and > 64 (global). */
-#define FARCUR_PINV_CLASSES 10
+ * maximum, so that small cores fit more workgroups per CU), (64,96] (LDS in
+ * the one-buffer form of the learned FP32 rounds, else global) and > 96
+ * (global). */
+#define FARCUR_PINV_CLASSES 11
 typedef struct {
     int pinv_threads[FARCUR_PINV_CLASSES];
     int fill_grid;     /* fill workgroups per launch (capped at the launch's tile count) */

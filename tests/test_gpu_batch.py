@@ -103,7 +103,7 @@ def test_batch_every_block_accurate(farcur, sphere, mode, precision):
     # in the launch shows up as a block whose CUR misses the dense block.
     ctx, g, part, (ro, ri, co, cix) = sphere
     bt = farcur.Batch(ctx, ro, ri, co, cix)
-    opts = dict(tol=1e-4, seed=17, precision=precision, heldout=8)
+    opts = dict(tol=1e-4, seed=17, precision=precision)  # the library's learned-check defaults
     if precision == farcur.F32:
         opts["pinv_rtol"] = 1e-6
     bt.compress(mode, **opts)

@@ -79,8 +79,9 @@ def test_operator_matches_dense_and_solves(farcur, small_sphere, oracle):
     assert np.linalg.norm(y - yd) <= 2e-5 * np.linalg.norm(yd)
     # GMRES on the device against the dense direct solve
     V = excitation.plane_wave(t["vertex"], t["edges"], t["ln"], t["cent"], K, 0.0, 0.0)
-    J, it, rr = op.solve(-V, tol=1e-8, restart=100, maxit=1000)
-    assert rr[0] <= 1e-8 and 0 < it[0] < 1000
+    # the far part runs in FP32: residuals below ~1e-7 are out of reach
+    J, it, rr = op.solve(-V, tol=1e-6, restart=100, maxit=1000)
+    assert rr[0] <= 1e-6 and 0 < it[0] < 1000
     Jd = np.linalg.solve(Zd, -V)
     assert np.linalg.norm(J - Jd) <= 1e-4 * np.linalg.norm(Jd)
     op.close()

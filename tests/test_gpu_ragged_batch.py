@@ -82,7 +82,7 @@ def test_ragged_f32_every_block_accurate(farcur, ragged, mode):
     # against the dense FP64 block.
     ctx, g, (ro, ri, co, cix), dense = ragged
     bt = farcur.Batch(ctx, ro, ri, co, cix)
-    bt.compress(mode, tol=1e-4, seed=35, precision=farcur.F32, pinv_rtol=1e-6, heldout=8)
+    bt.compress(mode, tol=1e-4, seed=35, precision=farcur.F32, pinv_rtol=1e-6)
     info = bt.info_arrays()
     for b, (m, n) in enumerate(SHAPES):
         out = bt.block(b)
