@@ -312,8 +312,7 @@ struct HeldErr {
     double est;    // ratio estimate of ||Z - C U R||_F / ||Z||_F over the whole block
 };
 template <typename PC>
-__device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, double* s_red,
-                                                 c128* sU);
+__device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, double* sP, int capd, double* s_red);
 
 // Element idx (column-major, r x r) of the core W: the W panel, or the rows
 // rpos of C (the same entries, filled once).
@@ -392,11 +391,11 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
             out[jb].sweeps = sweeps;
         }
         if (heldout && J.h > 0) {
-            // group buffers: the two halves of the core buffer (rmax^2 complex,
-            // rmax >= 16, so each half holds >= 8 rows of r <= rmax)
-            const int cap = rmax * rmax / 2;
-            const HeldErr e = f32 ? heldout_error<c64>(J, L.a, L.a + cap, cap, L.norm, L.a)
-                                  : heldout_error<c128>(J, L.a, L.a + cap, cap, L.norm, L.a);
+            // the core buffer (rmax^2 complex, rmax >= 16) holds the 16 x 2r
+            // interpolation tile of the held-out rows
+            const int capd = 2 * rmax * rmax;
+            const HeldErr e = f32 ? heldout_error<c64>(J, (double*)L.a, capd, L.norm)
+                                  : heldout_error<c128>(J, (double*)L.a, capd, L.norm);
             if (threadIdx.x == 0) {
                 out[jb].err = e.plain;
                 out[jb].err_est = e.est;
@@ -517,12 +516,115 @@ void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double at
 // ---------------------------------------------------------------- residual
 constexpr int kResidMaxR = 512;
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// Complex product X = A B (M x K times K x N) as the real product
+//   [X_re X_im] = [A_re A_im] [[B_re B_im], [-B_im B_re]]
+// on the FP64 matrix cores (v_mfma_f64_16x16x4f64: D(16 x 16) += A(16 x 4) B(4 x 16),
+// A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i % 4),
+// element i / 4). Real inner and column indices interleave the parts,
+// kk = 2 l + p and jj = 2 j + q (p, q: 0 real, 1 imaginary):
+//   A''[i][2l + p] = p ? Im A[i][l] : Re A[i][l]
+//   B''[2l + p][2j + q] = Re B[l][j] (p = q), Im B[l][j] (p = 0, q = 1), -Im B[l][j] (p = 1, q = 0).
+// The waves of the workgroup take the 16 x 16 output tiles in turn and hand
+// each to epi(i0, jj0, acc): lane t, element v holds X''[i0 + t / 16 + 4 v][jj0 + t % 16].
+// Operands outside M x K and K x N read as zero, so padded outputs are zero.
+// fa(i, l), fb(l, j): the complex entries. Every thread of the workgroup calls.
+template <typename FA, typename FB, typename Epi>
+__device__ __forceinline__ void cgemm_mfma(int M, int N, int K, const FA& fa, const FB& fb, const Epi& epi) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const int mt = (M + 15) >> 4, ntl = (2 * N + 15) >> 4;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int t = wave; t < mt * ntl; t += nwaves) {
+        const int i0 = (t / ntl) << 4, jj0 = (t % ntl) << 4;
+        const int i = i0 + li, jj = jj0 + li, j = jj >> 1, q = jj & 1;
+        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+        // four k-steps per iteration: their operand loads are all in flight
+        // before the first product
+        for (int k0 = 0; k0 < 2 * K; k0 += 16) {
+            double a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int kk = k0 + 4 * u + lk, l = kk >> 1, p = kk & 1;
+                a[u] = b[u] = 0.0;
+                if (l < K) {
+                    if (i < M) {
+                        const c128 x = fa(i, l);
+                        a[u] = p ? x.im : x.re;
+                    }
+                    if (j < N) {
+                        const c128 y = fb(l, j);
+                        b[u] = p == q ? y.re : (q ? y.im : -y.im);
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+        }
+        epi(i0, jj0, acc);
+    }
+}
+
+// sum over cnt rows pos(i) of |C[pos(i), :] U|^2 on the FP64 matrix cores, in
+// the real form of cgemm_mfma. U is read from sUt (r x r, row-major: sUt[l r + j]
+// = U[l][j], LDS or global). A wave takes a 16-row tile and loads its C
+// operands once per k-step for up to four column tiles (64 real columns).
+template <typename PC, typename Pos>
+__device__ __forceinline__ double pnorm_mfma(const PC* __restrict__ C, int m, int r, int cnt, const Pos& pos,
+                                             const c128* sUt) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int ncg = (2 * r + 31) >> 5;  // column groups of two 16-column tiles
+    const double* ud = (const double*)sUt;
+    double s = 0;
+    for (int t = wave; t < ((cnt + 15) >> 4) * ncg; t += nwaves) {
+        const int rt = t / ncg, cg = t - rt * ncg;
+        const int i = (rt << 4) + li;
+        const int row = i < cnt ? pos(i) : -1;
+        dbl4 acc[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < 2 * r; k0 += 16) {
+            double a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int kk = k0 + 4 * u + lk, l = kk >> 1;
+                a[u] = 0.0;
+                if (row >= 0 && l < r) {
+                    const c128 x = to_c128(C[(size_t)l * m + row]);
+                    a[u] = kk & 1 ? x.im : x.re;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int kk = k0 + 4 * u + lk, l = kk >> 1, p = kk & 1;
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const int jj = (cg << 5) + (c << 4) + li, j = jj >> 1, q = jj & 1;
+                    double b = 0.0;
+                    if (l < r && j < r) {
+                        // component p ^ q of U[l][j], negated for (p, q) = (1, 0)
+                        b = ud[2 * ((size_t)l * r + j) + (p ^ q)];
+                        if (p && !q) b = -b;
+                    }
+                    acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b, acc[c], 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) s += acc[c][0] * acc[c][0] + acc[c][1] * acc[c][1] + acc[c][2] * acc[c][2] +
+                                         acc[c][3] * acc[c][3];
+    }
+    return s;
+}
+
 // Relative Frobenius error of the CUR on the held-out rows,
 // ||H - C[hpos, :] U R||_F / ||H||_F, and a ratio estimate of the whole
-// block's error. The rows go in groups of up to kHeldGroup (as many as
-// sT / sY, `cap` complex values each, hold), so that U and R are read once per
-// group rather than once per row: T = C[rows, :] (g x r), Y = T U (the rows'
-// interpolation coefficients P_i = C[i, :] U), z_j = Y R[:, j] against H[j, :].
+// block's error, all products on the FP64 matrix cores (cgemm_mfma). The
+// held-out rows go 16 at a time (one tile row): Y = C[rows, :] U (their
+// interpolation coefficients P_i) into sP (16 x 2r doubles, capacity capd),
+// then Y R against H[rows, :]. The buffer first holds U row-major (2 r^2
+// doubles) for the interpolation norms and, when both fit, for Y.
 //
 // The estimate: a row's error |Z_i - P_i R| grows with |P_i| (the
 // interpolation from the sampled rows amplifies what they miss), so the
@@ -532,115 +634,63 @@ constexpr int kResidMaxR = 512;
 //           / (|R|_F^2 + (m - r) / h |H|_F^2).
 // Uniform held-out rows alone miss blocks whose error sits in a few rows;
 // those rows have the largest |P_i| and enter the estimate through it.
-constexpr int kHeldGroup = 8;
-
-// sum over `cnt` rows (positions pos(i)) of |C[pos, :] U|^2. A work item is
-// (row, chunk of 8 columns of U): consecutive threads take consecutive rows of
-// one chunk (coalesced C loads, U entries broadcast), each C element is read
-// once per chunk, and the 8 products accumulate in registers. U is read from
-// sU (LDS, r x r) when given, else from the job's U panel.
-constexpr int kPnChunk = 8;
-template <typename PC, typename Pos>
-__device__ __forceinline__ double pnorm_sum(const BlockJob& J, int cnt, Pos pos, const c128* sU) {
-    const PC* C = (const PC*)J.C;
-    const c128* U = sU ? sU : J.U;
-    const int m = J.m, r = J.r, tid = threadIdx.x, nt = blockDim.x;
-    const int nch = (r + kPnChunk - 1) / kPnChunk;
-    double acc = 0;
-    for (int item = tid; item < cnt * nch; item += nt) {
-        const int kc = item / cnt, i = pos(item - kc * cnt), k0 = kc * kPnChunk;
-        const int kn = min(kPnChunk, r - k0);
-        double ar[kPnChunk], ai[kPnChunk];
-#pragma unroll
-        for (int q = 0; q < kPnChunk; ++q) ar[q] = ai[q] = 0.0;
-        for (int l = 0; l < r; ++l) {
-            const c128 c = to_c128(C[(size_t)l * m + i]);
-            const c128* ul = U + (size_t)k0 * r + l;
-#pragma unroll
-            for (int q = 0; q < kPnChunk; ++q) {
-                if (q < kn) {
-                    const c128 u = ul[(size_t)q * r];
-                    ar[q] += c.re * u.re - c.im * u.im;
-                    ai[q] += c.re * u.im + c.im * u.re;
-                }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < kPnChunk; ++q) acc += ar[q] * ar[q] + ai[q] * ai[q];
-    }
-    return acc;
-}
-
-// sU: LDS room for U (r x r) used before the group buffers, or NULL (U read
-// from the job's panel).
 template <typename PC>
-__device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, c128* sT, c128* sY, int cap, double* s_red,
-                                                 c128* sU) {
+__device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, double* sP, int capd, double* s_red) {
     const PC* C = (const PC*)J.C;
     const PC* R = (const PC*)J.R;
     const PC* H = (const PC*)J.H;
+    const c128* U = J.U;
     const int m = J.m, n = J.n, r = J.r, h = J.h, tid = threadIdx.x, nt = blockDim.x;
-    const int G = max(1, min(kHeldGroup, cap / r));
-    // interpolation norms of every row and of the sampled rows (their
-    // difference: the unsampled rows), with U staged in LDS when there is room
-    if (sU) {
-        for (int idx = tid; idx < r * r; idx += nt) sU[idx] = J.U[idx];
-        __syncthreads();
+    const int lane = tid & 63;
+    // tile row stride: odd in doubles (rows on distinct LDS banks) when it fits
+    const int ldp = 16 * (2 * r + 1) <= capd ? 2 * r + 1 : 2 * r;
+    auto sumsq = [](const dbl4& a) { return a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + a[3] * a[3]; };
+    // U row-major into the buffer
+    c128* sUt = (c128*)sP;
+    for (int idx = tid; idx < r * r; idx += nt) {
+        const int j = idx / r, l = idx - j * r;
+        sUt[(size_t)l * r + j] = U[idx];
     }
-    const double pall = pnorm_sum<PC>(J, m, [&](int i) { return i; }, sU);
-    const double psam = pnorm_sum<PC>(J, r, [&](int i) { return J.rpos[i]; }, sU);
-    if (sU) __syncthreads();  // sU overlaps the group buffers
+    __syncthreads();
+    // interpolation norms of every row and of the sampled rows (their
+    // difference: the unsampled rows)
+    const double pall = pnorm_mfma(C, m, r, m, [](int i) { return i; }, sUt);
+    const double psam = pnorm_mfma(C, m, r, r, [&](int i) { return J.rpos[i]; }, sUt);
+    // Y's tile after U when both fit, else over it (U then read from the panel)
+    const bool keep_u = 2 * r * r + 16 * ldp <= capd;
+    double* sY = keep_u ? sP + 2 * r * r : sP;
+    if (!keep_u) __syncthreads();
+    auto fu = [&](int l, int j) { return keep_u ? sUt[(size_t)l * r + j] : U[(size_t)j * r + l]; };
     double num = 0, den = 0, ph = 0;
-    for (int i0 = 0; i0 < h; i0 += G) {
-        const int g = min(G, h - i0);
-        for (int idx = tid; idx < g * r; idx += nt) {
-            const int i = idx / r, l = idx - i * r;
-            sT[idx] = to_c128(C[(size_t)l * m + J.hpos[i0 + i]]);
-        }
+    for (int c0 = 0; c0 < h; c0 += 16) {
+        const int g = min(16, h - c0);
+        cgemm_mfma(g, r, r, [&](int i, int l) { return to_c128(C[(size_t)l * m + J.hpos[c0 + i]]); }, fu,
+                   [&](int i0, int jj0, const dbl4& a) {
+                       const int jj = jj0 + (lane & 15);
+#pragma unroll
+                       for (int v = 0; v < 4; ++v) {
+                           const int i = i0 + (lane >> 4) + 4 * v;
+                           if (i < g && jj < 2 * r) sY[i * ldp + jj] = a[v];
+                       }
+                       ph += sumsq(a);
+                   });
         __syncthreads();
-        // Y[i][k] = sum_l T[i][l] U[l][k], one entry per thread; the unrolled
-        // loop keeps several loads of U's column in flight.
-        for (int idx = tid; idx < g * r; idx += nt) {
-            const int i = idx / r, k = idx - i * r;
-            const c128* uc = J.U + (size_t)k * r;
-            const c128* ti = sT + i * r;
-            double ar = 0, ai = 0;
-#pragma unroll 8
-            for (int l = 0; l < r; ++l) {
-                const c128 u = uc[l], t = ti[l];
-                ar += t.re * u.re - t.im * u.im;
-                ai += t.re * u.im + t.im * u.re;
-            }
-            sY[idx] = c128{ar, ai};
-            ph += ar * ar + ai * ai;
-        }
-        __syncthreads();
-        for (int j = tid; j < n; j += nt) {  // z_i = sum_k Y[i][k] R[k, j]
-            double zr[kHeldGroup], zi[kHeldGroup];
+        cgemm_mfma(g, n, r, [&](int i, int l) { return c128{sY[i * ldp + 2 * l], sY[i * ldp + 2 * l + 1]}; },
+                   [&](int l, int j) { return to_c128(R[(size_t)j * r + l]); },
+                   [&](int i0, int jj0, const dbl4& a) {
+                       const int jj = jj0 + (lane & 15), j = jj >> 1, q = jj & 1;
+                       if (j >= n) return;
 #pragma unroll
-            for (int i = 0; i < kHeldGroup; ++i) zr[i] = zi[i] = 0.0;
-#pragma unroll 8
-            for (int k = 0; k < r; ++k) {
-                const c128 b = to_c128(R[(size_t)j * r + k]);
-#pragma unroll
-                for (int i = 0; i < kHeldGroup; ++i) {
-                    if (i < g) {
-                        const c128 y = sY[i * r + k];
-                        zr[i] += y.re * b.re - y.im * b.im;
-                        zi[i] += y.re * b.im + y.im * b.re;
-                    }
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < kHeldGroup; ++i) {
-                if (i < g) {
-                    const c128 hv = to_c128(H[(size_t)j * h + i0 + i]);
-                    const double dr = hv.re - zr[i], di = hv.im - zi[i];
-                    num += dr * dr + di * di;
-                    den += hv.re * hv.re + hv.im * hv.im;
-                }
-            }
-        }
+                       for (int v = 0; v < 4; ++v) {
+                           const int i = i0 + (lane >> 4) + 4 * v;
+                           if (i < g) {
+                               const c128 hv = to_c128(H[(size_t)j * h + c0 + i]);
+                               const double x = q ? hv.im : hv.re, d = x - a[v];
+                               num += d * d;
+                               den += x * x;
+                           }
+                       }
+                   });
         __syncthreads();
     }
     // |R|_F^2: the sampled rows' norms
@@ -679,10 +729,12 @@ __global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, int ski
         const BlockJob& J = jobs[jb];
         if (J.h > 0 && J.r <= skip_heldout_rmax) continue;
         if (J.h > 0) {
-            // Beyond kResidMaxR the group buffers use the W panel (r*r >= 2r),
-            // which the pinv has already consumed.
-            const HeldErr e = J.r <= kResidMaxR ? heldout_error<PC>(J, s_t1, s_t2, kResidMaxR, s_red, nullptr)
-                                                : heldout_error<PC>(J, J.W, J.W + J.r, J.r, s_red, nullptr);
+            // The 16 x 2r interpolation tile goes to s_t1 while it fits, else
+            // to the W panel (2 r^2 doubles >= 16 (2r + 1) for r >= 17), which
+            // the pinv has already consumed.
+            const HeldErr e = 16 * (2 * J.r + 1) <= 2 * kResidMaxR
+                                  ? heldout_error<PC>(J, (double*)s_t1, 2 * kResidMaxR, s_red)
+                                  : heldout_error<PC>(J, (double*)J.W, 2 * J.r * J.r, s_red);
             if (threadIdx.x == 0) {
                 out[jb].err = e.plain;
                 out[jb].err_est = e.est;
