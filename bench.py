@@ -616,7 +616,8 @@ def main():
                                f"Z_ba = Z_ab^T), tol 1e-4, learned rank + batched rounds",
                    "blocks": nb, "represented_entries": represented, "rank_model": model_src,
                    "learned_checks": {k: getattr(dflt, k) for k in ("oversample", "heldout", "heldout_safety",
-                                                                     "growth")}},
+                                                                     "growth")},
+                   "learned_core": "truncated QR" if dflt.learned_core == fc.CORE_QR else "SVD"},
         "represented_entries_per_s": represented / (ms * 1e-3),
         "filled_entries_per_s": filled / (ms * 1e-3),
         "miss_rate": misses / nb,

@@ -589,6 +589,7 @@ struct RoundPlan {
     int resid_threads = 256;
     double atol = 1e-10, rtol = 0, label_eps = 1e-4;
     int one_buf = 0;  // every job gathers its core from C (one-buffer pinv allowed)
+    int qr_core = 0;  // learned jobs take the truncated QR pseudo-inverse
     uint64_t seed = 0;
     // device pointers
     const BlockJob* jobs = nullptr;
@@ -641,7 +642,7 @@ void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt, hipStream_
         hipStream_t sc = side && (k++ & 1) == 0 ? side : s;
         kt.begin(4 + c, sc, &a);
         launch_pinv(P.jobs, P.list[c], P.nlist[c], P.atol, P.rtol, P.label_eps, kPinvClassMax[c], P.pinv_threads[c],
-                    P.f32, P.one_buf, 1, P.out, sc);
+                    P.f32, P.one_buf, 1, P.qr_core, P.out, sc);
         kt.end(4 + c, sc, a);
     }
     if (side) {
@@ -767,6 +768,8 @@ int validate_opts(const farcur_opts* o) {
     if (!(o->pinv_tol >= 0) || !(o->pinv_rtol >= 0)) return fail(FARCUR_EINVAL, "pinv thresholds must be >= 0");
     if (o->precision != FARCUR_F64 && o->precision != FARCUR_F32) return fail(FARCUR_EINVAL, "bad precision");
     if (o->row_mode != FARCUR_ROWS_UNIFORM && o->row_mode != FARCUR_ROWS_PIVOT) return fail(FARCUR_EINVAL, "bad row_mode");
+    if (o->learned_core != FARCUR_CORE_QR && o->learned_core != FARCUR_CORE_SVD)
+        return fail(FARCUR_EINVAL, "bad learned_core");
     if (o->r_start < 0 || o->max_rank < 0) return fail(FARCUR_EINVAL, "r_start and max_rank must be >= 0");
     if (o->heldout < 0 || o->heldout > 256) return fail(FARCUR_EINVAL, "heldout must be in [0, 256]");
     if (o->oversample < 0) return fail(FARCUR_EINVAL, "oversample must be >= 0");
@@ -811,6 +814,7 @@ void farcur_opts_default(farcur_opts* o) {
     o->label_eps = 1e-4;
     o->heldout_safety = 0.3;
     o->growth = 1.5;
+    o->learned_core = FARCUR_CORE_QR;
 }
 
 // ------------------------------------------------------------------ meshes
@@ -1081,7 +1085,8 @@ int farcur_pinv(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, doubl
     Plan* dp = ctx->d_plan.as<Plan>();
     const int cls = pinv_class(r);
     const int threads = cls < kPinvClasses - 1 ? ctx->tune.pinv_threads[cls] : (r > 128 ? 1024 : 256);
-    launch_pinv(&dp->job, dp->list, 1, atol, rtol, 0.0, kPinvClassMax[cls], threads, 0, 0, 0, &dp->out, ctx->stream);
+    launch_pinv(&dp->job, dp->list, 1, atol, rtol, 0.0, kPinvClassMax[cls], threads, 0, 0, 0, 0, &dp->out,
+                ctx->stream);
     HIPTRY(hipGetLastError());
     HIPTRY(hipMemcpyAsync(U, ctx->d_U.p, rr, hipMemcpyDeviceToHost, ctx->stream));
     std::vector<double> s(r);
@@ -2023,7 +2028,7 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
         uint64_t seed, geom;
         Tuning tune;
         const void* geom_ptr[5];  // device geometry the fill takes by value
-        int side, unused;
+        int side, qr_core;
     } kp;
     std::memset((void*)&kp, 0, sizeof kp);
     kp.f32 = B->f32;
@@ -2044,6 +2049,7 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
         std::memcpy(kp.geom_ptr, gp, sizeof gp);
     }
     kp.side = B->pinv_side_stream ? 1 : 0;
+    kp.qr_core = o.learned_core == FARCUR_CORE_QR ? 1 : 0;
     // Enqueues the round: plan upload from `hplan` to `dplan`, kernels, JobOut
     // download to `hout`.
     std::vector<RoundPlan> plans(nchunks);
@@ -2062,6 +2068,7 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
             P.rtol = o.pinv_rtol;
             P.seed = o.seed;
             P.label_eps = o.label_eps;
+            P.qr_core = kp.qr_core;
             P.tune(ctx->tune);
             // one chunk: the pinv classes use the second stream (not while
             // profiling, which times every class alone)
@@ -2918,7 +2925,7 @@ int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* 
         HIPTRY(hipMemcpyAsync(W.p, W0.p, nprob * rr * 16, hipMemcpyDeviceToDevice, s));
         HIPTRY(hipEventRecord(ctx->ev0, s));
         launch_pinv((const BlockJob*)d, (const int32_t*)(d + bj), nprob, 1e-10, gathered ? 1e-6 : 0.0, 0.0, rmax,
-                    threads, 0, gathered ? 1 : 0, 0, (JobOut*)(d + bj + bl + 256), s);
+                    threads, 0, gathered ? 1 : 0, 0, 0, (JobOut*)(d + bj + bl + 256), s);
         HIPTRY(hipEventRecord(ctx->ev1, s));
         HIPTRY(hipGetLastError());
         HIPTRY(hipStreamSynchronize(s));

@@ -484,6 +484,79 @@ __device__ __forceinline__ void apply_q_left(c128* Z, int r, int ld, const c128*
     }
 }
 
+// Truncated QR pseudo-inverse of W (in A, LDS; destroyed): W P = Q R with
+// column pivoting, k = the leading |R_jj| >= tau = max(atol, rtol |R_00|),
+// and U = P [R_11^{-1} Q_k^H; 0]. For k = r this is W^{-1}; with truncation
+// C U R interpolates the pivot columns (an interpolative decomposition)
+// instead of projecting on W's leading singular space. Uses U (ldu) as
+// scratch for R_11 and Wg for the reflectors. sigma[j] = |R_jj| (logical
+// order), smax = |R_00|, rank_eps counts |R_jj| >= label_eps |R_00|.
+template <int kT>
+__device__ __forceinline__ void pinv_qr_trunc(c128* A, int r, int ld, double atol, double rtol,
+                                              c128* __restrict__ U, int ldu, double* __restrict__ sigma,
+                                              c128* __restrict__ Wg, int* s_flag, int* s_phys, double* s_norm,
+                                              double* s_tau, c128* s_alpha, int* kept_out, double* smax_out,
+                                              double label_eps, int* rank_eps_out) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int cur = qrcp_lds<kT>(A, r, ld, s_phys, s_norm, s_tau, s_alpha, Wg);
+    const int* phys = s_phys + cur * r;
+    __syncthreads();
+    const double smax = sqrt(s_alpha[0].re * s_alpha[0].re + s_alpha[0].im * s_alpha[0].im);
+    const double tau = fmax(atol, rtol * smax);
+    int k = 0, reps = 0;
+    for (int j = 0; j < r; ++j) {
+        const double a = sqrt(s_alpha[j].re * s_alpha[j].re + s_alpha[j].im * s_alpha[j].im);
+        if (k == j && a >= tau && a > 0) k = j + 1;
+        reps += (a >= label_eps * smax && smax > 0);
+    }
+    for (int j = tid; j < r; j += nt)
+        sigma[j] = sqrt(s_alpha[j].re * s_alpha[j].re + s_alpha[j].im * s_alpha[j].im);
+    // R_11's strict upper triangle (logical order) to the U scratch
+    for (int idx = tid; idx < r * r; idx += nt) {
+        const int i = idx % r, j = idx / r;
+        if (i < j && j < k) U[(size_t)j * ldu + i] = A[(size_t)phys[j] * ld + i];
+    }
+    __syncthreads();
+    // 1 / R_jj in place of R_jj
+    for (int j = tid; j < k; j += nt) {
+        const c128 a = s_alpha[j];
+        const double ia = rcp_nr(a.re * a.re + a.im * a.im);
+        s_alpha[j] = c128{a.re * ia, -a.im * ia};
+    }
+    for (int idx = tid; idx < r * r; idx += nt) {
+        const int i = idx % r, j = idx / r;
+        A[(size_t)j * ld + i] = c128{i == j ? 1.0 : 0.0, 0.0};
+    }
+    __syncthreads();  // a team's column is written by other threads
+    apply_q_left<kT>(A, r, ld, Wg, s_tau);  // A = Q
+    __syncthreads();
+    // row c of Q_k: R_11 u = conj(Q[c, 0:k])^T by back substitution, in place
+    for (int c = tid; c < r; c += nt) {
+        for (int l = k - 1; l >= 0; --l) {
+            const c128 q = A[(size_t)l * ld + c];
+            double xr = q.re, xi = -q.im;
+            for (int s2 = l + 1; s2 < k; ++s2) {
+                const c128 rv = U[(size_t)s2 * ldu + l], u = A[(size_t)s2 * ld + c];
+                xr -= rv.re * u.re - rv.im * u.im;
+                xi -= rv.re * u.im + rv.im * u.re;
+            }
+            const c128 ia = s_alpha[l];
+            A[(size_t)l * ld + c] = c128{xr * ia.re - xi * ia.im, xr * ia.im + xi * ia.re};
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < r * r; idx += nt) {
+        const int l = idx % r, c = idx / r;
+        U[(size_t)c * ldu + phys[l]] = l < k ? A[(size_t)l * ld + c] : c128{0.0, 0.0};
+    }
+    if (tid == 0) {
+        *kept_out = k;
+        *smax_out = smax;
+        *rank_eps_out = reps;
+    }
+    __syncthreads();
+}
+
 // Pinv of W (in A, LDS; destroyed) preconditioned by QR with column pivoting:
 // W P = Q R, then one-sided Jacobi on X = R^H (X V = X_f, orthogonal columns
 // x_j = sigma_j u_j). X = R^H of a column-pivoted R has nearly orthogonal

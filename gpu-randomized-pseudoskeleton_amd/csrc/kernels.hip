@@ -354,10 +354,11 @@ size_t pinv_lds_bytes(int rmax, bool one_buf) {
 template <int kT, int kE>
 // heldout: the workgroup also computes the held-out error of each job with
 // h > 0 (it has just written U; R, H and C are filled), reusing the core's
-// LDS buffer; the residual launch then skips those jobs.
+// LDS buffer; the residual launch then skips those jobs. qr_core: jobs with
+// h > 0 in the one-buffer form take the truncated QR pseudo-inverse.
 __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist, int rmax,
                            double atol, double rtol, double label_eps, int f32, int one_buf, int heldout,
-                           JobOut* __restrict__ out) {
+                           int qr_core, JobOut* __restrict__ out) {
     extern __shared__ __align__(16) char smem[];
     const PinvLds L(smem, rmax, one_buf != 0);
     for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
@@ -382,8 +383,15 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
             const size_t at = (size_t)c * m + rp[k];
             return f32 ? to_c128(((const c64*)Cg)[at]) : ((const c128*)Cg)[at];
         };
-        pinv_qr_lds<kT, kE>(L.a, L.b, r, ld, atol, rtol, vfree, wload, J.U, r, J.sigma, J.W, L.flag, L.w, L.phys,
-                            L.norm, L.tau, L.alpha, &kept, &smax, label_eps, &reps, &sweeps);
+        if (qr_core && one_buf && J.h > 0) {
+            // learned rounds: the truncated QR pseudo-inverse (no SVD)
+            pinv_qr_trunc<kT>(L.a, r, ld, atol, rtol, J.U, r, J.sigma, J.W, L.flag, L.phys, L.norm, L.tau, L.alpha,
+                              &kept, &smax, label_eps, &reps);
+            sweeps = 0;
+        } else {
+            pinv_qr_lds<kT, kE>(L.a, L.b, r, ld, atol, rtol, vfree, wload, J.U, r, J.sigma, J.W, L.flag, L.w,
+                                L.phys, L.norm, L.tau, L.alpha, &kept, &smax, label_eps, &reps, &sweeps);
+        }
         if (threadIdx.x == 0) {
             out[jb].kept = kept;
             out[jb].sigma_max = smax;
@@ -460,7 +468,7 @@ namespace {
 template <int kT, int kE>
 void launch_lds(int grid, int threads, size_t lds, hipStream_t s, const BlockJob* jobs, const int32_t* list,
                 int nlist, int rmax, double atol, double rtol, double label_eps, int f32, int one_buf, int heldout,
-                JobOut* out) {
+                int qr_core, JobOut* out) {
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute(
@@ -469,7 +477,7 @@ void launch_lds(int grid, int threads, size_t lds, hipStream_t s, const BlockJob
         attr_set = true;
     }
     hipLaunchKernelGGL((k_pinv_lds<kT, kE>), dim3(grid), dim3(threads), lds, s, jobs, list, nlist, rmax, atol, rtol,
-                       label_eps, f32, one_buf, heldout, out);
+                       label_eps, f32, one_buf, heldout, qr_core, out);
 }
 
 // E rounded up to a compiled register count (1, 2, 3, 4, 6, 8, 16); past 16 elements the team
@@ -477,21 +485,21 @@ void launch_lds(int grid, int threads, size_t lds, hipStream_t s, const BlockJob
 template <int kT>
 void launch_lds_t(int E, int grid, int threads, size_t lds, hipStream_t s, const BlockJob* jobs,
                   const int32_t* list, int nlist, int rmax, double atol, double rtol, double label_eps, int f32,
-                  int one_buf, int heldout, JobOut* out) {
-    if (E <= 1) launch_lds<kT, 1>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
-    else if (E <= 2) launch_lds<kT, 2>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
-    else if (E <= 3) launch_lds<kT, 3>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
-    else if (E <= 4) launch_lds<kT, 4>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
-    else if (E <= 6) launch_lds<kT, 6>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
-    else if (E <= 8) launch_lds<kT, 8>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
-    else if (E <= 16) launch_lds<kT, 16>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
-    else launch_lds<kT, 0>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out);
+                  int one_buf, int heldout, int qr_core, JobOut* out) {
+    if (E <= 1) launch_lds<kT, 1>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, qr_core, out);
+    else if (E <= 2) launch_lds<kT, 2>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, qr_core, out);
+    else if (E <= 3) launch_lds<kT, 3>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, qr_core, out);
+    else if (E <= 4) launch_lds<kT, 4>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, qr_core, out);
+    else if (E <= 6) launch_lds<kT, 6>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, qr_core, out);
+    else if (E <= 8) launch_lds<kT, 8>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, qr_core, out);
+    else if (E <= 16) launch_lds<kT, 16>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, qr_core, out);
+    else launch_lds<kT, 0>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, qr_core, out);
 }
 
 }  // namespace
 
 void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, double label_eps,
-                 int rmax, int threads, int f32, int one_buf, int heldout, JobOut* out, hipStream_t s) {
+                 int rmax, int threads, int f32, int one_buf, int heldout, int qr_core, JobOut* out, hipStream_t s) {
     if (nlist <= 0) return;
     const int grid = nlist < 16384 ? nlist : 16384;
     if (rmax > 0 && rmax <= pinv_lds_limit(one_buf, rtol)) {
@@ -501,11 +509,11 @@ void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double at
         const size_t lds = pinv_lds_bytes(rmax, one_buf != 0);
         switch (T) {
             case 1:
-            case 2: launch_lds_t<2>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out); break;
-            case 4: launch_lds_t<4>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out); break;
-            case 8: launch_lds_t<8>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out); break;
-            case 16: launch_lds_t<16>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out); break;
-            default: launch_lds_t<32>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, out); break;
+            case 2: launch_lds_t<2>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, qr_core, out); break;
+            case 4: launch_lds_t<4>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, qr_core, out); break;
+            case 8: launch_lds_t<8>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, qr_core, out); break;
+            case 16: launch_lds_t<16>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, qr_core, out); break;
+            default: launch_lds_t<32>(E, grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, f32, one_buf, heldout, qr_core, out); break;
         }
     } else {
         hipLaunchKernelGGL(k_pinv_global, dim3(grid), dim3(threads), 0, s, jobs, list, nlist, atol, rtol, label_eps,

@@ -139,8 +139,10 @@ constexpr int kMaxSamples = 8192;
 // variant may use one r x r buffer (W panel and U as global scratch).
 // heldout: the LDS variant also computes the held-out error of jobs with h > 0
 // (launch_residual then skips them: skip_heldout_rmax = pinv_lds_limit).
+// qr_core: in the one-buffer LDS form, jobs with h > 0 (learned rounds) take the
+// truncated QR pseudo-inverse U = P [R_11^{-1} Q_k^H; 0] instead of the SVD one.
 void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, double label_eps,
-                 int rmax, int threads, int f32, int one_buf, int heldout, JobOut* out, hipStream_t s);
+                 int rmax, int threads, int f32, int one_buf, int heldout, int qr_core, JobOut* out, hipStream_t s);
 // Largest core the LDS variant takes for a launch with these flags.
 inline int pinv_lds_limit(int one_buf, double rtol) {
     return one_buf && rtol >= 1e-8 ? kPinvLdsOneBufMaxR : kPinvLdsMaxR;

@@ -110,7 +110,7 @@ void launch_pinv_noalloc(unsigned char* base, const c128* W, int r, double atol,
     const int grid = (int)std::min<size_t>((rr + 255) / 256, 256);
     hipLaunchKernelGGL(k_noalloc_setup, dim3(grid), dim3(256), 0, s, base, W, r, job);
     launch_pinv((const BlockJob*)(base + kOffJob), (const int32_t*)(base + kOffList), 1, atol, rtol, 0.0, rmax,
-                threads, 0, 0, 0, (JobOut*)(base + kOffOut), s);
+                threads, 0, 0, 0, 0, (JobOut*)(base + kOffOut), s);
     hipLaunchKernelGGL(k_noalloc_finish, dim3(1), dim3(256), 0, s, (const unsigned char*)base, r, sigma, kept);
 }
 

@@ -27,6 +27,7 @@ LIB_PATH = HERE / "libfarcur.so"
 F64, F32 = 0, 1
 ROWS_UNIFORM, ROWS_PIVOT = 0, 1
 MODE_PLAIN, MODE_BATCHED, MODE_LEARNED = 0, 1, 2
+CORE_QR, CORE_SVD = 0, 1  # FARCUR_CORE_*: learned mode's core pseudo-inverse
 MV_VALU, MV_MFMA_F32, MV_MFMA_BF16X3 = 0, 1, 2
 
 _lib = None
@@ -84,6 +85,7 @@ class Opts(ctypes.Structure):
         ("label_eps", cd),
         ("heldout_safety", cd),
         ("growth", cd),
+        ("learned_core", ci),
     ]
 
 

@@ -25,6 +25,7 @@ struct Opts
     label_eps::Cdouble
     heldout_safety::Cdouble
     growth::Cdouble
+    learned_core::Cint
 end
 
 struct BlockInfo

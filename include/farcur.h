@@ -208,6 +208,18 @@ enum {
     FARCUR_ROWS_PIVOT = 1    /* rows = argmax |C| per sampled column */
 };
 
+/* Core pseudo-inverse of the learned mode's rounds (complex-FP32 panels,
+ * relative threshold tau = max(pinv_tol, pinv_rtol * scale), cores up to r = 96):
+ *  FARCUR_CORE_QR: truncated QR with column pivoting, W P = Q R, k = the leading
+ *    |R_jj| >= tau (scale |R_00|), U = P [R_11^{-1} Q_k^H; 0] (W^{-1} when k = r;
+ *    with truncation C U R interpolates the k pivot columns). No SVD.
+ *  FARCUR_CORE_SVD: the SVD pseudo-inverse of the plain modes (scale sigma_max).
+ * Larger cores and FP64 learned rounds use the SVD pseudo-inverse. */
+enum {
+    FARCUR_CORE_QR = 0,
+    FARCUR_CORE_SVD = 1
+};
+
 typedef struct {
     double tol;      /* successive-residual tolerance (reference experiments: 1e-1, SW2:1155) */
     double pinv_tol; /* absolute singular value threshold (reference: 1e-10, SW2:884) */
@@ -226,6 +238,8 @@ typedef struct {
                               (0.3); the estimate scales the held-out rows' error by the interpolation norms
                               |C[i, :] U|^2 of every unsampled row (a ratio estimator) */
     double growth;   /* learned mode: sample count factor after a failed check (1.5; the plain loop doubles) */
+    int learned_core;/* learned mode's core pseudo-inverse: FARCUR_CORE_QR (default) or FARCUR_CORE_SVD;
+                        the plain modes always use the reference's SVD pseudo-inverse */
 } farcur_opts;
 
 void farcur_opts_default(farcur_opts* opts);

@@ -276,6 +276,46 @@ def set_threads(n: int) -> None:
     lib().orc_set_threads(int(n))
 
 
+def pinv_qr_trunc(W, atol=1e-10, rtol=0.0):
+    """Truncated QR pseudo-inverse of the learned rounds' cores (this build's
+    learned mode; its SVD alternative is pinv(), CU:1873-1924): Householder QR
+    with column pivoting W P = Q R, the pivot at step j being the logical
+    column with the largest trailing norm |A[j:, c]| (the first on ties);
+    k = the leading |R_jj| >= tau = max(atol, rtol |R_00|); U = P [R_11^{-1}
+    Q_k^H; 0]. Returns (U, |R_jj| in pivot order, k, pivots). Pins: inv(W) at
+    full rank, LAPACK geqp3's pivots, U W e_c = e_c on the k pivot columns and
+    W U an orthogonal projector of rank k (tests/test_oracle.py)."""
+    A = np.array(W, dtype=np.complex128)
+    r = A.shape[0]
+    perm = np.arange(r)
+    Q = np.eye(r, dtype=np.complex128)
+    for j in range(r):
+        norms = np.sum(np.abs(A[j:, perm[j:]]) ** 2, axis=0)
+        p = j + int(np.argmax(norms))
+        perm[[j, p]] = perm[[p, j]]
+        x = A[j:, perm[j]].copy()
+        xn = np.linalg.norm(x)
+        if xn == 0:
+            continue
+        ph = x[0] / abs(x[0]) if x[0] != 0 else 1.0
+        v = x.copy()
+        v[0] += ph * xn
+        v /= np.linalg.norm(v)
+        cols = perm[j:]
+        A[j:, cols] -= 2.0 * np.outer(v, v.conj() @ A[j:, cols])
+        Q[:, j:] -= 2.0 * np.outer(Q[:, j:] @ v, v.conj())
+    R = A[:, perm]
+    d = np.abs(np.diag(R))
+    tau = max(atol, rtol * d[0]) if r else 0.0
+    k = 0
+    while k < r and d[k] >= tau and d[k] > 0:
+        k += 1
+    U = np.zeros((r, r), dtype=np.complex128)
+    if k:
+        U[perm[:k], :] = np.linalg.solve(np.triu(R[:k, :k]), Q[:, :k].conj().T)
+    return U, d, k, perm
+
+
 def svd_compress(C, U, R, thresh):
     """SVD recompression of C U R (codes/ACA_vs_CUR/seed/svd_compress.m:19-34,
     called through postcompress_wrapper.m:1-5): QR of C and of R^H (economy),

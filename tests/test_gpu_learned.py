@@ -152,3 +152,52 @@ def test_learned_gain_and_accuracy(farcur, gctx, ellipsoid):
         assert e <= 1e-4 and abs(e - errs[b]) <= 1e-6 * e + 1e-9
     print(f"rounds {ib['rounds'].mean():.2f} -> {il['rounds'].mean():.2f}, filled {filled(ib):.3e} -> {filled(il):.3e}, "
           f"miss rate {learned.misses() / learned.nblocks:.3f}, max err {np.max(errs):.2e}")
+
+
+def test_learned_core_is_the_truncated_qr(farcur, gctx, ellipsoid, oracle):
+    # Learned rounds invert their cores by truncated QR with column pivoting
+    # (FARCUR_CORE_QR): each sampled block's U against the oracle's restatement
+    # on the same gathered core W = C[rpos, :]; on every sampled block, the
+    # pivot columns are interpolated exactly (U W e_c = e_c for k columns).
+    mesh, part, g = ellipsoid
+    gctx.set_mesh(mesh)
+    ro, ri, co, cix = part.far_blocks()
+    bt = farcur.Batch(gctx, ro, ri, co, cix)
+    bt.compress(farcur.MODE_LEARNED, tol=1e-4, seed=13, precision=farcur.F32, pinv_rtol=1e-6)
+    info = bt.info_arrays()
+    checked = agree = 0
+    for b in range(0, bt.nblocks, max(1, bt.nblocks // 60)):
+        m, n, r = ro[b + 1] - ro[b], co[b + 1] - co[b], info["r"][b]
+        if r >= min(m, n) or r > 96:
+            continue  # stored exactly, or an SVD core above the LDS classes
+        out = bt.block(b)
+        at = {int(x): i for i, x in enumerate(ri[ro[b]:ro[b + 1]])}
+        W = out["C"][[at[int(x)] for x in out["rs"]], :].astype(np.complex128)
+        Uo, d, k, perm = oracle.pinv_qr_trunc(W, 1e-10, 1e-6)
+        U = out["U"]
+        assert info["kept"][b] >= 1
+        UW = U @ W
+        exact = np.sum(np.linalg.norm(UW - np.eye(r), axis=0) <= 1e-6 * max(1.0, np.linalg.norm(UW, 2)))
+        assert exact >= info["kept"][b], (b, exact, info["kept"][b])
+        checked += 1
+        if k == info["kept"][b] and np.linalg.norm(U - Uo) <= 1e-7 * np.linalg.norm(Uo):
+            agree += 1
+    assert checked >= 20 and agree >= 0.9 * checked, (checked, agree)
+
+
+def test_learned_svd_core_option(farcur, gctx, ellipsoid):
+    # FARCUR_CORE_SVD keeps the SVD pseudo-inverse in learned rounds: a
+    # different U, and every block still within tol of its FP64 dense block.
+    mesh, part, g = ellipsoid
+    gctx.set_mesh(mesh)
+    ro, ri, co, cix = part.far_blocks()
+    a = farcur.Batch(gctx, ro, ri, co, cix)
+    s = farcur.Batch(gctx, ro, ri, co, cix)
+    opts = dict(tol=1e-4, seed=13, precision=farcur.F32, pinv_rtol=1e-6)
+    a.compress(farcur.MODE_LEARNED, **opts)
+    s.compress(farcur.MODE_LEARNED, learned_core=farcur.CORE_SVD, **opts)
+    assert np.all(s.info_arrays()["sweeps"] > 0) or np.any(a.info_arrays()["sweeps"] != s.info_arrays()["sweeps"])
+    errs, _, _, _ = s.verify(np.ones((mesh.sizes()[2], 1), complex), want_product=False)
+    assert np.max(errs) <= 1e-4
+    with pytest.raises(farcur.FarcurError, match="learned_core"):
+        s.compress(farcur.MODE_LEARNED, learned_core=7, **opts)

@@ -101,6 +101,43 @@ def test_pinv_zero_matrix(oracle):
     assert kept == 0 and np.all(U == 0)
 
 
+def test_pinv_qr_trunc_pins(oracle):
+    import scipy.linalg
+
+    rng = np.random.default_rng(3)
+    cplx = lambda *shape: rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    # full rank: the inverse
+    W = cplx(12, 12)
+    U, d, k, perm = oracle.pinv_qr_trunc(W)
+    Wi = np.linalg.inv(W)
+    assert k == 12
+    np.testing.assert_allclose(U, Wi, rtol=0, atol=1e-10 * np.abs(Wi).max())
+    # the greedy pivots and |R_jj| of LAPACK's geqp3 (column norms graded so
+    # that no two candidates tie)
+    for n in (5, 17, 40):
+        A = cplx(n, n) * np.logspace(0, -5, n)[rng.permutation(n)]
+        _, Rl, pl = scipy.linalg.qr(A, pivoting=True)
+        _, d, _, perm = oracle.pinv_qr_trunc(A)
+        np.testing.assert_array_equal(perm, pl)
+        np.testing.assert_allclose(d, np.abs(np.diag(Rl)), rtol=1e-9)
+    # a planted rank under the relative threshold: k pivot columns are
+    # interpolated exactly and W U is the orthogonal projector of rank k
+    n, k0 = 30, 9
+    X = cplx(n, k0) @ cplx(k0, n) + 1e-9 * cplx(n, n)
+    U, d, k, perm = oracle.pinv_qr_trunc(X, 1e-10, 1e-6)
+    assert k == k0
+    UX = U @ X
+    np.testing.assert_allclose(UX[:, perm[:k]], np.eye(n)[:, perm[:k]], atol=1e-9)
+    P = X @ U
+    assert np.abs(P @ P - P).max() <= 1e-8 and np.abs(P - P.conj().T).max() <= 1e-8
+    assert np.trace(P).real == pytest.approx(k0, abs=1e-8)
+    assert np.all(U[perm[k:], :] == 0)
+    # the threshold removes a planted weak direction: the SVD pseudo-inverse's
+    # kept count agrees when the gap is wide
+    Us, ss, kept = oracle.pinv(X, 1e-10, 1e-6)
+    assert kept == k0
+
+
 # ---------------------------------------------------------------- residual, pivots
 def test_residual_matches_numpy(oracle):
     rng = np.random.default_rng(3)
