@@ -484,6 +484,39 @@ __device__ __forceinline__ void apply_q_left(c128* Z, int r, int ld, const c128*
     }
 }
 
+// Z[:, j] = Q e_j for j < ncols (Q = H_0 H_1 ... H_{r-1}, the reflectors of
+// qrcp_lds in Wg, tau_k in s_tau). H_i leaves e_j unchanged for i > j, so
+// column j needs H_j, ..., H_0 only. A team owns a column: no workgroup barrier.
+template <int kT>
+__device__ __forceinline__ void form_q_cols(c128* Z, int r, int ld, int ncols, const c128* __restrict__ Wg,
+                                            const double* s_tau) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nteams = nt / kT, team = tid / kT, tl = tid - team * kT;
+    for (int j = team; j < ncols; j += nteams) {
+        c128* z = Z + (size_t)j * ld;
+        for (int i = tl; i < r; i += kT) z[i] = c128{i == j ? 1.0 : 0.0, 0.0};
+        for (int k = j; k >= 0; --k) {
+            const c128* v = Wg + (size_t)k * r;
+            const double tau = s_tau[k];
+            double sr = 0, si = 0;
+            for (int i = k + tl; i < r; i += kT) {  // s = v^H z
+                const c128 vi = v[i], zi = z[i];
+                sr += vi.re * zi.re + vi.im * zi.im;
+                si += vi.re * zi.im - vi.im * zi.re;
+            }
+            sr = team_sum_t<kT>(sr) * tau;
+            si = team_sum_t<kT>(si) * tau;
+            for (int i = k + tl; i < r; i += kT) {
+                const c128 vi = v[i];
+                c128 zi = z[i];
+                zi.re -= vi.re * sr - vi.im * si;
+                zi.im -= vi.re * si + vi.im * sr;
+                z[i] = zi;
+            }
+        }
+    }
+}
+
 // Truncated QR pseudo-inverse of W (in A, LDS; destroyed): W P = Q R with
 // column pivoting, k = the leading |R_jj| >= tau = max(atol, rtol |R_00|),
 // and U = P [R_11^{-1} Q_k^H; 0]. For k = r this is W^{-1}; with truncation
@@ -523,18 +556,15 @@ __device__ __forceinline__ void pinv_qr_trunc(c128* A, int r, int ld, double ato
         const double ia = rcp_nr(a.re * a.re + a.im * a.im);
         s_alpha[j] = c128{a.re * ia, -a.im * ia};
     }
-    for (int idx = tid; idx < r * r; idx += nt) {
-        const int i = idx % r, j = idx / r;
-        A[(size_t)j * ld + i] = c128{i == j ? 1.0 : 0.0, 0.0};
-    }
-    __syncthreads();  // a team's column is written by other threads
-    apply_q_left<kT>(A, r, ld, Wg, s_tau);  // A = Q
+    __syncthreads();  // A's R is consumed; the teams overwrite its columns
+    form_q_cols<kT>(A, r, ld, k, Wg, s_tau);  // A[:, :k] = Q_k
     __syncthreads();
     // row c of Q_k: R_11 u = conj(Q[c, 0:k])^T by back substitution, in place
     for (int c = tid; c < r; c += nt) {
         for (int l = k - 1; l >= 0; --l) {
             const c128 q = A[(size_t)l * ld + c];
             double xr = q.re, xi = -q.im;
+#pragma unroll 4
             for (int s2 = l + 1; s2 < k; ++s2) {
                 const c128 rv = U[(size_t)s2 * ldu + l], u = A[(size_t)s2 * ld + c];
                 xr -= rv.re * u.re - rv.im * u.im;
