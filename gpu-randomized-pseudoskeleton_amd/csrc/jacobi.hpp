@@ -579,11 +579,10 @@ __device__ __forceinline__ void pinv_qr_trunc(c128* A, int r, int ld, double ato
         const int l = idx % r, c = idx / r;
         U[(size_t)c * ldu + phys[l]] = l < k ? A[(size_t)l * ld + c] : c128{0.0, 0.0};
     }
-    if (tid == 0) {
-        *kept_out = k;
-        *smax_out = smax;
-        *rank_eps_out = reps;
-    }
+    // every thread computed k, smax and reps alike (the outputs are per thread)
+    *kept_out = k;
+    *smax_out = smax;
+    *rank_eps_out = reps;
     __syncthreads();
 }
 

@@ -312,7 +312,8 @@ struct HeldErr {
     double est;    // ratio estimate of ||Z - C U R||_F / ||Z||_F over the whole block
 };
 template <typename PC>
-__device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, double* sP, int capd, double* s_red);
+__device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, double* sP, int capd, double* s_red,
+                                                 int psam_known);
 
 // Element idx (column-major, r x r) of the core W: the W panel, or the rows
 // rpos of C (the same entries, filled once).
@@ -402,8 +403,11 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
             // the core buffer (rmax^2 complex, rmax >= 16) holds the 16 x 2r
             // interpolation tile of the held-out rows
             const int capd = 2 * rmax * rmax;
-            const HeldErr e = f32 ? heldout_error<c64>(J, (double*)L.a, capd, L.norm)
-                                  : heldout_error<c128>(J, (double*)L.a, capd, L.norm);
+            // with the QR core, W U = Q_k Q_k^H: the sampled rows' interpolation
+            // norms sum to k
+            const int pk = qr_core && one_buf ? kept : -1;
+            const HeldErr e = f32 ? heldout_error<c64>(J, (double*)L.a, capd, L.norm, pk)
+                                  : heldout_error<c128>(J, (double*)L.a, capd, L.norm, pk);
             if (threadIdx.x == 0) {
                 out[jb].err = e.plain;
                 out[jb].err_est = e.est;
@@ -626,6 +630,62 @@ __device__ __forceinline__ double pnorm_mfma(const PC* __restrict__ C, int m, in
     return s;
 }
 
+// Held-out comparison of g rows: Y R (Y = sY, g x r complex, real-interleaved
+// with row stride ldp) against H[c0 .. c0 + g, :], accumulating |H - Y R|^2
+// and |H|^2 in the real form of cgemm_mfma. With sRw (kZSlot doubles per
+// wave in LDS) a wave stages each 8 x 8 complex block of R (eight 64-byte
+// column segments, one entry per lane) once for its four k-steps, instead of
+// every lane loading its own entry of R for every product; without, R is read
+// from the panel. A wave owns its slot (LDS is in order within a wave).
+constexpr int kZSlot = 2 * 8 * 9;  // 8 columns x 8 rows of c128, column stride 9 (bank spread)
+template <typename PC>
+__device__ __forceinline__ void zpass_mfma(int g, int n, int r, const double* sY, int ldp, const PC* __restrict__ R,
+                                           const PC* __restrict__ H, int h, int c0, double* sRw, double& num,
+                                           double& den) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int ntl = (2 * n + 15) >> 4;
+    c128* slot = sRw ? (c128*)(sRw + (size_t)wave * kZSlot) : nullptr;
+    for (int t = wave; t < ntl; t += nwaves) {
+        const int jj0 = t << 4, j0 = jj0 >> 1;
+        const int jl = li >> 1, q = li & 1, j = j0 + jl;
+        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+        for (int l0 = 0; l0 < r; l0 += 8) {
+            if (slot) {
+                const int sj = lane >> 3, sl = lane & 7;
+                c128 x{0.0, 0.0};
+                if (j0 + sj < n && l0 + sl < r) x = to_c128(R[(size_t)(j0 + sj) * r + l0 + sl]);
+                slot[sj * 9 + sl] = x;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int kk = 4 * u + lk, ll = kk >> 1, p = kk & 1, l = l0 + ll;
+                double a = 0.0, b = 0.0;
+                if (l < r) {
+                    if (li < g) a = sY[li * ldp + 2 * l + p];
+                    if (j < n) {
+                        const c128 y = slot ? slot[jl * 9 + ll] : to_c128(R[(size_t)j * r + l]);
+                        b = p == q ? y.re : (q ? y.im : -y.im);
+                    }
+                }
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            }
+        }
+        if (j < n) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = lk + 4 * v;
+                if (i < g) {
+                    const c128 hv = to_c128(H[(size_t)j * h + c0 + i]);
+                    const double x = q ? hv.im : hv.re, d = x - acc[v];
+                    num += d * d;
+                    den += x * x;
+                }
+            }
+        }
+    }
+}
+
 // Relative Frobenius error of the CUR on the held-out rows,
 // ||H - C[hpos, :] U R||_F / ||H||_F, and a ratio estimate of the whole
 // block's error, all products on the FP64 matrix cores (cgemm_mfma). The
@@ -642,8 +702,11 @@ __device__ __forceinline__ double pnorm_mfma(const PC* __restrict__ C, int m, in
 //           / (|R|_F^2 + (m - r) / h |H|_F^2).
 // Uniform held-out rows alone miss blocks whose error sits in a few rows;
 // those rows have the largest |P_i| and enter the estimate through it.
+// psam_known >= 0: the sampled rows' interpolation norms sum to it (the QR
+// core's W U is a projector of that rank), and their pass is skipped.
 template <typename PC>
-__device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, double* sP, int capd, double* s_red) {
+__device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, double* sP, int capd, double* s_red,
+                                                 int psam_known) {
     const PC* C = (const PC*)J.C;
     const PC* R = (const PC*)J.R;
     const PC* H = (const PC*)J.H;
@@ -663,12 +726,16 @@ __device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, double* sP, 
     // interpolation norms of every row and of the sampled rows (their
     // difference: the unsampled rows)
     const double pall = pnorm_mfma(C, m, r, m, [](int i) { return i; }, sUt);
-    const double psam = pnorm_mfma(C, m, r, r, [&](int i) { return J.rpos[i]; }, sUt);
+    const double psam = psam_known >= 0 ? (threadIdx.x == 0 ? (double)psam_known : 0.0)
+                                        : pnorm_mfma(C, m, r, r, [&](int i) { return J.rpos[i]; }, sUt);
     // Y's tile after U when both fit, else over it (U then read from the panel)
     const bool keep_u = 2 * r * r + 16 * ldp <= capd;
     double* sY = keep_u ? sP + 2 * r * r : sP;
     if (!keep_u) __syncthreads();
     auto fu = [&](int l, int j) { return keep_u ? sUt[(size_t)l * r + j] : U[(size_t)j * r + l]; };
+    // R staging slots after Y's tile, when they fit
+    const size_t zoff = (size_t)(sY - sP) + 16 * (size_t)ldp;
+    double* sRw = zoff + (size_t)(blockDim.x >> 6) * kZSlot <= (size_t)capd ? sP + zoff : nullptr;
     double num = 0, den = 0, ph = 0;
     for (int c0 = 0; c0 < h; c0 += 16) {
         const int g = min(16, h - c0);
@@ -683,22 +750,7 @@ __device__ __forceinline__ HeldErr heldout_error(const BlockJob& J, double* sP, 
                        ph += sumsq(a);
                    });
         __syncthreads();
-        cgemm_mfma(g, n, r, [&](int i, int l) { return c128{sY[i * ldp + 2 * l], sY[i * ldp + 2 * l + 1]}; },
-                   [&](int l, int j) { return to_c128(R[(size_t)j * r + l]); },
-                   [&](int i0, int jj0, const dbl4& a) {
-                       const int jj = jj0 + (lane & 15), j = jj >> 1, q = jj & 1;
-                       if (j >= n) return;
-#pragma unroll
-                       for (int v = 0; v < 4; ++v) {
-                           const int i = i0 + (lane >> 4) + 4 * v;
-                           if (i < g) {
-                               const c128 hv = to_c128(H[(size_t)j * h + c0 + i]);
-                               const double x = q ? hv.im : hv.re, d = x - a[v];
-                               num += d * d;
-                               den += x * x;
-                           }
-                       }
-                   });
+        zpass_mfma<PC>(g, n, r, sY, ldp, R, H, h, c0, sRw, num, den);
         __syncthreads();
     }
     // |R|_F^2: the sampled rows' norms
@@ -741,8 +793,8 @@ __global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, int ski
             // to the W panel (2 r^2 doubles >= 16 (2r + 1) for r >= 17), which
             // the pinv has already consumed.
             const HeldErr e = 16 * (2 * J.r + 1) <= 2 * kResidMaxR
-                                  ? heldout_error<PC>(J, (double*)s_t1, 2 * kResidMaxR, s_red)
-                                  : heldout_error<PC>(J, (double*)J.W, 2 * J.r * J.r, s_red);
+                                  ? heldout_error<PC>(J, (double*)s_t1, 2 * kResidMaxR, s_red, -1)
+                                  : heldout_error<PC>(J, (double*)J.W, 2 * J.r * J.r, s_red, -1);
             if (threadIdx.x == 0) {
                 out[jb].err = e.plain;
                 out[jb].err_est = e.est;
