@@ -557,21 +557,47 @@ __device__ __forceinline__ void pinv_qr_trunc(c128* A, int r, int ld, double ato
         s_alpha[j] = c128{a.re * ia, -a.im * ia};
     }
     __syncthreads();  // A's R is consumed; the teams overwrite its columns
-    form_q_cols<kT>(A, r, ld, k, Wg, s_tau);  // A[:, :k] = Q_k
+    // A[:, :k] = Q_k, with teams narrow enough to give every column its own
+    // (the reflector chain of a column is serial)
+    if constexpr (kT >= 8) {
+        if (blockDim.x / kT < k && blockDim.x / (kT / 2) >= k)
+            form_q_cols<kT / 2>(A, r, ld, k, Wg, s_tau);
+        else if (blockDim.x / kT < k)
+            form_q_cols<kT / 4>(A, r, ld, k, Wg, s_tau);
+        else
+            form_q_cols<kT>(A, r, ld, k, Wg, s_tau);
+    } else if constexpr (kT >= 4) {
+        if (blockDim.x / kT < k)
+            form_q_cols<kT / 2>(A, r, ld, k, Wg, s_tau);
+        else
+            form_q_cols<kT>(A, r, ld, k, Wg, s_tau);
+    } else {
+        form_q_cols<kT>(A, r, ld, k, Wg, s_tau);
+    }
     __syncthreads();
-    // row c of Q_k: R_11 u = conj(Q[c, 0:k])^T by back substitution, in place
-    for (int c = tid; c < r; c += nt) {
+    // row c of Q_k: R_11 u = conj(Q[c, 0:k])^T by back substitution, in place;
+    // four lanes per row share each step's dot product (quad sums), the
+    // first of them writes u_l
+    for (int c0 = tid >> 2; c0 < r; c0 += nt >> 2) {  // a quad shares c0: its DPP lanes are all active
+        const int c = c0, sub = tid & 3;
+        const bool live = c < r;
         for (int l = k - 1; l >= 0; --l) {
-            const c128 q = A[(size_t)l * ld + c];
-            double xr = q.re, xi = -q.im;
-#pragma unroll 4
-            for (int s2 = l + 1; s2 < k; ++s2) {
-                const c128 rv = U[(size_t)s2 * ldu + l], u = A[(size_t)s2 * ld + c];
-                xr -= rv.re * u.re - rv.im * u.im;
-                xi -= rv.re * u.im + rv.im * u.re;
+            double xr = 0, xi = 0;
+            if (live) {
+                for (int s2 = l + 1 + sub; s2 < k; s2 += 4) {
+                    const c128 rv = U[(size_t)s2 * ldu + l], u = A[(size_t)s2 * ld + c];
+                    xr += rv.re * u.re - rv.im * u.im;
+                    xi += rv.re * u.im + rv.im * u.re;
+                }
             }
-            const c128 ia = s_alpha[l];
-            A[(size_t)l * ld + c] = c128{xr * ia.re - xi * ia.im, xr * ia.im + xi * ia.re};
+            xr = team_sum_t<4>(xr);
+            xi = team_sum_t<4>(xi);
+            if (live && sub == 0) {
+                const c128 q = A[(size_t)l * ld + c];
+                const double yr = q.re - xr, yi = -q.im - xi;
+                const c128 ia = s_alpha[l];
+                A[(size_t)l * ld + c] = c128{yr * ia.re - yi * ia.im, yr * ia.im + yi * ia.re};
+            }
         }
     }
     __syncthreads();
