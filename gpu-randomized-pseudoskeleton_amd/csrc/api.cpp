@@ -2875,7 +2875,10 @@ namespace {
 // as in a batch round: each core gathered from a panel (here the pristine
 // copy W0 through identity row positions) with the FP32 pipeline's relative
 // threshold 1e-6, i.e. the one-buffer V-free form.
-int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* best_ms, double* sweeps = nullptr) {
+// qr_core: the gathered cores take the truncated QR pseudo-inverse of the
+// learned rounds instead of the SVD one.
+int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* best_ms, double* sweeps = nullptr,
+              bool qr_core = false) {
     hipStream_t s = ctx->stream;
     const size_t rr = (size_t)r * r;
     DBuf W0, W, V, U, sig, plan, ids;
@@ -2925,7 +2928,7 @@ int time_pinv(farcur_ctx* ctx, int r, int nprob, int threads, int rmax, double* 
         HIPTRY(hipMemcpyAsync(W.p, W0.p, nprob * rr * 16, hipMemcpyDeviceToDevice, s));
         HIPTRY(hipEventRecord(ctx->ev0, s));
         launch_pinv((const BlockJob*)d, (const int32_t*)(d + bj), nprob, 1e-10, gathered ? 1e-6 : 0.0, 0.0, rmax,
-                    threads, 0, gathered ? 1 : 0, 0, 0, (JobOut*)(d + bj + bl + 256), s);
+                    threads, 0, gathered ? 1 : 0, 0, gathered && qr_core ? 2 : 0, (JobOut*)(d + bj + bl + 256), s);
         HIPTRY(hipEventRecord(ctx->ev1, s));
         HIPTRY(hipGetLastError());
         HIPTRY(hipStreamSynchronize(s));
@@ -2973,7 +2976,8 @@ extern "C" int farcur_autotune(farcur_ctx* ctx, farcur_tuning* out) {
         for (int k = 0; k < 3; ++k) {
             const int th = base << k;
             double ms;
-            TRY(time_pinv(ctx, r, nprob, th, kPinvClassMax[c], &ms));
+            // LDS classes as the learned rounds run them (truncated QR core)
+            TRY(time_pinv(ctx, r, nprob, th, kPinvClassMax[c], &ms, nullptr, kPinvClassMax[c] > 0));
             if (ms < best) {
                 best = ms;
                 t.pinv_threads[c] = th;

@@ -356,7 +356,8 @@ template <int kT, int kE>
 // heldout: the workgroup also computes the held-out error of each job with
 // h > 0 (it has just written U; R, H and C are filled), reusing the core's
 // LDS buffer; the residual launch then skips those jobs. qr_core: jobs with
-// h > 0 in the one-buffer form take the truncated QR pseudo-inverse.
+// h > 0 in the one-buffer form take the truncated QR pseudo-inverse (2: every
+// job of the one-buffer form).
 __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist, int rmax,
                            double atol, double rtol, double label_eps, int f32, int one_buf, int heldout,
                            int qr_core, JobOut* __restrict__ out) {
@@ -384,8 +385,9 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
             const size_t at = (size_t)c * m + rp[k];
             return f32 ? to_c128(((const c64*)Cg)[at]) : ((const c128*)Cg)[at];
         };
-        if (qr_core && one_buf && J.h > 0) {
-            // learned rounds: the truncated QR pseudo-inverse (no SVD)
+        if (qr_core && one_buf && (J.h > 0 || qr_core == 2)) {
+            // learned rounds (qr_core 2: every core, the autotuner's probes):
+            // the truncated QR pseudo-inverse (no SVD)
             pinv_qr_trunc<kT>(L.a, r, ld, atol, rtol, J.U, r, J.sigma, J.W, L.flag, L.phys, L.norm, L.tau, L.alpha,
                               &kept, &smax, label_eps, &reps);
             sweeps = 0;
@@ -405,7 +407,7 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
             const int capd = 2 * rmax * rmax;
             // with the QR core, W U = Q_k Q_k^H: the sampled rows' interpolation
             // norms sum to k
-            const int pk = qr_core && one_buf ? kept : -1;
+            const int pk = qr_core && one_buf ? kept : -1;  // (jobs with h > 0 took the QR core)
             const HeldErr e = f32 ? heldout_error<c64>(J, (double*)L.a, capd, L.norm, pk)
                                   : heldout_error<c128>(J, (double*)L.a, capd, L.norm, pk);
             if (threadIdx.x == 0) {

@@ -140,7 +140,8 @@ constexpr int kMaxSamples = 8192;
 // heldout: the LDS variant also computes the held-out error of jobs with h > 0
 // (launch_residual then skips them: skip_heldout_rmax = pinv_lds_limit).
 // qr_core: in the one-buffer LDS form, jobs with h > 0 (learned rounds) take the
-// truncated QR pseudo-inverse U = P [R_11^{-1} Q_k^H; 0] instead of the SVD one.
+// truncated QR pseudo-inverse U = P [R_11^{-1} Q_k^H; 0] instead of the SVD one;
+// 2: every job of that form (timing probes without held-out rows).
 void launch_pinv(const BlockJob* jobs, const int32_t* list, int nlist, double atol, double rtol, double label_eps,
                  int rmax, int threads, int f32, int one_buf, int heldout, int qr_core, JobOut* out, hipStream_t s);
 // Largest core the LDS variant takes for a launch with these flags.
