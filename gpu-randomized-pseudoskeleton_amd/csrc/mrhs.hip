@@ -20,6 +20,8 @@
 // xy ~ hi hi + hi lo + lo hi) with FP32 accumulation.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "mrhs.hpp"
 
@@ -101,51 +103,102 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
     lo = (__bf16)(x - (float)hi);
 }
 
-// Loads rows [row0, row0 + 64) x K chunk [k0, k0 + 32) of a row-segmented A
-// (per-row pointer, leading dimension and local row in LDS; A column-major, so
-// consecutive rows of one segment are contiguous) and the gathered B chunk.
+// BF16x3 staging: each element split once into hi / lo as it is staged, real
+// and imaginary planes, A as [row][k] and B as [col][k], so that an MFMA
+// operand (8 consecutive k) is one 16-byte LDS read. Rows padded by 8 (16 B).
+constexpr int kKP = kKC + 8;
+struct GemmLdsB {
+    __bf16 ah[2][64][kKP], al[2][64][kKP];
+    __bf16 bh[2][kNC][kKP], bl[2][kNC][kKP];
+};
+__device__ __forceinline__ void stage_a(GemmLdsB& L, int i, int k, float2 v) {
+    __bf16 h, l;
+    split_bf16(v.x, h, l);
+    L.ah[0][i][k] = h;
+    L.al[0][i][k] = l;
+    split_bf16(v.y, h, l);
+    L.ah[1][i][k] = h;
+    L.al[1][i][k] = l;
+}
+__device__ __forceinline__ void stage_b(GemmLdsB& L, int k, int j, float2 v) {
+    __bf16 h, l;
+    split_bf16(v.x, h, l);
+    L.bh[0][j][k] = h;
+    L.bl[0][j][k] = l;
+    split_bf16(v.y, h, l);
+    L.bh[1][j][k] = h;
+    L.bl[1][j][k] = l;
+}
+typedef short short8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ bf16x8 neg_bf16x8(bf16x8 x) {
+    short8 b = __builtin_bit_cast(short8, x);
+    b ^= (short)0x8000;
+    return __builtin_bit_cast(bf16x8, b);
+}
+
+// Software-pipelined chunks: a thread fetches its 8 A and 8 B entries of the
+// next chunk into registers before the current chunk's products, and stores
+// them to LDS afterwards (the launch has 256 threads: 64 x 32 / 256 = 8 each).
+constexpr int kGemmThreads = 256;
+constexpr int kPerThread = 64 * kKC / kGemmThreads;
+static_assert(kKC * kNC == 64 * kKC, "A and B chunks have the same size");
+struct ChunkRegs {
+    float2 a[kPerThread], b[kPerThread];
+};
+// pass A layout: A rows from the row table (i = idx % 64), B rows gathered by b_ids
 template <typename PC>
-__device__ __forceinline__ void load_chunk_rows(const void* const* rt_a, const int* rt_lda, const int* rt_lrow,
-                                                const int32_t* __restrict__ b_ids, int K, int j0, int s, int k0,
-                                                const float2* __restrict__ B, GemmLds& L) {
-    for (int idx = threadIdx.x; idx < 64 * kKC; idx += blockDim.x) {
+__device__ __forceinline__ void fetch_rows(ChunkRegs& cr, const void* const* rt_a, const int* rt_lda,
+                                           const int* rt_lrow, const int32_t* __restrict__ b_ids, int K, int j0,
+                                           int s, int k0, const float2* __restrict__ B) {
+#pragma unroll
+    for (int e = 0; e < kPerThread; ++e) {
+        const int idx = threadIdx.x + e * kGemmThreads;
         const int i = idx % 64, k = idx / 64;
         float2 v = make_float2(0.f, 0.f);
         const PC* A = (const PC*)rt_a[i];
         if (A && k0 + k < K) v = ld_c(A, (size_t)(k0 + k) * rt_lda[i] + rt_lrow[i]);
-        L.ar[i][k] = v.x;
-        L.ai[i][k] = v.y;
-    }
-    for (int idx = threadIdx.x; idx < kKC * kNC; idx += blockDim.x) {
-        const int j = idx % kNC, k = idx / kNC;
-        float2 v = make_float2(0.f, 0.f);
-        if (k0 + k < K && j0 + j < s) v = B[(size_t)b_ids[k0 + k] * s + j0 + j];
-        L.br[k][j] = v.x;
-        L.bi[k][j] = v.y;
+        cr.a[e] = v;
+        const int j = idx % kNC, kb = idx / kNC;
+        float2 w = make_float2(0.f, 0.f);
+        if (k0 + kb < K && j0 + j < s) w = B[(size_t)b_ids[k0 + kb] * s + j0 + j];
+        cr.b[e] = w;
     }
 }
-
-// Loads rows [row0, row0 + 64) x one packed K chunk: column k of the chunk is
-// local column kt_lk[k] of the transposed A at kt_a[k] (A[i][k] = a[i * lda + lk])
-// and B row kt_brow[k] (dense).
+template <typename Lds>
+__device__ __forceinline__ void store_rows(Lds& L, const ChunkRegs& cr) {
+#pragma unroll
+    for (int e = 0; e < kPerThread; ++e) {
+        const int idx = threadIdx.x + e * kGemmThreads;
+        stage_a(L, idx % 64, idx / 64, cr.a[e]);
+        stage_b(L, idx / kNC, idx % kNC, cr.b[e]);
+    }
+}
+// pass B layout: A columns from the per-k table (i = idx / kKC), dense B rows
 template <typename PC>
-__device__ __forceinline__ void load_chunk_packed(const void* const* kt_a, const int* kt_lda, const int* kt_lk,
-                                                  const size_t* kt_brow, int row0, int m, int j0, int s,
-                                                  const float2* __restrict__ B, GemmLds& L) {
-    for (int idx = threadIdx.x; idx < 64 * kKC; idx += blockDim.x) {
-        const int i = idx / kKC, k = idx % kKC;  // k fastest: consecutive local columns are contiguous
+__device__ __forceinline__ void fetch_packed(ChunkRegs& cr, const void* const* kt_a, const int* kt_lda,
+                                             const int* kt_lk, const size_t* kt_brow, int row0, int m, int j0,
+                                             int s, const float2* __restrict__ B) {
+#pragma unroll
+    for (int e = 0; e < kPerThread; ++e) {
+        const int idx = threadIdx.x + e * kGemmThreads;
+        const int i = idx / kKC, k = idx % kKC;
         float2 v = make_float2(0.f, 0.f);
         const PC* A = (const PC*)kt_a[k];
         if (A && row0 + i < m) v = ld_c(A, (size_t)(row0 + i) * kt_lda[k] + kt_lk[k]);
-        L.ar[i][k] = v.x;
-        L.ai[i][k] = v.y;
+        cr.a[e] = v;
+        const int j = idx % kNC, kb = idx / kNC;
+        float2 w = make_float2(0.f, 0.f);
+        if (kt_a[kb] && j0 + j < s) w = B[kt_brow[kb] + j0 + j];
+        cr.b[e] = w;
     }
-    for (int idx = threadIdx.x; idx < kKC * kNC; idx += blockDim.x) {
-        const int j = idx % kNC, k = idx / kNC;
-        float2 v = make_float2(0.f, 0.f);
-        if (kt_a[k] && j0 + j < s) v = B[kt_brow[k] + j0 + j];
-        L.br[k][j] = v.x;
-        L.bi[k][j] = v.y;
+}
+template <typename Lds>
+__device__ __forceinline__ void store_packed(Lds& L, const ChunkRegs& cr) {
+#pragma unroll
+    for (int e = 0; e < kPerThread; ++e) {
+        const int idx = threadIdx.x + e * kGemmThreads;
+        stage_a(L, idx / kKC, idx % kKC, cr.a[e]);
+        stage_b(L, idx / kNC, idx % kNC, cr.b[e]);
     }
 }
 
@@ -181,12 +234,13 @@ __global__ __launch_bounds__(256) void k_mrhs_gemm(const MrhsProb* __restrict__ 
                                                    const MrhsSeg* __restrict__ segs,
                                                    const MrhsRowSeg* __restrict__ rsegs, int nchunks,
                                                    const float2* __restrict__ B, int s, float2* __restrict__ Out) {
-    __shared__ GemmLds L;
+    using Lds = typename std::conditional<kMode == kMrhsMfmaBf16x3, GemmLdsB, GemmLds>::type;
+    __shared__ Lds L;
     __shared__ const void* rt_a[64];
     __shared__ int rt_lda[64], rt_lrow[64];
-    __shared__ const void* kt_a[kKC];
-    __shared__ int kt_lda[kKC], kt_lk[kKC];
-    __shared__ size_t kt_brow[kKC];
+    __shared__ const void* kt_a[2][kKC];  // per-k tables of the current and the next chunk
+    __shared__ int kt_lda[2][kKC], kt_lk[2][kKC];
+    __shared__ size_t kt_brow[2][kKC];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const XcdItems it(ntiles_total * nchunks);
     for (int w = it.first; w < it.end; w += it.stride) {
@@ -250,32 +304,24 @@ __global__ __launch_bounds__(256) void k_mrhs_gemm(const MrhsProb* __restrict__ 
                             }
                     }
                 } else {
-                    // v_mfma_f32_16x16x32_bf16: lane supplies A[l%16][8*(l/16) + 0..7], B[8*(l/16) + 0..7][l%16]
+                    // v_mfma_f32_16x16x32_bf16: lane supplies A[l%16][8*(l/16) + 0..7], B[8*(l/16) + 0..7][l%16],
+                    // read as 16-byte vectors of the staged hi / lo planes
                     const int li = lane & 15, lk = (lane >> 4) * 8;
                     bf16x8 arh[2], arl[2], aih[2], ail[2], naih[2], nail[2], brh[2], brl[2], bih[2], bil[2];
 #pragma unroll
                     for (int t = 0; t < 2; ++t) {
                         const int row = 32 * (wv >> 1) + 16 * t + li;
                         const int col = 32 * (wv & 1) + 16 * t + li;
-#pragma unroll
-                        for (int e = 0; e < 8; ++e) {
-                            __bf16 h, l;
-                            split_bf16(L.ar[row][lk + e], h, l);
-                            arh[t][e] = h;
-                            arl[t][e] = l;
-                            split_bf16(L.ai[row][lk + e], h, l);
-                            aih[t][e] = h;
-                            ail[t][e] = l;
-                            split_bf16(-L.ai[row][lk + e], h, l);
-                            naih[t][e] = h;
-                            nail[t][e] = l;
-                            split_bf16(L.br[lk + e][col], h, l);
-                            brh[t][e] = h;
-                            brl[t][e] = l;
-                            split_bf16(L.bi[lk + e][col], h, l);
-                            bih[t][e] = h;
-                            bil[t][e] = l;
-                        }
+                        arh[t] = *(const bf16x8*)&L.ah[0][row][lk];
+                        arl[t] = *(const bf16x8*)&L.al[0][row][lk];
+                        aih[t] = *(const bf16x8*)&L.ah[1][row][lk];
+                        ail[t] = *(const bf16x8*)&L.al[1][row][lk];
+                        naih[t] = neg_bf16x8(aih[t]);
+                        nail[t] = neg_bf16x8(ail[t]);
+                        brh[t] = *(const bf16x8*)&L.bh[0][col][lk];
+                        brl[t] = *(const bf16x8*)&L.bl[0][col][lk];
+                        bih[t] = *(const bf16x8*)&L.bh[1][col][lk];
+                        bil[t] = *(const bf16x8*)&L.bl[1][col][lk];
                     }
 #pragma unroll
                     for (int rt = 0; rt < 2; ++rt)
@@ -313,17 +359,21 @@ __global__ __launch_bounds__(256) void k_mrhs_gemm(const MrhsProb* __restrict__ 
                 rt_lda[tid] = lda;
                 rt_lrow[tid] = lrow;
             }
+            __syncthreads();
+            ChunkRegs cr;
+            fetch_rows<PC>(cr, rt_a, rt_lda, rt_lrow, G.b_ids, G.k, j0, s, 0, B);
             for (int k0 = 0; k0 < G.k; k0 += kKC) {
+                __syncthreads();  // the previous chunk's products are done with L
+                store_rows(L, cr);
                 __syncthreads();
-                load_chunk_rows<PC>(rt_a, rt_lda, rt_lrow, G.b_ids, G.k, j0, s, k0, B, L);
-                __syncthreads();
+                if (k0 + kKC < G.k) fetch_rows<PC>(cr, rt_a, rt_lda, rt_lrow, G.b_ids, G.k, j0, s, k0 + kKC, B);
                 compute();
             }
         } else {
             // K packed across segments (segment g covers [g.kbase, g.kbase + g.k)):
-            // each 32-wide chunk gets a per-k table of (A column, B row).
-            for (int k0 = 0; k0 < G.k; k0 += kKC) {
-                __syncthreads();
+            // each 32-wide chunk gets a per-k table of (A column, B row), built
+            // one chunk ahead in the other table buffer.
+            auto build_table = [&](int k0, int t) {
                 if (tid < kKC) {
                     const int kk = k0 + tid;
                     const void* pa = nullptr;
@@ -341,14 +391,28 @@ __global__ __launch_bounds__(256) void k_mrhs_gemm(const MrhsProb* __restrict__ 
                         lk = kk - g.kbase;
                         brow = g.b_off + (size_t)lk * s;
                     }
-                    kt_a[tid] = pa;
-                    kt_lda[tid] = lda;
-                    kt_lk[tid] = lk;
-                    kt_brow[tid] = brow;
+                    kt_a[t][tid] = pa;
+                    kt_lda[t][tid] = lda;
+                    kt_lk[t][tid] = lk;
+                    kt_brow[t][tid] = brow;
                 }
+            };
+            __syncthreads();  // the previous item may still read the tables
+            build_table(0, 0);
+            __syncthreads();
+            ChunkRegs cr;
+            fetch_packed<PC>(cr, kt_a[0], kt_lda[0], kt_lk[0], kt_brow[0], row0, m, j0, s, B);
+            int c = 0;
+            for (int k0 = 0; k0 < G.k; k0 += kKC, ++c) {
+                __syncthreads();  // the previous chunk's products are done with L
+                store_packed(L, cr);
+                const bool next = k0 + kKC < G.k;
+                if (next) build_table(k0 + kKC, (c + 1) & 1);
                 __syncthreads();
-                load_chunk_packed<PC>(kt_a, kt_lda, kt_lk, kt_brow, row0, m, j0, s, B, L);
-                __syncthreads();
+                if (next) {
+                    const int t = (c + 1) & 1;
+                    fetch_packed<PC>(cr, kt_a[t], kt_lda[t], kt_lk[t], kt_brow[t], row0, m, j0, s, B);
+                }
                 compute();
             }
         }
