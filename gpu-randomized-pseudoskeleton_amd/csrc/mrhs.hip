@@ -136,6 +136,16 @@ __device__ __forceinline__ bf16x8 neg_bf16x8(bf16x8 x) {
     return __builtin_bit_cast(bf16x8, b);
 }
 
+// FP32 staging (VALU and FP32 MFMA): real and imaginary planes.
+__device__ __forceinline__ void stage_a(GemmLds& L, int i, int k, float2 v) {
+    L.ar[i][k] = v.x;
+    L.ai[i][k] = v.y;
+}
+__device__ __forceinline__ void stage_b(GemmLds& L, int k, int j, float2 v) {
+    L.br[k][j] = v.x;
+    L.bi[k][j] = v.y;
+}
+
 // Software-pipelined chunks: a thread fetches its 8 A and 8 B entries of the
 // next chunk into registers before the current chunk's products, and stores
 // them to LDS afterwards (the launch has 256 threads: 64 x 32 / 256 = 8 each).
