@@ -237,8 +237,13 @@ struct XcdItems {
 // kRowSeg: pass A layout (a problem's rows are stacked contributions with
 // their own A, one gathered K range); otherwise pass B layout (one row range,
 // K concatenated over segments with their own A and dense B).
+// Three workgroups per CU (at most 168 VGPRs + AGPRs): the BF16x3 form would
+// otherwise take ~200 and run two; the few registers spilled (<= 112 bytes)
+// cost less than the lost overlap (9.4 ms against 10.5 ms for 256
+// right-hand sides on the 10242-vertex sphere). Four per CU gains nothing for
+// the FP32 forms.
 template <typename PC, int kMode, bool kRowSeg>
-__global__ __launch_bounds__(256) void k_mrhs_gemm(const MrhsProb* __restrict__ probs, int ntiles_total,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_mrhs_gemm(const MrhsProb* __restrict__ probs, int ntiles_total,
                                                    const int32_t* __restrict__ tile_prob,
                                                    const int32_t* __restrict__ tile_row0,
                                                    const MrhsSeg* __restrict__ segs,
@@ -317,36 +322,37 @@ __global__ __launch_bounds__(256) void k_mrhs_gemm(const MrhsProb* __restrict__ 
                     // v_mfma_f32_16x16x32_bf16: lane supplies A[l%16][8*(l/16) + 0..7], B[8*(l/16) + 0..7][l%16],
                     // read as 16-byte vectors of the staged hi / lo planes
                     const int li = lane & 15, lk = (lane >> 4) * 8;
-                    bf16x8 arh[2], arl[2], aih[2], ail[2], naih[2], nail[2], brh[2], brl[2], bih[2], bil[2];
+                    // B operands of both column tiles, then one row tile's A operands at a time
+                    bf16x8 brh[2], brl[2], bih[2], bil[2];
 #pragma unroll
                     for (int t = 0; t < 2; ++t) {
-                        const int row = 32 * (wv >> 1) + 16 * t + li;
                         const int col = 32 * (wv & 1) + 16 * t + li;
-                        arh[t] = *(const bf16x8*)&L.ah[0][row][lk];
-                        arl[t] = *(const bf16x8*)&L.al[0][row][lk];
-                        aih[t] = *(const bf16x8*)&L.ah[1][row][lk];
-                        ail[t] = *(const bf16x8*)&L.al[1][row][lk];
-                        naih[t] = neg_bf16x8(aih[t]);
-                        nail[t] = neg_bf16x8(ail[t]);
                         brh[t] = *(const bf16x8*)&L.bh[0][col][lk];
                         brl[t] = *(const bf16x8*)&L.bl[0][col][lk];
                         bih[t] = *(const bf16x8*)&L.bh[1][col][lk];
                         bil[t] = *(const bf16x8*)&L.bl[1][col][lk];
                     }
 #pragma unroll
-                    for (int rt = 0; rt < 2; ++rt)
+                    for (int rt = 0; rt < 2; ++rt) {
+                        const int row = 32 * (wv >> 1) + 16 * rt + li;
+                        const bf16x8 arh = *(const bf16x8*)&L.ah[0][row][lk];
+                        const bf16x8 arl = *(const bf16x8*)&L.al[0][row][lk];
+                        const bf16x8 aih = *(const bf16x8*)&L.ah[1][row][lk];
+                        const bf16x8 ail = *(const bf16x8*)&L.al[1][row][lk];
+                        const bf16x8 naih = neg_bf16x8(aih), nail = neg_bf16x8(ail);
 #pragma unroll
                         for (int ct = 0; ct < 2; ++ct) {
-#define FARCUR_BF3(acc, AH, AL, BH, BL)                                                     \
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AH[rt], BH[ct], acc, 0, 0, 0);            \
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AH[rt], BL[ct], acc, 0, 0, 0);            \
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL[rt], BH[ct], acc, 0, 0, 0);
+#define FARCUR_BF3(acc, AH, AL, BH, BL)                                                \
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AH, BH[ct], acc, 0, 0, 0);           \
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AH, BL[ct], acc, 0, 0, 0);           \
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL, BH[ct], acc, 0, 0, 0);
                             FARCUR_BF3(accr[rt][ct], arh, arl, brh, brl)
                             FARCUR_BF3(accr[rt][ct], naih, nail, bih, bil)
                             FARCUR_BF3(acci[rt][ct], arh, arl, bih, bil)
                             FARCUR_BF3(acci[rt][ct], aih, ail, brh, brl)
 #undef FARCUR_BF3
                         }
+                    }
                 }
         };
         if constexpr (kRowSeg) {
