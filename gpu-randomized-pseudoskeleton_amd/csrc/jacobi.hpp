@@ -341,8 +341,9 @@ __device__ __forceinline__ void jacobi_pinv_t(c128* A, c128* V, int r, double at
 // The map from logical to physical columns and the trailing norms are double
 // buffered, so each step needs one barrier. On exit:
 //   R[i][j] = A[phys[j] * ld + i] for i < j, R[j][j] = s_alpha[j],
-//   v_k (rows k..r-1) in Wg[k * r + i] (Wg global, r x r; not stored when
-//   Wg is NULL), tau_k in s_tau[k],
+//   v_k (rows k..r-1) in A[phys[k] * ld + i] for i >= k, and in
+//   Wg[k * r + i] (Wg global, r x r; not stored when Wg is NULL),
+//   tau_k in s_tau[k],
 //   phys = s_phys (the final map).
 // Returns the buffer (0 or 1) of s_phys / s_norm that holds the final map.
 template <int kT>
@@ -361,8 +362,11 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, dou
     }
     __syncthreads();
     int cur = 0;
+    c128 v0_prev{0.0, 0.0};  // v_{k-1}[k-1]: written to its column's row k-1 in step k,
+                             // which does not touch the columns of earlier pivots
     for (int k = 0; k < r; ++k) {
         const int* phys = s_phys + cur * r;
+        if (tid == 0 && k > 0) A[(size_t)phys[k - 1] * ld + k - 1] = v0_prev;
         int* phys_next = s_phys + (cur ^ 1) * r;
         const double* norm = s_norm + cur * r;
         double* norm_next = s_norm + (cur ^ 1) * r;
@@ -403,6 +407,7 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, dou
             alpha = c128{-xn, 0.0};
         }
         const c128 v0{x0.re - alpha.re, x0.im - alpha.im};
+        v0_prev = v0;
         const double den = best + xn * ax0;
         const double tau = den > 0 ? rcp_nr(den) : 0.0;
         for (int j = tid; j < r; j += nt) phys_next[j] = j == k ? ck : (j == p ? co : phys[j]);
@@ -441,8 +446,9 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, dou
         __syncthreads();
         cur ^= 1;
     }
-    // reflector bodies (rows below k) to global
     const int* phys = s_phys + cur * r;
+    if (tid == 0 && r > 0) A[(size_t)phys[r - 1] * ld + r - 1] = v0_prev;
+    // reflector bodies (rows below k) to global
     if (Wg) {
         for (int idx = tid; idx < r * r; idx += nt) {
             const int i = idx % r, k = idx / r;
@@ -484,37 +490,62 @@ __device__ __forceinline__ void apply_q_left(c128* Z, int r, int ld, const c128*
     }
 }
 
-// Z[:, j] = Q e_j for j < ncols (Q = H_0 H_1 ... H_{r-1}, the reflectors of
-// qrcp_lds in Wg, tau_k in s_tau). H_i leaves e_j unchanged for i > j, so
-// column j needs H_j, ..., H_0 only. A team owns a column: no workgroup barrier.
+// Q_k = H_0 H_1 ... H_{k-1} [I_k; 0] in place of the reflectors of qrcp_lds
+// (v_j in A's physical column phys[j], rows j..r-1; tau_j in s_tau): on exit
+// column phys[j] holds Q e_j, all r rows. Step i applies H_i to the columns
+// j > i, which hold H_{i+1} ... H_{k-1} e_j (zero above row i + 1); column
+// i + 1 is set to H_{i+1} e_{i+1} in the same step by its own team, after
+// step i + 1 has read v_{i+1} for the last time. One barrier per step; a team
+// per column (the team width must leave a team for each of the k - 1 columns
+// to run the steps without a serial loop, but any width is correct).
 template <int kT>
-__device__ __forceinline__ void form_q_cols(c128* Z, int r, int ld, int ncols, const c128* __restrict__ Wg,
-                                            const double* s_tau) {
+__device__ __forceinline__ void form_q_inplace(c128* A, int r, int ld, int k, const int* phys, const double* s_tau) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nteams = nt / kT, team = tid / kT, tl = tid - team * kT;
-    for (int j = team; j < ncols; j += nteams) {
-        c128* z = Z + (size_t)j * ld;
-        for (int i = tl; i < r; i += kT) z[i] = c128{i == j ? 1.0 : 0.0, 0.0};
-        for (int k = j; k >= 0; --k) {
-            const c128* v = Wg + (size_t)k * r;
-            const double tau = s_tau[k];
+    // z = H_j e_j in column phys[j]: rows < j zero, row j 1 - tau |v0|^2,
+    // rows > j -tau conj(v0) v_j[l] (v0 = v_j[j]); each lane rewrites its own rows
+    auto init_col = [&](int j) {
+        c128* z = A + (size_t)phys[j] * ld;
+        const double tj = s_tau[j];
+        const c128 v0 = z[j];
+        __builtin_amdgcn_wave_barrier();  // every lane of the team has v0 before row j is rewritten
+        const double fr = -tj * v0.re, fi = tj * v0.im;  // -tau conj(v0)
+        for (int l = tl; l < r; l += kT) {
+            c128 x{0.0, 0.0};
+            if (l == j) {
+                x = c128{1.0 - tj * (v0.re * v0.re + v0.im * v0.im), 0.0};
+            } else if (l > j) {
+                const c128 vl = z[l];
+                x = c128{fr * vl.re - fi * vl.im, fr * vl.im + fi * vl.re};
+            }
+            z[l] = x;
+        }
+    };
+    for (int i = k - 2; i >= 0; --i) {
+        const c128* v = A + (size_t)phys[i] * ld;
+        const double tau = s_tau[i];
+        for (int j = i + 1 + team; j < k; j += nteams) {
+            if (j == i + 1) init_col(j);
+            c128* z = A + (size_t)phys[j] * ld;
             double sr = 0, si = 0;
-            for (int i = k + tl; i < r; i += kT) {  // s = v^H z
-                const c128 vi = v[i], zi = z[i];
-                sr += vi.re * zi.re + vi.im * zi.im;
-                si += vi.re * zi.im - vi.im * zi.re;
+            for (int l = i + tl; l < r; l += kT) {  // s = v^H z
+                const c128 vl = v[l], zl = z[l];
+                sr += vl.re * zl.re + vl.im * zl.im;
+                si += vl.re * zl.im - vl.im * zl.re;
             }
             sr = team_sum_t<kT>(sr) * tau;
             si = team_sum_t<kT>(si) * tau;
-            for (int i = k + tl; i < r; i += kT) {
-                const c128 vi = v[i];
-                c128 zi = z[i];
-                zi.re -= vi.re * sr - vi.im * si;
-                zi.im -= vi.re * si + vi.im * sr;
-                z[i] = zi;
+            for (int l = i + tl; l < r; l += kT) {
+                const c128 vl = v[l];
+                c128 zl = z[l];
+                zl.re -= vl.re * sr - vl.im * si;
+                zl.im -= vl.re * si + vl.im * sr;
+                z[l] = zl;
             }
         }
+        __syncthreads();
     }
+    if (k > 0 && team == 0) init_col(0);
 }
 
 // Truncated QR pseudo-inverse of W (in A, LDS; destroyed): W P = Q R with
@@ -522,16 +553,15 @@ __device__ __forceinline__ void form_q_cols(c128* Z, int r, int ld, int ncols, c
 // and U = P [R_11^{-1} Q_k^H; 0]. For k = r this is W^{-1}; with truncation
 // C U R interpolates the pivot columns (an interpolative decomposition)
 // instead of projecting on W's leading singular space. Uses U (ldu) as
-// scratch for R_11 and Wg for the reflectors. sigma[j] = |R_jj| (logical
-// order), smax = |R_00|, rank_eps counts |R_jj| >= label_eps |R_00|.
+// scratch for R_11; Q_k is formed in place of the reflectors. sigma[j] =
+// |R_jj| (logical order), smax = |R_00|, rank_eps counts |R_jj| >= label_eps |R_00|.
 template <int kT>
 __device__ __forceinline__ void pinv_qr_trunc(c128* A, int r, int ld, double atol, double rtol,
                                               c128* __restrict__ U, int ldu, double* __restrict__ sigma,
-                                              c128* __restrict__ Wg, int* s_flag, int* s_phys, double* s_norm,
-                                              double* s_tau, c128* s_alpha, int* kept_out, double* smax_out,
-                                              double label_eps, int* rank_eps_out) {
+                                              int* s_phys, double* s_norm, double* s_tau, c128* s_alpha,
+                                              int* kept_out, double* smax_out, double label_eps, int* rank_eps_out) {
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int cur = qrcp_lds<kT>(A, r, ld, s_phys, s_norm, s_tau, s_alpha, Wg);
+    const int cur = qrcp_lds<kT>(A, r, ld, s_phys, s_norm, s_tau, s_alpha, nullptr);
     const int* phys = s_phys + cur * r;
     __syncthreads();
     const double smax = sqrt(s_alpha[0].re * s_alpha[0].re + s_alpha[0].im * s_alpha[0].im);
@@ -556,23 +586,23 @@ __device__ __forceinline__ void pinv_qr_trunc(c128* A, int r, int ld, double ato
         const double ia = rcp_nr(a.re * a.re + a.im * a.im);
         s_alpha[j] = c128{a.re * ia, -a.im * ia};
     }
-    __syncthreads();  // A's R is consumed; the teams overwrite its columns
-    // A[:, :k] = Q_k, with teams narrow enough to give every column its own
-    // (the reflector chain of a column is serial)
+    __syncthreads();  // A's R is consumed; Q_k replaces the reflectors
+    // A[phys[j] * ld + :] = Q e_j, with teams narrow enough to give every
+    // column its own
     if constexpr (kT >= 8) {
         if (blockDim.x / kT < k && blockDim.x / (kT / 2) >= k)
-            form_q_cols<kT / 2>(A, r, ld, k, Wg, s_tau);
+            form_q_inplace<kT / 2>(A, r, ld, k, phys, s_tau);
         else if (blockDim.x / kT < k)
-            form_q_cols<kT / 4>(A, r, ld, k, Wg, s_tau);
+            form_q_inplace<kT / 4>(A, r, ld, k, phys, s_tau);
         else
-            form_q_cols<kT>(A, r, ld, k, Wg, s_tau);
+            form_q_inplace<kT>(A, r, ld, k, phys, s_tau);
     } else if constexpr (kT >= 4) {
         if (blockDim.x / kT < k)
-            form_q_cols<kT / 2>(A, r, ld, k, Wg, s_tau);
+            form_q_inplace<kT / 2>(A, r, ld, k, phys, s_tau);
         else
-            form_q_cols<kT>(A, r, ld, k, Wg, s_tau);
+            form_q_inplace<kT>(A, r, ld, k, phys, s_tau);
     } else {
-        form_q_cols<kT>(A, r, ld, k, Wg, s_tau);
+        form_q_inplace<kT>(A, r, ld, k, phys, s_tau);
     }
     __syncthreads();
     // row c of Q_k: R_11 u = conj(Q[c, 0:k])^T by back substitution, in place;
@@ -585,7 +615,7 @@ __device__ __forceinline__ void pinv_qr_trunc(c128* A, int r, int ld, double ato
             double xr = 0, xi = 0;
             if (live) {
                 for (int s2 = l + 1 + sub; s2 < k; s2 += 4) {
-                    const c128 rv = U[(size_t)s2 * ldu + l], u = A[(size_t)s2 * ld + c];
+                    const c128 rv = U[(size_t)s2 * ldu + l], u = A[(size_t)phys[s2] * ld + c];
                     xr += rv.re * u.re - rv.im * u.im;
                     xi += rv.re * u.im + rv.im * u.re;
                 }
@@ -593,17 +623,17 @@ __device__ __forceinline__ void pinv_qr_trunc(c128* A, int r, int ld, double ato
             xr = team_sum_t<4>(xr);
             xi = team_sum_t<4>(xi);
             if (live && sub == 0) {
-                const c128 q = A[(size_t)l * ld + c];
+                const c128 q = A[(size_t)phys[l] * ld + c];
                 const double yr = q.re - xr, yi = -q.im - xi;
                 const c128 ia = s_alpha[l];
-                A[(size_t)l * ld + c] = c128{yr * ia.re - yi * ia.im, yr * ia.im + yi * ia.re};
+                A[(size_t)phys[l] * ld + c] = c128{yr * ia.re - yi * ia.im, yr * ia.im + yi * ia.re};
             }
         }
     }
     __syncthreads();
     for (int idx = tid; idx < r * r; idx += nt) {
         const int l = idx % r, c = idx / r;
-        U[(size_t)c * ldu + phys[l]] = l < k ? A[(size_t)l * ld + c] : c128{0.0, 0.0};
+        U[(size_t)c * ldu + phys[l]] = l < k ? A[(size_t)phys[l] * ld + c] : c128{0.0, 0.0};
     }
     // every thread computed k, smax and reps alike (the outputs are per thread)
     *kept_out = k;

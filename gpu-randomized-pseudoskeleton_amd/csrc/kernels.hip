@@ -388,8 +388,8 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
         if (qr_core && one_buf && (J.h > 0 || qr_core == 2)) {
             // learned rounds (qr_core 2: every core, the autotuner's probes):
             // the truncated QR pseudo-inverse (no SVD)
-            pinv_qr_trunc<kT>(L.a, r, ld, atol, rtol, J.U, r, J.sigma, J.W, L.flag, L.phys, L.norm, L.tau, L.alpha,
-                              &kept, &smax, label_eps, &reps);
+            pinv_qr_trunc<kT>(L.a, r, ld, atol, rtol, J.U, r, J.sigma, L.phys, L.norm, L.tau, L.alpha, &kept, &smax,
+                              label_eps, &reps);
             sweeps = 0;
         } else {
             pinv_qr_lds<kT, kE>(L.a, L.b, r, ld, atol, rtol, vfree, wload, J.U, r, J.sigma, J.W, L.flag, L.w,
@@ -598,17 +598,26 @@ __device__ __forceinline__ double pnorm_mfma(const PC* __restrict__ C, int m, in
         dbl4 acc[2];
 #pragma unroll
         for (int c = 0; c < 2; ++c) acc[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+        // the C entries of the next k-step are loaded while this one's products
+        // run; a lane's real index kk = k0 + 4 u + lk has the parity of lk, so
+        // it needs one part of each entry
+        using Real = typename ComplexTraits<PC>::real;
+        const Real* Cp = (const Real*)C + (lk & 1);
+        Real xn[4];
+        auto fetch = [&](int k0) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int l = (k0 + 4 * u + lk) >> 1;
+                xn[u] = 0;
+                if (row >= 0 && l < r) xn[u] = Cp[2 * ((size_t)l * m + row)];
+            }
+        };
+        fetch(0);
         for (int k0 = 0; k0 < 2 * r; k0 += 16) {
             double a[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int kk = k0 + 4 * u + lk, l = kk >> 1;
-                a[u] = 0.0;
-                if (row >= 0 && l < r) {
-                    const c128 x = to_c128(C[(size_t)l * m + row]);
-                    a[u] = kk & 1 ? x.im : x.re;
-                }
-            }
+            for (int u = 0; u < 4; ++u) a[u] = (double)xn[u];
+            if (k0 + 16 < 2 * r) fetch(k0 + 16);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int kk = k0 + 4 * u + lk, l = kk >> 1, p = kk & 1;
@@ -652,12 +661,15 @@ __device__ __forceinline__ void zpass_mfma(int g, int n, int r, const double* sY
         const int jj0 = t << 4, j0 = jj0 >> 1;
         const int jl = li >> 1, q = li & 1, j = j0 + jl;
         dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+        // the next block of R is loaded while this one's products run
+        const int sj = lane >> 3, sl = lane & 7;
+        PC xn{};
+        if (slot && j0 + sj < n && sl < r) xn = R[(size_t)(j0 + sj) * r + sl];
         for (int l0 = 0; l0 < r; l0 += 8) {
             if (slot) {
-                const int sj = lane >> 3, sl = lane & 7;
-                c128 x{0.0, 0.0};
-                if (j0 + sj < n && l0 + sl < r) x = to_c128(R[(size_t)(j0 + sj) * r + l0 + sl]);
-                slot[sj * 9 + sl] = x;
+                slot[sj * 9 + sl] = to_c128(xn);
+                xn = PC{};
+                if (j0 + sj < n && l0 + 8 + sl < r) xn = R[(size_t)(j0 + sj) * r + l0 + 8 + sl];
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
