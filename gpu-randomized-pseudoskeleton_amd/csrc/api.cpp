@@ -214,7 +214,9 @@ struct KTimer {
 
 // Launch shapes, chosen by the autotuner (farcur_autotune) or defaults.
 struct Tuning {
-    int pinv_threads[kPinvClasses] = {64, 128, 128, 256, 256, 256, 512, 512, 512, 1024, 512};
+    // the autotuner's choice on an MI355X for the truncated QR core of the
+    // learned rounds (narrow teams: small cores take few threads)
+    int pinv_threads[kPinvClasses] = {64, 64, 128, 128, 128, 256, 256, 256, 512, 512, 512};
     // fill workgroups per launch, capped at the launch's tile count: one per
     // tile balances tiles of unequal cost best (sphere: fill 5.40 ms with a
     // persistent grid of 4096, 5.10 with one workgroup per tile)
@@ -585,7 +587,7 @@ struct RoundPlan {
     int f32 = 0;
     bool pivot = false;
     int fill_grid = 0;
-    int pinv_threads[kPinvClasses] = {64, 128, 128, 256, 256, 256, 512, 512, 512, 1024, 512};
+    int pinv_threads[kPinvClasses] = {64, 64, 128, 128, 128, 256, 256, 256, 512, 512, 512};
     int resid_threads = 256;
     double atol = 1e-10, rtol = 0, label_eps = 1e-4;
     int one_buf = 0;  // every job gathers its core from C (one-buffer pinv allowed)
