@@ -373,7 +373,11 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, dou
         // Pivot: the largest trailing norm, first logical column on ties, by a
         // wave-wide argmax (one lane per column, r <= 64) that every wave
         // computes alike. (value, column) is totally ordered (a non-finite or
-        // negative norm counts as -1), so all lanes agree on the result.
+        // negative norm counts as -1), so the result does not depend on the
+        // order of the combinations. The r - k candidates sit in the first
+        // lanes: DPP exchanges within rows of 16 (the rounds that reach past
+        // r - k lanes are skipped), lane swaps across rows only for more than
+        // 16 candidates, and lane 0's result is broadcast.
         double best;
         int p;
         {
@@ -381,17 +385,22 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, dou
             double v = j < r ? norm[phys[j]] : -1.0;
             if (!(v >= 0.0)) v = -1.0;
             int pj = j;
-#pragma unroll
-            for (int off = kWave / 2; off >= 1; off >>= 1) {
-                const double ov = __shfl_xor(v, off, kWave);
-                const int oj = __shfl_xor(pj, off, kWave);
+            auto take = [&](double ov, int oj) {
                 if (ov > v || (ov == v && oj < pj)) {
                     v = ov;
                     pj = oj;
                 }
-            }
-            best = v;
-            p = pj;
+            };
+            const int nc = r - k;  // candidates (uniform)
+            if (nc > 1) take(dpp_move<0xB1>(v), __builtin_amdgcn_mov_dpp(pj, 0xB1, 0xF, 0xF, true));    // lane ^ 1
+            if (nc > 2) take(dpp_move<0x4E>(v), __builtin_amdgcn_mov_dpp(pj, 0x4E, 0xF, 0xF, true));    // lane ^ 2
+            if (nc > 4) take(dpp_move<0x141>(v), __builtin_amdgcn_mov_dpp(pj, 0x141, 0xF, 0xF, true));  // 7 - i in 8
+            if (nc > 8) take(dpp_move<0x140>(v), __builtin_amdgcn_mov_dpp(pj, 0x140, 0xF, 0xF, true));  // 15 - i in 16
+            if (nc > 16) take(__shfl_xor(v, 16, kWave), __shfl_xor(pj, 16, kWave));
+            if (nc > 32) take(__shfl_xor(v, 32, kWave), __shfl_xor(pj, 32, kWave));
+            best = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 0),
+                                    __builtin_amdgcn_readlane(__double2loint(v), 0));
+            p = __builtin_amdgcn_readlane(pj, 0);
         }
         const int ck = phys[p], co = phys[k];
         const c128 x0 = A[(size_t)ck * ld + k];
