@@ -129,6 +129,25 @@ __device__ __forceinline__ void stage_b(GemmLdsB& L, int k, int j, float2 v) {
     L.bh[1][j][k] = h;
     L.bl[1][j][k] = l;
 }
+// Eight consecutive k of one A row or B column at once: four 16-byte writes
+// (hi / lo of the real and imaginary planes) instead of 32 two-byte ones.
+__device__ __forceinline__ void stage8(__bf16 (*h)[64][kKP], __bf16 (*l)[64][kKP], int i, int k0, const float2* v) {
+    bf16x8 hr, lr, hi, li;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        __bf16 a, b;
+        split_bf16(v[e].x, a, b);
+        hr[e] = a;
+        lr[e] = b;
+        split_bf16(v[e].y, a, b);
+        hi[e] = a;
+        li[e] = b;
+    }
+    *(bf16x8*)&h[0][i][k0] = hr;
+    *(bf16x8*)&l[0][i][k0] = lr;
+    *(bf16x8*)&h[1][i][k0] = hi;
+    *(bf16x8*)&l[1][i][k0] = li;
+}
 typedef short short8 __attribute__((ext_vector_type(8)));
 __device__ __forceinline__ bf16x8 neg_bf16x8(bf16x8 x) {
     short8 b = __builtin_bit_cast(short8, x);
@@ -149,9 +168,12 @@ __device__ __forceinline__ void stage_b(GemmLds& L, int k, int j, float2 v) {
 // Software-pipelined chunks: a thread fetches its 8 A and 8 B entries of the
 // next chunk into registers before the current chunk's products, and stores
 // them to LDS afterwards (the launch has 256 threads: 64 x 32 / 256 = 8 each).
+// A thread's 8 entries of a B column (and, in pass A, of an A row) are 8
+// consecutive k, lanes running over the rows / columns (coalesced loads).
 constexpr int kGemmThreads = 256;
 constexpr int kPerThread = 64 * kKC / kGemmThreads;
 static_assert(kKC * kNC == 64 * kKC, "A and B chunks have the same size");
+static_assert(kGemmThreads / 64 * kPerThread == kKC && kNC == 64, "four groups of eight k per row or column");
 struct ChunkRegs {
     float2 a[kPerThread], b[kPerThread];
 };
@@ -160,27 +182,31 @@ template <typename PC>
 __device__ __forceinline__ void fetch_rows(ChunkRegs& cr, const void* const* rt_a, const int* rt_lda,
                                            const int* rt_lrow, const int32_t* __restrict__ b_ids, int K, int j0,
                                            int s, int k0, const float2* __restrict__ B) {
+    const int i = threadIdx.x % 64, kg = (threadIdx.x / 64) * kPerThread;
+    const PC* A = (const PC*)rt_a[i];
 #pragma unroll
     for (int e = 0; e < kPerThread; ++e) {
-        const int idx = threadIdx.x + e * kGemmThreads;
-        const int i = idx % 64, k = idx / 64;
+        const int k = kg + e;
         float2 v = make_float2(0.f, 0.f);
-        const PC* A = (const PC*)rt_a[i];
         if (A && k0 + k < K) v = ld_c(A, (size_t)(k0 + k) * rt_lda[i] + rt_lrow[i]);
         cr.a[e] = v;
-        const int j = idx % kNC, kb = idx / kNC;
         float2 w = make_float2(0.f, 0.f);
-        if (k0 + kb < K && j0 + j < s) w = B[(size_t)b_ids[k0 + kb] * s + j0 + j];
+        if (k0 + k < K && j0 + i < s) w = B[(size_t)b_ids[k0 + k] * s + j0 + i];
         cr.b[e] = w;
     }
 }
 template <typename Lds>
 __device__ __forceinline__ void store_rows(Lds& L, const ChunkRegs& cr) {
+    const int i = threadIdx.x % 64, kg = (threadIdx.x / 64) * kPerThread;
+    if constexpr (std::is_same<Lds, GemmLdsB>::value) {
+        stage8(L.ah, L.al, i, kg, cr.a);
+        stage8(L.bh, L.bl, i, kg, cr.b);
+    } else {
 #pragma unroll
-    for (int e = 0; e < kPerThread; ++e) {
-        const int idx = threadIdx.x + e * kGemmThreads;
-        stage_a(L, idx % 64, idx / 64, cr.a[e]);
-        stage_b(L, idx / kNC, idx % kNC, cr.b[e]);
+        for (int e = 0; e < kPerThread; ++e) {
+            stage_a(L, i, kg + e, cr.a[e]);
+            stage_b(L, kg + e, i, cr.b[e]);
+        }
     }
 }
 // pass B layout: A columns from the per-k table (i = idx / kKC), dense B rows
@@ -196,7 +222,7 @@ __device__ __forceinline__ void fetch_packed(ChunkRegs& cr, const void* const* k
         const PC* A = (const PC*)kt_a[k];
         if (A && row0 + i < m) v = ld_c(A, (size_t)(row0 + i) * kt_lda[k] + kt_lk[k]);
         cr.a[e] = v;
-        const int j = idx % kNC, kb = idx / kNC;
+        const int j = threadIdx.x % kNC, kb = (threadIdx.x / kNC) * kPerThread + e;
         float2 w = make_float2(0.f, 0.f);
         if (kt_a[kb] && j0 + j < s) w = B[kt_brow[kb] + j0 + j];
         cr.b[e] = w;
@@ -204,11 +230,17 @@ __device__ __forceinline__ void fetch_packed(ChunkRegs& cr, const void* const* k
 }
 template <typename Lds>
 __device__ __forceinline__ void store_packed(Lds& L, const ChunkRegs& cr) {
+    const int j = threadIdx.x % kNC, kg = (threadIdx.x / kNC) * kPerThread;
 #pragma unroll
     for (int e = 0; e < kPerThread; ++e) {
         const int idx = threadIdx.x + e * kGemmThreads;
-        stage_a(L, idx / kKC, idx % kKC, cr.a[e]);
-        stage_b(L, idx / kNC, idx % kNC, cr.b[e]);
+        stage_a(L, idx / kKC, idx % kKC, cr.a[e]);  // A rows: lanes over k (row-major sources)
+    }
+    if constexpr (std::is_same<Lds, GemmLdsB>::value) {
+        stage8(L.bh, L.bl, j, kg, cr.b);
+    } else {
+#pragma unroll
+        for (int e = 0; e < kPerThread; ++e) stage_b(L, kg + e, j, cr.b[e]);
     }
 }
 
