@@ -1101,6 +1101,51 @@ int farcur_pinv(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, doubl
     return FARCUR_OK;
 }
 
+int farcur_pinv_qr(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, double rtol, farcur_c128* U,
+                   double* diag, int* kept) {
+    if (!ctx || !W || !U) return fail(FARCUR_EINVAL, "NULL argument");
+    if (r <= 0 || r > kPinvLdsOneBufMaxR)
+        return fail(FARCUR_EINVAL, "r = %d outside [1, %d] (the learned core's LDS form)", r, kPinvLdsOneBufMaxR);
+    if (!(atol >= 0) || !(rtol >= 1e-8))
+        return fail(FARCUR_EINVAL, "thresholds: atol >= 0 and rtol >= 1e-8 (the learned core's LDS form)");
+    TRY(ctx->select());
+    const size_t rr = (size_t)r * r * sizeof(c128);
+    TRY(ctx->d_W.ensure(rr));
+    TRY(ctx->d_U.ensure(rr));
+    TRY(ctx->d_sigma.ensure(r * sizeof(double)));
+    HIPTRY(hipMemcpyAsync(ctx->d_W.p, W, rr, hipMemcpyHostToDevice, ctx->stream));
+    // the core is gathered as in a learned round: W = C[rpos, :] with C = W (m = r)
+    struct Plan {
+        BlockJob job;
+        int32_t list[4];
+        JobOut out;
+        int32_t rpos[kPinvLdsOneBufMaxR];
+    } plan{};
+    for (int i = 0; i < r; ++i) plan.rpos[i] = i;
+    TRY(ctx->d_plan.ensure(sizeof(Plan)));
+    Plan* dp = ctx->d_plan.as<Plan>();
+    BlockJob& J = plan.job;
+    J.m = J.n = J.r = r;
+    J.C = ctx->d_W.p;
+    J.W = ctx->d_W.as<c128>();
+    J.U = ctx->d_U.as<c128>();
+    J.sigma = ctx->d_sigma.as<double>();
+    J.rpos = dp->rpos;
+    HIPTRY(hipMemcpyAsync(ctx->d_plan.p, &plan, sizeof(Plan), hipMemcpyHostToDevice, ctx->stream));
+    const int cls = pinv_class(r);
+    launch_pinv(&dp->job, dp->list, 1, atol, rtol, 0.0, kPinvClassMax[cls], ctx->tune.pinv_threads[cls], 0, 1, 0, 2,
+                &dp->out, ctx->stream);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(U, ctx->d_U.p, rr, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> d(r);
+    HIPTRY(hipMemcpyAsync(d.data(), ctx->d_sigma.p, r * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(hipMemcpyAsync(&plan, ctx->d_plan.p, sizeof(Plan), hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    if (diag) std::copy(d.begin(), d.end(), diag);
+    if (kept) *kept = plan.out.kept;
+    return FARCUR_OK;
+}
+
 int farcur_pinv_noalloc(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, double rtol, farcur_c128* U,
                         double* sigma, int* kept, void* stream) {
     if (!ctx || !W || !U) return fail(FARCUR_EINVAL, "NULL argument");

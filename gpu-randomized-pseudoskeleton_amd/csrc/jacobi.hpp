@@ -371,13 +371,13 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, dou
         const double* norm = s_norm + cur * r;
         double* norm_next = s_norm + (cur ^ 1) * r;
         // Pivot: the largest trailing norm, first logical column on ties, by a
-        // wave-wide argmax (one lane per column, r <= 64) that every wave
-        // computes alike. (value, column) is totally ordered (a non-finite or
-        // negative norm counts as -1), so the result does not depend on the
-        // order of the combinations. The r - k candidates sit in the first
-        // lanes: DPP exchanges within rows of 16 (the rounds that reach past
-        // r - k lanes are skipped), lane swaps across rows only for more than
-        // 16 candidates, and lane 0's result is broadcast.
+        // wave-wide argmax that every wave computes alike. (value, column) is
+        // totally ordered (a non-finite or negative norm counts as -1), so the
+        // result does not depend on the order of the combinations. Lane l
+        // takes candidates k + l and, past 64 candidates, k + l + 64 (r <= 128);
+        // then DPP exchanges within rows of 16 (the rounds that reach past the
+        // occupied lanes are skipped), lane swaps across rows only for more
+        // than 16 occupied lanes, and lane 0's result is broadcast.
         double best;
         int p;
         {
@@ -391,7 +391,13 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, dou
                     pj = oj;
                 }
             };
-            const int nc = r - k;  // candidates (uniform)
+            if (r - k > kWave) {
+                const int j2 = j + kWave;
+                double v2 = j2 < r ? norm[phys[j2]] : -1.0;
+                if (!(v2 >= 0.0)) v2 = -1.0;
+                take(v2, j2);
+            }
+            const int nc = r - k;  // candidates (uniform); occupied lanes min(nc, 64)
             if (nc > 1) take(dpp_move<0xB1>(v), __builtin_amdgcn_mov_dpp(pj, 0xB1, 0xF, 0xF, true));    // lane ^ 1
             if (nc > 2) take(dpp_move<0x4E>(v), __builtin_amdgcn_mov_dpp(pj, 0x4E, 0xF, 0xF, true));    // lane ^ 2
             if (nc > 4) take(dpp_move<0x141>(v), __builtin_amdgcn_mov_dpp(pj, 0x141, 0xF, 0xF, true));  // 7 - i in 8

@@ -117,6 +117,7 @@ constexpr int kPinvLdsMaxR = 64;
 // The one-buffer LDS form (every core gathered from C, relative threshold
 // >= 1e-8: the learned FP32 rounds) fits cores up to this size in 160 KB.
 constexpr int kPinvLdsOneBufMaxR = 96;
+static_assert(kPinvLdsOneBufMaxR <= 128, "the QR pivot search takes at most two columns per lane");
 // r classes (0,16], (16,20], ..., (48,64], (64,96], > 96: the LDS of a launch is
 // sized for its class maximum, so finer classes let small cores pack more
 // workgroups per CU. The (64,96] class runs in LDS only in the one-buffer form,

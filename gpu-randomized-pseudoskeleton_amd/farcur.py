@@ -150,6 +150,7 @@ def _setup(L: ctypes.CDLL) -> None:
     L.farcur_fill.argtypes = [vp, vp, ci, vp, ci, ci, vp]
     L.farcur_pinv.argtypes = [vp, vp, ci, cd, cd, vp, vp, vp]
     L.farcur_pinv_noalloc.argtypes = [vp, vp, ci, cd, cd, vp, vp, vp, vp]
+    L.farcur_pinv_qr.argtypes = [vp, vp, ci, cd, cd, vp, vp, vp]
     L.farcur_cur_residual.argtypes = [vp, vp, ci, ci, vp, vp, ci, vp, vp, vp, vp]
     L.farcur_sample.argtypes = [vp, cu64, ci, ci, ci, ci, ci, vp]
     L.farcur_vtest.argtypes = [cu64, ci, ci, vp]
@@ -453,6 +454,17 @@ class Context:
         kept = ci()
         check(lib().farcur_pinv(self._h, _p(W), r, atol, rtol, _p(U), _p(s), ctypes.byref(kept)))
         return U, s, kept.value
+
+    def pinv_qr(self, W, atol=1e-10, rtol=1e-6):
+        """The learned core's truncated QR pseudo-inverse: (U, |R_jj| in pivot
+        order, k)."""
+        W = _cmat(W)
+        r = W.shape[0]
+        U = np.empty((r, r), dtype=np.complex128, order="F")
+        d = np.empty(r)
+        kept = ci()
+        check(lib().farcur_pinv_qr(self._h, _p(W), r, atol, rtol, _p(U), _p(d), ctypes.byref(kept)))
+        return U, d, kept.value
 
     def pinv_noalloc(self, W_dev: int, r: int, atol: float, rtol: float, U_dev: int, sigma_dev: int = 0,
                      kept_dev: int = 0, stream: int = 0) -> None:

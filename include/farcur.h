@@ -167,6 +167,18 @@ int farcur_fill(farcur_ctx* ctx, const int* rows, int nr, const int* cols, int n
 int farcur_pinv(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, double rtol, farcur_c128* U,
                 double* sigma, int* kept);
 
+/* The truncated QR pseudo-inverse that the learned rounds apply to their cores
+ * (FARCUR_CORE_QR): W P = Q R with column pivoting (the pivot at step j is
+ * the column with the largest trailing norm, the first on ties), k = the
+ * leading |R_jj| >= max(atol, rtol |R_00|), U = P [R_11^{-1} Q_k^H; 0] (for
+ * k = r, W^{-1}). No counterpart in the reference, which inverts every core
+ * by SVD (farcur_pinv): for callers and tests of the learned core. Same
+ * kernel, launch shapes and LDS form as the learned rounds: r <= 96 and
+ * rtol >= 1e-8, else FARCUR_EINVAL. diag (r, may be NULL) receives |R_jj| in
+ * pivot order, *kept (may be NULL) the k. */
+int farcur_pinv_qr(farcur_ctx* ctx, const farcur_c128* W, int r, double atol, double rtol, farcur_c128* U,
+                   double* diag, int* kept);
+
 /* farcur_pinv without any allocation, for callers that cannot allocate device
  * memory (code under HIP graph capture, serving loops that forbid hipMalloc).
  * No counterpart in the reference. W, U (r x r), sigma (r, descending; may be

@@ -154,6 +154,41 @@ def test_learned_gain_and_accuracy(farcur, gctx, ellipsoid):
           f"miss rate {learned.misses() / learned.nblocks:.3f}, max err {np.max(errs):.2e}")
 
 
+@pytest.mark.parametrize("r,k", [(12, 12), (24, 1), (30, 2), (32, 20), (40, 33), (64, 50), (96, 60)])
+def test_pinv_qr_matches_oracle_on_planted_ranks(farcur, gctx, oracle, r, k):
+    # Cores of exact rank k plus 1e-13 noise: the relative threshold 1e-8
+    # keeps k columns. Covers the LDS classes from 16 to 96 (QR steps in LDS),
+    # k = 1 and 2 (the in-place Q formation's first steps), k = r (U = W^-1).
+    rng = np.random.default_rng(7000 + 100 * r + k)
+    A = rng.standard_normal((r, k)) + 1j * rng.standard_normal((r, k))
+    B = rng.standard_normal((k, r)) + 1j * rng.standard_normal((k, r))
+    W = A @ B + 1e-13 * (rng.standard_normal((r, r)) + 1j * rng.standard_normal((r, r)))
+    U, d, kept = gctx.pinv_qr(W, atol=0.0, rtol=1e-8)
+    Uo, do, ko, perm = oracle.pinv_qr_trunc(W, atol=0.0, rtol=1e-8)
+    assert kept == ko == k
+    # the kept diagonal and pivots must agree; past k the noise columns'
+    # order is not fixed by the data
+    np.testing.assert_allclose(d[:k], do[:k], rtol=1e-10)
+    assert np.all(d[k:] < 1e-8 * d[0])
+    assert np.linalg.norm(U - Uo) <= 1e-9 * np.linalg.norm(Uo)
+    # interpolation on the pivot columns: (U W)[:, c] = e_c
+    UW = U @ W
+    for c in perm[:k]:
+        e = np.zeros(r)
+        e[c] = 1.0
+        assert np.linalg.norm(UW[:, c] - e) < 1e-8
+    if k == r:
+        np.testing.assert_allclose(U, np.linalg.inv(W), rtol=0, atol=1e-10 * np.abs(np.linalg.inv(W)).max())
+
+
+def test_pinv_qr_rejects_what_its_form_cannot_take(farcur, gctx):
+    W = np.eye(8, dtype=np.complex128)
+    with pytest.raises(farcur.FarcurError):
+        gctx.pinv_qr(W, atol=0.0, rtol=0.0)  # relative thresholds below 1e-8
+    with pytest.raises(farcur.FarcurError):
+        gctx.pinv_qr(np.eye(97, dtype=np.complex128))  # above the largest LDS class
+
+
 def test_learned_core_is_the_truncated_qr(farcur, gctx, ellipsoid, oracle):
     # Learned rounds invert their cores by truncated QR with column pivoting
     # (FARCUR_CORE_QR): each sampled block's U against the oracle's restatement
