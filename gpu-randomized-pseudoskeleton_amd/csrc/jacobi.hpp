@@ -431,15 +431,25 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, dou
             s_tau[k] = tau;
             if (Wg) Wg[(size_t)k * r + k] = v0;
         }
+        // a lane's first row i0 = k + tl is row k itself for tl = 0, whose
+        // element of v is v0 (the panel still holds x_k there): peeled, so
+        // the loops over the lane's further rows read v plainly
+        const c128* v = A + (size_t)ck * ld;
+        const int i0 = k + tl;
+        const c128 vf = tl == 0 ? v0 : (i0 < r ? v[i0] : c128{0.0, 0.0});
         for (int j = k + 1 + team; j < r; j += nteams) {
             int c = co;
             if (j != p) c = phys[j];
             c128* a = A + (size_t)c * ld;
-            const c128* v = A + (size_t)ck * ld;
-            // w = v^H a (row k uses v0)
+            // w = v^H a
             double wr = 0, wi = 0;
-            for (int i = k + tl; i < r; i += kT) {
-                const c128 vi = i == k ? v0 : v[i];
+            if (i0 < r) {
+                const c128 ai = a[i0];
+                wr += vf.re * ai.re + vf.im * ai.im;
+                wi += vf.re * ai.im - vf.im * ai.re;
+            }
+            for (int i = i0 + kT; i < r; i += kT) {
+                const c128 vi = v[i];
                 const c128 ai = a[i];
                 wr += vi.re * ai.re + vi.im * ai.im;
                 wi += vi.re * ai.im - vi.im * ai.re;
@@ -447,13 +457,20 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, dou
             wr = team_sum_t<kT>(wr) * tau;
             wi = team_sum_t<kT>(wi) * tau;
             double nn = 0;
-            for (int i = k + tl; i < r; i += kT) {
-                const c128 vi = i == k ? v0 : v[i];
+            if (i0 < r) {
+                c128 ai = a[i0];
+                ai.re -= vf.re * wr - vf.im * wi;
+                ai.im -= vf.re * wi + vf.im * wr;
+                a[i0] = ai;
+                if (tl > 0) nn += ai.re * ai.re + ai.im * ai.im;  // row k is R's, not trailing
+            }
+            for (int i = i0 + kT; i < r; i += kT) {
+                const c128 vi = v[i];
                 c128 ai = a[i];
                 ai.re -= vi.re * wr - vi.im * wi;
                 ai.im -= vi.re * wi + vi.im * wr;
                 a[i] = ai;
-                if (i > k) nn += ai.re * ai.re + ai.im * ai.im;
+                nn += ai.re * ai.re + ai.im * ai.im;
             }
             nn = team_sum_t<kT>(nn);
             if (tl == 0) norm_next[c] = nn;
