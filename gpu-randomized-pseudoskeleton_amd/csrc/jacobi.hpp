@@ -338,7 +338,8 @@ __device__ __forceinline__ void jacobi_pinv_t(c128* A, c128* V, int r, double at
 // column k and reflects it onto alpha_k e_k with
 //   H_k = I - tau_k v_k v_k^H,  v_k = x - alpha_k e_k,  alpha_k = -e^{i arg x_k} |x|,
 //   tau_k = 2 / |v_k|^2 = 1 / (|x|^2 + |x| |x_k|)   (real; H_k Hermitian and unitary).
-// The map from logical to physical columns and the trailing norms are double
+// The map from logical to physical columns and the trailing norms (indexed
+// by logical position, so the pivot search reads them directly) are double
 // buffered, so each step needs one barrier. On exit:
 //   R[i][j] = A[phys[j] * ld + i] for i < j, R[j][j] = s_alpha[j],
 //   v_k (rows k..r-1) in A[phys[k] * ld + i] for i >= k, and in
@@ -382,7 +383,7 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, dou
         int p;
         {
             const int j = k + (tid & (kWave - 1));
-            double v = j < r ? norm[phys[j]] : -1.0;
+            double v = j < r ? norm[j] : -1.0;
             if (!(v >= 0.0)) v = -1.0;
             int pj = j;
             auto take = [&](double ov, int oj) {
@@ -393,7 +394,7 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, dou
             };
             if (r - k > kWave) {
                 const int j2 = j + kWave;
-                double v2 = j2 < r ? norm[phys[j2]] : -1.0;
+                double v2 = j2 < r ? norm[j2] : -1.0;
                 if (!(v2 >= 0.0)) v2 = -1.0;
                 take(v2, j2);
             }
@@ -473,7 +474,7 @@ __device__ __forceinline__ int qrcp_lds(c128* A, int r, int ld, int* s_phys, dou
                 nn += ai.re * ai.re + ai.im * ai.im;
             }
             nn = team_sum_t<kT>(nn);
-            if (tl == 0) norm_next[c] = nn;
+            if (tl == 0) norm_next[j] = nn;
         }
         __syncthreads();
         cur ^= 1;
