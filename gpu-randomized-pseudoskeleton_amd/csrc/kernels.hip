@@ -107,9 +107,112 @@ __global__ void k_sample(const BlockJob* __restrict__ jobs, int njobs, uint64_t 
     for (uint32_t k = 0; k < got; ++k) out[k] = ids[pos[k]];
 }
 
+// The same scans with a wave per (job, kind), for launches of few jobs. A
+// scan takes candidate t of N when q_t = (u_t (N - t)) >> 32 is below the
+// number still needed (select_step). The lanes hold 64 consecutive candidates
+// and compute their q_t in parallel; the decisions, a chain through the count
+// taken so far, run over the chunk's candidate lanes in order on scalar
+// registers (one scalar unit per CU: many such waves contend for it, so a
+// launch of many jobs keeps the thread-per-scan kernel). The held-out
+// decisions do not depend on the sample rows, so a chunk's held-out chain runs
+// before its sample-row chain, as in the interleaved scan.
+__device__ __forceinline__ uint64_t select_chunk(uint32_t q, uint64_t cands, uint32_t& got, uint32_t need) {
+    uint64_t taken = 0;
+    while (cands && got < need) {
+        const int l = __builtin_ctzll(cands);
+        cands &= cands - 1;
+        if ((uint32_t)__builtin_amdgcn_readlane((int)q, l) < need - got) {
+            taken |= 1ull << l;
+            ++got;
+        }
+    }
+    return taken;
+}
+// set bits of mask below this lane
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+__global__ __launch_bounds__(256) void k_sample_wave(const BlockJob* __restrict__ jobs, int njobs, uint64_t seed) {
+    // the wave's index, made visibly uniform: the job's fields and the chains
+    // then live in scalar registers
+    const int wid = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
+    const int lane = threadIdx.x & (kWave - 1);
+    const int j = wid >> 1, kind = wid & 1;
+    if (j >= njobs) return;
+    const BlockJob& J = jobs[j];
+    if (J.sample_mode == 1) return;
+    if (kind == 1 && J.sample_mode == 2) return;
+    const uint32_t b = J.b, round = J.round;
+    if (kind == 1 && J.sample_mode == 3) {
+        const uint32_t m = J.m, h = J.h, r = J.r, N = m - h;
+        uint32_t goth = 0, got = 0, cand0 = 0;
+        for (uint32_t base = 0; base < m && (goth < h || got < r); base += kWave) {
+            const uint32_t t = base + lane;
+            const bool in = t < m;
+            const uint64_t valid = __ballot(in);
+            uint64_t held = 0;
+            if (goth < h) {
+                uint32_t q = 0;
+                if (in) q = (uint32_t)(((uint64_t)stream_word(seed, b, round, kStreamHeldout, t) * (m - t)) >> 32);
+                const uint32_t g0 = goth;
+                held = select_chunk(q, valid, goth, h);
+                if ((held >> lane) & 1) {
+                    const uint32_t k = g0 + lane_rank(held);
+                    J.hpos[k] = t;
+                    J.hs[k] = J.rows[t];
+                }
+            }
+            const uint64_t cands = valid & ~held;
+            if (got < r) {
+                const uint32_t ci = cand0 + lane_rank(cands);  // this position's candidate index
+                uint32_t q = 0;
+                if ((cands >> lane) & 1)
+                    q = (uint32_t)(((uint64_t)stream_word(seed, b, round, kStreamRows, ci) * (N - ci)) >> 32);
+                const uint32_t g0 = got;
+                const uint64_t taken = select_chunk(q, cands, got, r);
+                if ((taken >> lane) & 1) {
+                    const uint32_t k = g0 + lane_rank(taken);
+                    J.rpos[k] = t;
+                    J.rs[k] = J.rows[t];
+                }
+            }
+            cand0 += __popcll(cands);
+        }
+        return;
+    }
+    const uint32_t N = kind ? J.m : J.n;
+    const int32_t* ids = kind ? J.rows : J.cols;
+    int32_t* out = kind ? J.rs : J.cs;
+    int32_t* pos = kind == 1 ? J.rpos : nullptr;  // sample rows keep their positions when the job does
+    const uint32_t purpose = kind ? kStreamRows : kStreamCols;
+    const uint32_t need = J.r;
+    uint32_t got = 0;
+    for (uint32_t base = 0; base < N && got < need; base += kWave) {
+        const uint32_t t = base + lane;
+        uint32_t q = 0;
+        if (t < N) q = (uint32_t)(((uint64_t)stream_word(seed, b, round, purpose, t) * (N - t)) >> 32);
+        const uint32_t g0 = got;
+        const uint64_t taken = select_chunk(q, __ballot(t < N), got, need);
+        if ((taken >> lane) & 1) {
+            const uint32_t k = g0 + lane_rank(taken);
+            out[k] = ids[t];
+            if (pos) pos[k] = t;
+        }
+    }
+}
+
+// Launches of at most this many jobs (the learned mode's later rounds, single
+// blocks) take the wave-per-scan kernel.
+constexpr int kSampleWaveMaxJobs = 4096;
+
 void launch_sample(const BlockJob* jobs, int njobs, uint64_t seed, hipStream_t s) {
     if (njobs <= 0) return;
     const int threads = 256;
+    if (njobs <= kSampleWaveMaxJobs) {
+        const int grid = (2 * njobs * kWave + threads - 1) / threads;  // four (job, kind) waves per workgroup
+        hipLaunchKernelGGL(k_sample_wave, dim3(grid), dim3(threads), 0, s, jobs, njobs, seed);
+        return;
+    }
     const int grid = (2 * njobs + threads - 1) / threads;
     hipLaunchKernelGGL(k_sample, dim3(grid), dim3(threads), 0, s, jobs, njobs, seed);
 }
