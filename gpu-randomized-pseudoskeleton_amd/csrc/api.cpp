@@ -592,6 +592,7 @@ struct RoundPlan {
     double atol = 1e-10, rtol = 0, label_eps = 1e-4;
     int one_buf = 0;  // every job gathers its core from C (one-buffer pinv allowed)
     int qr_core = 0;  // learned jobs take the truncated QR pseudo-inverse
+    int n_noheld = 0, rmax_held = 0;  // as PackedRound: whether the residual launch has work
     uint64_t seed = 0;
     // device pointers
     const BlockJob* jobs = nullptr;
@@ -651,10 +652,14 @@ void run_round(const farcur_ctx* ctx, const RoundPlan& P, KTimer& kt, hipStream_
         (void)hipEventRecord(ctx->ev_pjoin, side);
         (void)hipStreamWaitEvent(s, ctx->ev_pjoin, 0);
     }
-    kt.begin(3, s, &a);
-    // held-out errors of the LDS-variant jobs came with their pinv launch
-    launch_residual(P.jobs, P.njobs, P.f32, P.resid_threads, pinv_lds_limit(P.one_buf, P.rtol), P.out, s);
-    kt.end(3, s, a);
+    // held-out errors of the LDS-variant jobs came with their pinv launch; the
+    // residual launch runs when some job is left to it
+    const int skip = pinv_lds_limit(P.one_buf, P.rtol);
+    if (P.n_noheld > 0 || P.rmax_held > skip) {
+        kt.begin(3, s, &a);
+        launch_residual(P.jobs, P.njobs, P.f32, P.resid_threads, skip, P.out, s);
+        kt.end(3, s, a);
+    }
 }
 
 // Host-side packing of a round: jobs, fill prefixes, pinv lists by r class,
@@ -667,6 +672,9 @@ struct PackedRound {
     int total_all = 0, total_wr = 0;
     int all_rpos = 1;  // every job gathers its core from C
     int nlist[kPinvClasses] = {};
+    // jobs of the plain loops (successive residual) and the largest core with
+    // held-out rows (held-out check in the pinv launch up to its LDS limit)
+    int n_noheld = 0, rmax_held = 0;
 };
 
 struct PackScratch {
@@ -677,9 +685,12 @@ struct PackScratch {
 // What the layout needs of a job: its sizes and whether it has row positions.
 struct JobDims {
     int32_t m, n, r, h, rpos;
+    int32_t learned;  // a learned job: without held-out rows it is stored exactly
 };
 
-inline JobDims dims_of(const BlockJob& J) { return JobDims{J.m, J.n, J.r, J.h, J.rpos != nullptr}; }
+inline JobDims dims_of(const BlockJob& J) {
+    return JobDims{J.m, J.n, J.r, J.h, J.rpos != nullptr, J.sample_mode == 3};
+}
 
 template <class Dims>
 void pack_layout(int nj, bool pivot, PackedRound& pk, PackScratch& sc, const Dims& dims) {
@@ -687,6 +698,7 @@ void pack_layout(int nj, bool pivot, PackedRound& pk, PackScratch& sc, const Dim
     sc.pwr.assign(nj + 1, 0);
     for (auto& l : sc.lists) l.clear();
     pk.all_rpos = 1;
+    pk.n_noheld = pk.rmax_held = 0;
     for (int j = 0; j < nj; ++j) {
         const JobDims J = dims(j);
         if (!J.rpos) pk.all_rpos = 0;
@@ -696,6 +708,10 @@ void pack_layout(int nj, bool pivot, PackedRound& pk, PackScratch& sc, const Dim
         sc.pa[j + 1] = sc.pa[j] + all;
         sc.pwr[j + 1] = sc.pwr[j] + (pivot ? fill_tiles(J.m, J.n, J.r, J.h, pw | kPanelR | kPanelH) : 0);
         sc.lists[pinv_class(J.r)].push_back(j);
+        if (J.h > 0)
+            pk.rmax_held = std::max(pk.rmax_held, J.r);
+        else if (!J.learned)
+            ++pk.n_noheld;
     }
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t o = 0;
@@ -761,6 +777,8 @@ RoundPlan plan_from(const PackedRound& pk, void* dev, int njobs) {
     }
     P.out = (JobOut*)(d + pk.off_out);
     P.one_buf = pk.all_rpos;
+    P.n_noheld = pk.n_noheld;
+    P.rmax_held = pk.rmax_held;
     return P;
 }
 
@@ -2027,7 +2045,7 @@ int batch_round(farcur_batch* B, const std::vector<int>& active, const farcur_op
         const BState& st = B->st[active[i]];
         const int h = B->learned ? st.h : 0;
         TRY(batch_alloc_round(B, st.r, st.m, st.n, h, bufs[i]));
-        dims[i] = JobDims{st.m, st.n, st.r, h, 1};  // batch jobs always carry row positions
+        dims[i] = JobDims{st.m, st.n, st.r, h, 1, B->learned ? 1 : 0};  // batch jobs always carry row positions
     }
     int nchunks = 1;
     if (!B->profile && B->pipeline_chunks > 1 && nj >= 64 * B->pipeline_chunks) nchunks = B->pipeline_chunks;

@@ -905,6 +905,7 @@ __global__ void k_residual(const BlockJob* __restrict__ jobs, int njobs, int ski
     for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
         const BlockJob& J = jobs[jb];
         if (J.h > 0 && J.r <= skip_heldout_rmax) continue;
+        if (J.h == 0 && J.sample_mode == 3) continue;  // learned, every row sampled: stored exactly
         if (J.h > 0) {
             // The 16 x 2r interpolation tile goes to s_t1 while it fits, else
             // to the W panel (2 r^2 doubles >= 16 (2r + 1) for r >= 17), which
