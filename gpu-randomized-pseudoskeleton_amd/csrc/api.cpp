@@ -2612,9 +2612,11 @@ extern "C" int farcur_batch_compress(farcur_batch* B, const farcur_opts* opts, i
     }
     B->arena.reset();
     hipStream_t st = ctx->stream;
-    launch_vtest(B->d_v.as<c128>(), B->d_vo.as<int32_t>(), B->nb, o.seed, st);
-    launch_ones(B->d_ya.as<c128>(), (size_t)B->row_off[B->nb], st);
-    HIPTRY(hipGetLastError());
+    if (mode != FARCUR_MODE_LEARNED) {  // the successive residual's vectors (learned rounds check held-out rows)
+        launch_vtest(B->d_v.as<c128>(), B->d_vo.as<int32_t>(), B->nb, o.seed, st);
+        launch_ones(B->d_ya.as<c128>(), (size_t)B->row_off[B->nb], st);
+        HIPTRY(hipGetLastError());
+    }
     if (mode == FARCUR_MODE_LEARNED) {
         // Sample counts from the rank model (device inference), then rounds of
         // every block whose held-out rows still miss tol.
