@@ -707,7 +707,7 @@ class Batch:
         check(lib().farcur_batch_profile(self._h, 1 if enable else 0))
 
     def kernel_times(self) -> dict:
-        cap = 16
+        cap = 32  # kernel classes and the host phases
         names = (ctypes.c_char_p * cap)()
         ms = np.zeros(cap)
         launches = np.zeros(cap, dtype=np.int32)
