@@ -189,6 +189,44 @@ def test_pinv_qr_rejects_what_its_form_cannot_take(farcur, gctx):
         gctx.pinv_qr(np.eye(97, dtype=np.complex128))  # above the largest LDS class
 
 
+def test_sampling_parity_in_large_and_small_rounds(farcur, gctx, oracle):
+    # Rounds of more than 4,096 jobs sample with a thread per scan, smaller
+    # rounds with a wave per scan (a scalar decision chain over 64 lanes):
+    # both against the oracle's scans, on the 31,166 far blocks of the
+    # 30,720-RWG sphere (without the network, nearly every block takes a
+    # second round; the few that need a third are a small round).
+    mesh = farcur.Mesh.icosphere(2.0, 2.0, 2.0, 5, 0.01, 1).build_rwg()
+    part = farcur.Partition(mesh, 0.5, 2.0)
+    gctx.set_mesh(mesh)
+    ro, ri, co, cix = part.far_blocks()
+    nb = len(ro) - 1
+    assert nb > 4096
+    bt = farcur.Batch(gctx, ro, ri, co, cix)
+    seed = 11
+    bt.compress(farcur.MODE_LEARNED, tol=1e-4, seed=seed, precision=farcur.F32, pinv_rtol=1e-6)  # no model: r0 = D/10
+    info = bt.info_arrays()
+    rounds = info["rounds"]
+    heldout = farcur.default_opts().heldout
+    # jobs of the round that drew each block's final samples
+    jobs_in_round = np.array([np.sum(rounds >= k) for k in range(1, rounds.max() + 1)])
+    final_jobs = jobs_in_round[rounds - 1]
+    large = np.flatnonzero(final_jobs > 4096)
+    small = np.flatnonzero(final_jobs <= 4096)
+    assert large.size and small.size, jobs_in_round
+    picks = list(large[:: max(1, large.size // 12)][:12]) + list(small[:: max(1, small.size // 12)][:12])
+    for b in picks:
+        b = int(b)
+        m, n, r = ro[b + 1] - ro[b], co[b + 1] - co[b], int(info["r"][b])
+        last = int(rounds[b]) - 1
+        h = max(0, min(min(max(heldout, -(-m // 32)), 4 * heldout), m - r))
+        hp, rsp = oracle.sample_heldout(seed, b, last, m, h, r)
+        rows, cols = ri[ro[b]:ro[b + 1]], cix[co[b]:co[b + 1]]
+        out = bt.block(b)
+        np.testing.assert_array_equal(out["rs"], rows[rsp])
+        np.testing.assert_array_equal(out["cs"], cols[oracle.sample(seed, b, last, 1, n, r)])
+    bt.close()
+
+
 def test_learned_core_is_the_truncated_qr(farcur, gctx, ellipsoid, oracle):
     # Learned rounds invert their cores by truncated QR with column pivoting
     # (FARCUR_CORE_QR): each sampled block's U against the oracle's restatement
