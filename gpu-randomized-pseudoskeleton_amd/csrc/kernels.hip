@@ -4,6 +4,8 @@
 // of a batch (persistent workgroups striding over the work).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "curmv.hpp"
 #include "element.hpp"
@@ -455,15 +457,18 @@ size_t pinv_lds_bytes(int rmax, bool one_buf) {
            (size_t)4 * 2 * rmax + 16;
 }
 
-template <int kT, int kE>
+// PC: the panels' precision (one kernel per precision, so that each carries
+// one held-out check: fewer registers live across the kernel).
+template <int kT, int kE, typename PC>
 // heldout: the workgroup also computes the held-out error of each job with
 // h > 0 (it has just written U; R, H and C are filled), reusing the core's
 // LDS buffer; the residual launch then skips those jobs. qr_core: jobs with
 // h > 0 in the one-buffer form take the truncated QR pseudo-inverse (2: every
 // job of the one-buffer form).
 __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __restrict__ list, int nlist, int rmax,
-                           double atol, double rtol, double label_eps, int f32, int one_buf, int heldout,
-                           int qr_core, JobOut* __restrict__ out) {
+                           double atol, double rtol, double label_eps, int one_buf, int heldout, int qr_core,
+                           JobOut* __restrict__ out) {
+    constexpr int f32 = std::is_same<PC, c64>::value ? 1 : 0;
     extern __shared__ __align__(16) char smem[];
     const PinvLds L(smem, rmax, one_buf != 0);
     for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
@@ -511,8 +516,7 @@ __global__ void k_pinv_lds(const BlockJob* __restrict__ jobs, const int32_t* __r
             // with the QR core, W U = Q_k Q_k^H: the sampled rows' interpolation
             // norms sum to k
             const int pk = qr_core && one_buf ? kept : -1;  // (jobs with h > 0 took the QR core)
-            const HeldErr e = f32 ? heldout_error<c64>(J, (double*)L.a, capd, L.norm, pk)
-                                  : heldout_error<c128>(J, (double*)L.a, capd, L.norm, pk);
+            const HeldErr e = heldout_error<PC>(J, (double*)L.a, capd, L.norm, pk);
             if (threadIdx.x == 0) {
                 out[jb].err = e.plain;
                 out[jb].err_est = e.est;
@@ -574,19 +578,30 @@ void pinv_lds_shape(int rmax, int threads, int* T, int* E) {
 
 namespace {
 
+template <int kT, int kE, typename PC>
+void launch_lds_pc(int grid, int threads, size_t lds, hipStream_t s, const BlockJob* jobs, const int32_t* list,
+                   int nlist, int rmax, double atol, double rtol, double label_eps, int one_buf, int heldout,
+                   int qr_core, JobOut* out) {
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute(
+            (const void*)k_pinv_lds<kT, kE, PC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+            (int)std::max(pinv_lds_bytes(kPinvLdsMaxR, false), pinv_lds_bytes(kPinvLdsOneBufMaxR, true)));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_pinv_lds<kT, kE, PC>), dim3(grid), dim3(threads), lds, s, jobs, list, nlist, rmax, atol,
+                       rtol, label_eps, one_buf, heldout, qr_core, out);
+}
 template <int kT, int kE>
 void launch_lds(int grid, int threads, size_t lds, hipStream_t s, const BlockJob* jobs, const int32_t* list,
                 int nlist, int rmax, double atol, double rtol, double label_eps, int f32, int one_buf, int heldout,
                 int qr_core, JobOut* out) {
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute(
-            (const void*)k_pinv_lds<kT, kE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-            (int)std::max(pinv_lds_bytes(kPinvLdsMaxR, false), pinv_lds_bytes(kPinvLdsOneBufMaxR, true)));
-        attr_set = true;
-    }
-    hipLaunchKernelGGL((k_pinv_lds<kT, kE>), dim3(grid), dim3(threads), lds, s, jobs, list, nlist, rmax, atol, rtol,
-                       label_eps, f32, one_buf, heldout, qr_core, out);
+    if (f32)
+        launch_lds_pc<kT, kE, c64>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, one_buf,
+                                   heldout, qr_core, out);
+    else
+        launch_lds_pc<kT, kE, c128>(grid, threads, lds, s, jobs, list, nlist, rmax, atol, rtol, label_eps, one_buf,
+                                    heldout, qr_core, out);
 }
 
 // E rounded up to a compiled register count (1, 2, 3, 4, 6, 8, 16); past 16 elements the team
